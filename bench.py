@@ -1,0 +1,182 @@
+"""PD-IPC step benchmark on one MI355X.
+
+Workload: a 256x256-vertex cloth (130,050 triangles) dropped onto a static
+noise-displaced icosphere (subdivision 6, 81,920 triangles) above a ground
+plane; dt = 1/100, 10 local/global passes x 15 Chebyshev sweeps per outer
+iteration, dHat = 1e-3, kappa = 1e3.  A "step" is one full PD-IPC time step
+(prediction, outer CCD iterations, velocity update) with all data resident in
+HBM.  The FP32-split step is the headline value; the plain FP64 step is timed
+from the same state, and the per-step error of FP32-split against FP64 is
+measured from identical states.
+
+Prints one JSON line.  --gpus above 1 exits: this build runs on one GPU.
+"""
+
+import argparse
+import json
+import os
+import subprocess
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.join(ROOT, "pd-ipc-armadillodemo_amd"))
+
+import pdipc  # noqa: E402
+import scenes  # noqa: E402
+
+METRIC = "PD-IPC steps/s, 256² cloth: FP32-split vs FP64 at 1e-5 err; adjoint/fwd ratio"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(msg, file=sys.stderr, flush=True)
+
+
+def timed_steps(sim, steps):
+    t0 = time.perf_counter()
+    outer = 0
+    for _ in range(steps):
+        sim.step(1)
+        outer += sim.counters()["outer_iters"]
+    return time.perf_counter() - t0, outer
+
+
+def cpu_baseline(scene, seconds_budget=20.0):
+    """The FP64 oracle (a single-threaded C++ restatement of the reference step)
+    on this host, on a bounded sample: full-size scene, as many steps as fit
+    the budget after the first."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # the checker, timed here as the CPU baseline
+
+    t0 = time.perf_counter()
+    ref = oracle.Oracle(scene)
+    init_s = time.perf_counter() - t0
+    n, t = 0, 0.0
+    while t < seconds_budget and n < 50:
+        t1 = time.perf_counter()
+        ref.step(1)
+        t += time.perf_counter() - t1
+        n += 1
+    return dict(value=n / t, unit="steps/s", cores=1, kind="port",
+                sample=f"{n} FP64 oracle steps of the same 256x256 scene from rest ({t:.1f} s; setup {init_s:.1f} s), "
+                       "single thread")
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--n", type=int, default=256, help="cloth vertices per side")
+    ap.add_argument("--subdiv", type=int, default=6)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-seconds", type=float, default=20.0)
+    ap.add_argument("--sell-lanes", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs the PD-IPC step on one GPU; --gpus {args.gpus} is not supported",
+              file=sys.stderr)
+        sys.exit(2)
+
+    scene = scenes.cloth_over_sphere(args.n, args.subdiv, ground=True, seed=0, patch_dim=args.n,
+                                     sphere_patch_dim=args.n)
+    t0 = time.perf_counter()
+    sims = {}
+    for name, prec in (("fp64", pdipc.FP64), ("fp32_split", pdipc.FP32_SPLIT)):
+        sims[name] = pdipc.Simulator.from_scene(scene, precision=prec, sell_lanes=args.sell_lanes)
+    log(f"setup {time.perf_counter() - t0:.1f} s; counters {sims['fp64'].counters()}")
+
+    # warm-up on the FP64 handle, then both precisions time K steps from that state
+    base = sims["fp64"]
+    for k in range(args.warmup):
+        base.step(1)
+    x_w, v_w = base.x(), base.v()
+    sims["fp32_split"].set_x(x_w)
+    sims["fp32_split"].set_v(v_w)
+    sims["fp32_split"].step(1)  # warm its kernels, then reset
+    sims["fp32_split"].set_x(x_w)
+    sims["fp32_split"].set_v(v_w)
+    log(f"warm-up done ({args.warmup} steps)")
+
+    res = {}
+    for name in ("fp64", "fp32_split"):
+        sim = sims[name]
+        sim.set_x(x_w)
+        sim.set_v(v_w)
+        secs, outer = timed_steps(sim, args.steps)
+        res[name] = dict(seconds=secs, outer=outer)
+        log(f"{name}: {args.steps} steps in {secs:.3f} s ({args.steps / secs:.2f} steps/s, {outer} outer iterations)")
+
+    # per-step error of FP32-split vs FP64 from identical states (teacher forced)
+    errs = []
+    a, b = sims["fp64"], sims["fp32_split"]
+    for k in range(5):
+        x, v = a.x(), a.v()
+        b.set_x(x)
+        b.set_v(v)
+        a.step(1)
+        b.step(1)
+        xa, xb = a.x()[: a.sim], b.x()[: b.sim]
+        errs.append(float(np.abs(xa - xb).max() / np.abs(xa).max()))
+    log(f"FP32-split per-step error vs FP64: {errs}")
+
+    # dominant kernel: the Chebyshev sweep (150 launches per outer iteration)
+    us64, bytes64 = a.time_kernel("sweep_fp64", 200)
+    us32, bytes32 = b.time_kernel("sweep_fp32", 200)
+    c = a.counters()
+    ach = bytes32 / (us32 * 1e-6) / 1e9
+    log(f"sweep fp64 {us64:.2f} us ({bytes64 / (us64 * 1e-6) / 1e9:.0f} GB/s), fp32 {us32:.2f} us ({ach:.0f} GB/s)")
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = cpu_baseline(scene, args.cpu_seconds)
+        log(f"cpu baseline {cpu}")
+
+    steps = args.steps
+    t32 = res["fp32_split"]["seconds"]
+    out = {
+        "metric": METRIC,
+        "value": steps / t32,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1000.0 * t32 / steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"cloth {args.n}x{args.n} ({c['cloth_points']} vertices) over icosphere subdiv "
+                        f"{args.subdiv} + ground, dt=1/100, 10 LG x 15 Chebyshev, dHat=1e-3, kappa=1e3",
+            "precision": "FP32-split (FP32 local step + sweeps on an FP64 residual; FP64 CCD/barrier)",
+            "fp64_steps_per_s": steps / res["fp64"]["seconds"],
+            "fp32_split_speedup_vs_fp64": res["fp64"]["seconds"] / t32,
+            "fp32_split_step_error_vs_fp64": max(errs),
+            "outer_iters_per_step": res["fp32_split"]["outer"] / steps,
+            "nnz_r2": c["nnz_r2"], "candidates": c["candidates"], "patches": c["patches"],
+            "parallelism": "single GPU",
+        },
+        "roofline": {
+            "kernel": "k_sweep32 (FP32 Chebyshev sweep, SELL M^2)",
+            "bound": "hbm",
+            "achieved": ach,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": ach / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_us": us32,
+            "bytes_per_launch": bytes32,
+            "fp64_sweep": {"avg_us": us64, "achieved": bytes64 / (us64 * 1e-6) / 1e9},
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

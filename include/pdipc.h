@@ -1,0 +1,164 @@
+/*
+ * pdipc.h -- C ABI of the MI355X PD-IPC step library (libpdipc.so).
+ *
+ * Drop-in for the reference simulator boundary of lanlei/PD-IPC-ArmadilloDemo:
+ * the entry points below replace, one for one, the calls a host makes on
+ * PD_IPC::PdIpcSimulator (Simulator/PD-IPC/PdIpcSimulator.h) and its
+ * configuration path (Simulator/BaseSimulator.cpp:4-65).  Plain pointers and
+ * sizes only; positions and velocities cross the boundary as AoS xyz doubles
+ * (the reference's devPoints layout, PdIpcSimulator.cpp:456-457) and are kept
+ * SoA on the device.
+ *
+ * Every function returns PDIPC_OK (0) or a negative status; the message of the
+ * last failure on the calling thread is available from pdipc_last_error().
+ * No function aborts the process.  A handle is not thread-safe; distinct
+ * handles may be used from distinct threads.
+ */
+#ifndef PDIPC_H
+#define PDIPC_H
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define PDIPC_OK 0
+#define PDIPC_E_ARG -1      /* invalid argument / call order */
+#define PDIPC_E_HIP -2      /* HIP runtime error (message has the HIP string) */
+#define PDIPC_E_CAPACITY -3 /* a fixed-capacity buffer could not be grown */
+#define PDIPC_E_BADMESH -4  /* degenerate or inconsistent mesh input */
+#define PDIPC_E_NAN -5      /* the step produced a non-finite state */
+#define PDIPC_E_NODEVICE -6 /* no HIP device available */
+
+#define PDIPC_FP64 0       /* every kernel in double precision */
+#define PDIPC_FP32_SPLIT 1 /* FP32 local step and Chebyshev sweeps on an FP64 residual; FP64 CCD / barrier */
+
+typedef struct pdipc_handle pdipc_handle;
+
+/* Step parameters.  Every constant the reference hard-codes is a field whose
+ * default (pdipc_default_params) is the reference value. */
+typedef struct pdipc_params {
+  double dt;           /* BaseSimulator _timeStep (config <timeStep>) */
+  double gravity;      /* along -y: force (0, gravity*m, 0), PdIpcSimulator.cpp:249-252 */
+  double dhat;         /* <dhat>, PdIpcSimulator.h:618-623 (ctor default 1e-3) */
+  double kappa;        /* <kappa>, PdIpcSimulator.h:624-628 (ctor default 1e4) */
+  int lg_iters;        /* local/global passes per outer iteration, PdIpcSimulator.cpp:1592 (10) */
+  int cheb_iters;      /* Chebyshev sweeps per solve, PdIpcSimulator.cpp:1597 (15) */
+  int max_outer;       /* outer (CCD) iterations, PdIpcSimulator.cpp:1685 (30) */
+  double residual_tol; /* |d|^2 break threshold, PdIpcSimulator.cpp:1776 (1e-5) */
+  double omega;        /* Jacobi relaxation, PdIpcGpuFunc.cu:949,1051 (2/3) */
+  double rho;          /* Chebyshev spectral estimate, PdIpcGpuFunc.cu:1318 (0.9992) */
+  double sigma_lo;     /* strain clamp, PdIpcGpuFunc.cu:307-316 (0.999) */
+  double sigma_hi;     /* (1.001) */
+  double ccd_filter;   /* CCD swept-box filter, PdIpcGpuFunc.cu:2656,2676 (5e-3) */
+  double toi_scale;    /* TOI safety factor, PdIpcGpuFunc.cu:2698-2699 (0.98) */
+  int ccd_iters;       /* ACCD iterations, PdIpcGpuFunc.cu:2497 (10) */
+  double ccd_step;     /* ACCD step factor (0.9) */
+  double ccd_gap;      /* ACCD stop fraction (0.1) */
+  double static_mass;  /* collision mass of static points, PdIpcSimulator.cpp:1494 (1e4) */
+  double inflation;    /* broad-phase box inflation; < 0 means 0.5*dhat (PdIpcSimulator.cpp:1154) */
+  int skip_static_pairs; /* 1: drop patch pairs whose patches are both static (they provably
+                            contribute nothing to the step); 0: the reference's full pair set */
+  int touch_policy;    /* 1: a pair already touching gives TOI 0 (conservative);
+                          0: the reference rule (NaN, pair ignored, PdIpcGpuFunc.cu:2499) */
+  int precision;       /* PDIPC_FP64 or PDIPC_FP32_SPLIT */
+  int sell_lanes;      /* lanes per row in the Chebyshev sweep (1, 2, 4, 8); 0 = default */
+  int use_graphs;      /* 1: replay the local/global loop from a HIP graph */
+} pdipc_params;
+
+typedef struct pdipc_counters {
+  int outer_iters;      /* outer iterations of the last step */
+  int lg_iters;         /* local/global passes of the last step */
+  int patch_pairs;      /* overlapping patch pairs of the last broad phase */
+  int candidates;       /* candidate primitive records (K) of the last broad phase */
+  int active_pairs;     /* records within dHat at the last constraint assembly */
+  double toi_min;       /* smallest TOI applied in the last step */
+  double residual;      /* |d|^2 of the last outer iteration */
+  long long nnz_r1;     /* off-diagonal nonzeros of R1 (plus one diagonal slot per row) */
+  long long nnz_r2;     /* nonzeros of the M^2 = R2 row operator */
+  long long n_2paths;   /* 2-path terms feeding the R2 operator update */
+  long long sell_entries; /* stored M^2 entries including SELL padding */
+  int sim_points, total_points, cloth_points, tets, patches, boundary_faces, boundary_edges;
+} pdipc_counters;
+
+/* --- lifecycle: PdIpcSimulator ctor + readExtraAttributeFromConfigFile (PdIpcSimulator.h:441-450,602-642) */
+void pdipc_default_params(pdipc_params* out);
+int pdipc_create(const pdipc_params* params, pdipc_handle** out);
+int pdipc_destroy(pdipc_handle* h);
+const char* pdipc_last_error(void);
+
+/* --- models: PdIpcSimulator::addModelFromConfigFile / addStaticModelFromConfigFile
+ * (PdIpcSimulator.h:462-600) with the mesh already loaded and transformed.
+ * verts: 3*nverts doubles (AoS); tris: 3*ntris; tets: 4*ntets; fixed: vertex ids
+ * (fixed_points.txt, PdIpcModel.h:56-77).  s_strain, s_bend_or_volume, s_pos are
+ * the <type> stiffness triple; patch_dim is <patchDim>. */
+int pdipc_add_cloth(pdipc_handle* h, const double* verts, int nverts, const int* tris, int ntris, double s_strain,
+                    double s_bend, double s_pos, double density, int patch_dim, const int* fixed, int nfixed);
+int pdipc_add_tet(pdipc_handle* h, const double* verts, int nverts, const int* tets, int ntets, double s_strain,
+                  double s_volume, double s_pos, double density, int patch_dim, const int* fixed, int nfixed);
+int pdipc_add_static(pdipc_handle* h, const double* verts, int nverts, const int* tris, int ntris, int patch_dim);
+
+/* --- PdIpcSimulator::initialization (PdIpcSimulator.cpp:34-268): host precompute
+ * then device upload.  pdipc_host_setup runs only the host precompute (no device
+ * needed); pdipc_init runs it if not done yet and uploads. */
+int pdipc_host_setup(pdipc_handle* h);
+int pdipc_init(pdipc_handle* h);
+
+/* --- PdIpcSimulator::run -> doTime -> advance (PdIpcSimulator.cpp:18,1538-1552,1683-1796) */
+int pdipc_step(pdipc_handle* h, int nsteps);
+int pdipc_set_precision(pdipc_handle* h, int precision);
+
+/* --- state: postRun (PdIpcSimulator.cpp:23-32) reads positions back; set_* restart
+ * from a given state.  x covers all points (simulated first, static after),
+ * v covers simulated points. */
+int pdipc_get_x(pdipc_handle* h, double* out, int npoints);
+int pdipc_set_x(pdipc_handle* h, const double* in, int npoints);
+int pdipc_get_v(pdipc_handle* h, double* out, int npoints);
+int pdipc_set_v(pdipc_handle* h, const double* in, int npoints);
+int pdipc_get_counters(pdipc_handle* h, pdipc_counters* out);
+
+/* --- saveSimulator (PdIpcSimulator.h:879-895): writes the positions as
+ * EigenMatrixIO::write_binary does (int64 rows = 3, int64 cols = npoints, then
+ * column-major data) in float (the reference's qeal) or double. */
+int pdipc_save_frame(pdipc_handle* h, const char* path, int as_double);
+
+/* --- candidate records of the last broad phase (int[5]: type, a, b, c, d as in
+ * getCcdPotentialICPairsListKernel, PdIpcGpuFunc.cu:1915-1945); returns the
+ * count in *count and copies min(count, cap) records. */
+int pdipc_get_candidates(pdipc_handle* h, int* out, int cap, int* count);
+int pdipc_get_patch_pairs(pdipc_handle* h, int* out, int cap, int* count);
+
+/* --- host precompute results for fixture comparison: the int32 image of a
+ * simulated model's boundary.dat (which = 0, PdIpcModel.h:219-251) or
+ * patchInfo.dat (which = 1, PdIpcModel.h:291-318).  Available after
+ * pdipc_host_setup; *count receives the int count. */
+int pdipc_get_model_dat(pdipc_handle* h, int model, int which, int* out, int cap, int* count);
+
+/* --- teacher-forced phase entry points (parity tests and profiling).  Arrays
+ * are AoS; x-like inputs cover all points. */
+/* gpuCcdCulling (PdIpcSimulator.cpp:1799-1869) on (xold, dx); returns K in *count */
+int pdipc_debug_broad_phase(pdipc_handle* h, const double* xold, const double* dx, int* count);
+/* getImpactOfTimeHost (PdIpcGpuFunc.cu:2706-2742) over the current candidates */
+int pdipc_debug_toi(pdipc_handle* h, const double* x0, const double* x1, double* toi);
+/* setCollisionConstraintsHost (PdIpcGpuFunc.cu:2894-2930): diag (sim), rhs (3*sim) */
+int pdipc_debug_collision(pdipc_handle* h, const double* x, double* diag, double* rhs, int* active);
+/* doLocalProjection (PdIpcSimulator.cpp:1624-1681) without the collision term */
+int pdipc_debug_local(pdipc_handle* h, const double* x, const double* init_rhs, double* rhs);
+/* updateSuperJacobiOperatorsHost + chebyshevJacobiR2SolverHost (PdIpcGpuFunc.cu:976-1346) */
+int pdipc_debug_solve(pdipc_handle* h, const double* diag, const double* rhs, const double* x0, double* xout);
+/* LBVH of the last broad phase: sorted 64-bit keys (P), internal nodes
+ * (left, right child as node ids: leaves are P-1+leaf) and parents (2P-1) */
+int pdipc_debug_lbvh(pdipc_handle* h, unsigned long long* keys, int* left, int* right, int* parent, int* npatches);
+
+/* --- measurement support (no counterpart in the reference): replays one
+ * kernel of the step `reps` times on the handle's stream, bracketed by HIP
+ * events, on the state left by the last step (work buffers are clobbered, the
+ * simulation state is not).  which = 0: FP64 Chebyshev sweep; 1: FP32-split
+ * sweep; 2: cloth local step; 3: CCD over the current candidates; 4: R2
+ * operator update.  Returns the mean duration per launch in microseconds and
+ * the algorithmic HBM bytes one launch moves (matrix entries once, vectors once). */
+int pdipc_time_kernel(pdipc_handle* h, int which, int reps, double* avg_us, double* bytes);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* PDIPC_H */

@@ -1,0 +1,95 @@
+// Host-side precompute of the PD-IPC step: mesh topology, per-model constraint
+// data, the PD system matrix, the R1 / R2 (M^2) Jacobi operators and the
+// collision patches.  Mirrors PdIpcSimulator::initialization
+// (Simulator/PD-IPC/PdIpcSimulator.cpp:34-268), setupHessMat (:270-414),
+// gpuSolverInit (:563-912) and gpuCollisionInit (:1151-1504), producing flat
+// arrays ready for the device.  No HIP calls: this file also builds and runs
+// on a machine without a GPU.
+#pragma once
+#include <array>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+namespace pdipc {
+
+enum ModelKind { MODEL_CLOTH = 0, MODEL_TET = 1, MODEL_STATIC = 2 };
+
+struct ModelInput {
+  int kind = MODEL_CLOTH;
+  std::vector<double> verts;  // 3*nv AoS
+  std::vector<int> elems;     // 3*nf triangles or 4*nt tets
+  double s0 = 0, s1 = 0, s2 = 0, density = 1;
+  int patchDim = 10;
+  std::vector<int> fixed;
+  int nv() const { return (int)verts.size() / 3; }
+};
+
+struct StepConstants {
+  double dt = 0.01, gravity = -9.8, dhat = 1e-3;
+};
+
+// Everything the device needs, in the order of the reference's device buffers.
+struct HostSetup {
+  int nTotal = 0, nSim = 0, nCloth = 0, nTets = 0;
+  std::vector<double> X;                 // 3*nTotal AoS, initial positions
+  std::vector<double> mass, invMass;     // nSim
+  std::vector<double> gravY;             // nSim: gravity * mass (y component)
+  std::vector<double> fixK;              // nSim: position stiffness, 0 if free
+  std::vector<int> fixFlag;              // nSim: 1 if fixed
+  std::vector<double> elasticDiag;       // nSim
+  // cloth: per cloth vertex (reference order), ring with the vertex itself first
+  std::vector<int> cCenter;              // nCloth: point id
+  std::vector<int> cOff, cNbr;           // CSR over slots
+  std::vector<double> cRest, cLbo, cBendIn;  // per slot
+  std::vector<double> cStrain, cH0;      // per cloth vertex
+  std::vector<int> clothOfPoint;         // nSim: cloth index or -1
+  // tets
+  std::vector<int> tetIdx;               // 4*T
+  std::vector<double> tetDminv;          // 9*T row-major Dm^{-1}
+  std::vector<double> tetW;              // T: stiffness * rest volume
+  std::vector<int> vtOff, vtList;        // point -> (tet*4 + corner)
+  // R1 = -offdiag(sysHess), CSR without the diagonal slot
+  std::vector<int> r1Off, r1Col;
+  std::vector<double> r1Val;
+  // M^2 pattern (CSR, ascending columns) with its R1 value and its 2-paths
+  std::vector<int> m2Off, m2Col;
+  std::vector<double> m2R1;              // R1(i,j) (0 if not an R1 entry)
+  std::vector<int64_t> p2Off;            // per M^2 entry
+  std::vector<int> p2K;                  // middle index k
+  std::vector<double> p2A;               // R1(i,k) * R1(k,j)
+  // collision
+  std::vector<int> bFaces, bEdges;       // 3*F, 2*E point ids
+  std::vector<int> pfOff, pfList;        // patch -> boundary faces
+  std::vector<int> ppOff, ppList;        // patch -> owned points
+  std::vector<int> peOff, peList;        // patch -> owned edges
+  std::vector<uint8_t> patchStatic;      // 1 if the patch belongs to a static model
+  std::vector<double> colMass;           // nTotal
+  // fixture images (boundary.dat / patchInfo.dat) per simulated model
+  std::vector<std::vector<int>> boundaryDat, patchDat;
+  int64_t nnzR1 = 0, nnzR2 = 0, n2paths = 0;
+};
+
+// Runs the whole precompute.  Returns an empty string on success, otherwise an
+// error message (degenerate input).
+std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vector<ModelInput>& statics,
+                             const StepConstants& k, HostSetup& out);
+
+// SELL-C layout of the M^2 pattern for the Chebyshev sweep: slices of
+// (64 / lanes) rows; within a slice, row r's c-th entry sits at
+//   sliceOff[s] + (c / lanes) * 64 + (r - s*rows) * lanes + (c % lanes).
+// Padding entries point at their own row with no 2-paths.
+struct SellLayout {
+  int lanes = 4, rowsPerSlice = 16, nSlices = 0;
+  std::vector<int64_t> sliceOff;   // nSlices + 1
+  std::vector<int> sliceW;         // nSlices: width in 64-entry columns
+  std::vector<int> col;            // per SELL entry
+  std::vector<int> rowFlag;        // row | 0x80000000 on the diagonal entry
+  std::vector<double> r1;          // per SELL entry
+  std::vector<int64_t> pOff;       // per SELL entry + 1
+  std::vector<int> pK;
+  std::vector<double> pA;
+};
+void build_sell(const HostSetup& hs, int lanes, SellLayout& out);
+
+}  // namespace pdipc

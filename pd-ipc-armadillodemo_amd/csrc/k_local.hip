@@ -1,0 +1,287 @@
+// The PD local step (doLocalProjection, Simulator/PD-IPC/PdIpcSimulator.cpp:1624-1681)
+// as two gathers instead of the reference's float atomics:
+//   1. k_cloth_curvature: per cloth vertex, the Laplace-Beltrami vector a_i,
+//      rescaled to the rest mean-curvature norm (PdIpcGpuFunc.cu:668-707);
+//      k_tet_project: per tet, the corotational strain projection
+//      w * U clamp(S) V^T Dm^{-T} G (PdIpcGpuFunc.cu:439-599) into 12 outputs.
+//   2. k_rhs: per point, rhs = rhs0 + tet outputs + springs + bending gathered
+//      from the 1-ring + collision rhs + position term.
+// Every point's sum is formed by one lane in a fixed order, so the FP64 result
+// is bit-reproducible run to run.  T is the arithmetic type of the projections
+// (double, or float in the FP32-split mode); edge vectors are formed in double
+// and rounded once, so FP32 never sees absolute coordinates.
+#include "device.hpp"
+
+namespace pdipc {
+namespace {
+
+constexpr int kBlock = 256;
+inline int grid_for(int64_t n) { return (int)((n + kBlock - 1) / kBlock); }
+
+template <class T> __device__ inline T dsqrt(T x);
+template <> __device__ inline double dsqrt<double>(double x) { return sqrt(x); }
+template <> __device__ inline float dsqrt<float>(float x) { return sqrtf(x); }
+
+// 3x3 SVD: cyclic Jacobi on F^T F with exact Givens rotations (run until the
+// off-diagonal mass is negligible), eigenvalues sorted descending, V made a
+// rotation, then Gram-Schmidt on F V gives U (a rotation) and the singular
+// values, the last one carrying the sign of det F -- the output convention of
+// McAdams et al. 2011 (the method of CudaSVD.cu:291-316).
+template <class T>
+__device__ inline void svd3(const T F[9], T U[9], T sig[3], T V[9]) {
+  T a00 = F[0] * F[0] + F[3] * F[3] + F[6] * F[6];
+  T a01 = F[0] * F[1] + F[3] * F[4] + F[6] * F[7];
+  T a02 = F[0] * F[2] + F[3] * F[5] + F[6] * F[8];
+  T a11 = F[1] * F[1] + F[4] * F[4] + F[7] * F[7];
+  T a12 = F[1] * F[2] + F[4] * F[5] + F[7] * F[8];
+  T a22 = F[2] * F[2] + F[5] * F[5] + F[8] * F[8];
+  T W[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1};  // row-major, columns are eigenvectors
+  const int maxSweeps = sizeof(T) == 8 ? 12 : 6;
+  const T tiny = sizeof(T) == 8 ? (T)1e-34 : (T)1e-14;
+  for (int sw = 0; sw < maxSweeps; sw++) {
+    T off = a01 * a01 + a12 * a12 + a02 * a02;
+    T dia = a00 * a00 + a11 * a11 + a22 * a22;
+    if (!(off > dia * tiny)) break;
+    // rotate (p,q) = (0,1), (1,2), (0,2)
+#pragma unroll
+    for (int r = 0; r < 3; r++) {
+      T *app, *aqq, *apq, *apr, *aqr;
+      int p, q;
+      if (r == 0) { app = &a00; aqq = &a11; apq = &a01; apr = &a02; aqr = &a12; p = 0; q = 1; }
+      else if (r == 1) { app = &a11; aqq = &a22; apq = &a12; apr = &a01; aqr = &a02; p = 1; q = 2; }
+      else { app = &a00; aqq = &a22; apq = &a02; apr = &a01; aqr = &a12; p = 0; q = 2; }
+      T x = *apq;
+      if (x != (T)0) {
+        T tau = (*aqq - *app) / (2 * x);
+        T t = (tau >= 0 ? (T)1 : (T)-1) / (fabs(tau) + dsqrt<T>((T)1 + tau * tau));
+        T c = (T)1 / dsqrt<T>((T)1 + t * t), s = t * c;
+        T pp = *app, qq = *aqq;
+        *app = pp - t * x;
+        *aqq = qq + t * x;
+        *apq = 0;
+        T pr = *apr, qr = *aqr;  // the entries coupling p and q to the third index
+        *apr = c * pr - s * qr;
+        *aqr = s * pr + c * qr;
+#pragma unroll
+        for (int k = 0; k < 3; k++) {
+          T wp = W[3 * k + p], wq = W[3 * k + q];
+          W[3 * k + p] = c * wp - s * wq;
+          W[3 * k + q] = s * wp + c * wq;
+        }
+      }
+    }
+  }
+  T lam[3] = {a00, a11, a22};
+  int o0 = 0, o1 = 1, o2 = 2, tmp;
+  if (lam[o1] > lam[o0]) { tmp = o0; o0 = o1; o1 = tmp; }
+  if (lam[o2] > lam[o0]) { tmp = o0; o0 = o2; o2 = tmp; }
+  if (lam[o2] > lam[o1]) { tmp = o1; o1 = o2; o2 = tmp; }
+  int ord[3] = {o0, o1, o2};
+#pragma unroll
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) V[3 * k + c] = W[3 * k + ord[c]];
+  T detV = V[0] * (V[4] * V[8] - V[5] * V[7]) - V[1] * (V[3] * V[8] - V[5] * V[6]) + V[2] * (V[3] * V[7] - V[4] * V[6]);
+  if (detV < 0) { V[2] = -V[2]; V[5] = -V[5]; V[8] = -V[8]; }
+  T b[3][3];
+#pragma unroll
+  for (int c = 0; c < 3; c++)
+#pragma unroll
+    for (int r = 0; r < 3; r++) b[c][r] = F[3 * r] * V[c] + F[3 * r + 1] * V[3 + c] + F[3 * r + 2] * V[6 + c];
+  T u0[3], u1[3], u2[3];
+  T s0 = dsqrt<T>(b[0][0] * b[0][0] + b[0][1] * b[0][1] + b[0][2] * b[0][2]);
+  if (s0 > (T)1e-30) { u0[0] = b[0][0] / s0; u0[1] = b[0][1] / s0; u0[2] = b[0][2] / s0; }
+  else { u0[0] = 1; u0[1] = 0; u0[2] = 0; }
+  T pr = u0[0] * b[1][0] + u0[1] * b[1][1] + u0[2] * b[1][2];
+  u1[0] = b[1][0] - pr * u0[0]; u1[1] = b[1][1] - pr * u0[1]; u1[2] = b[1][2] - pr * u0[2];
+  T s1 = dsqrt<T>(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+  if (s1 > (T)1e-30 * (s0 > 1 ? s0 : (T)1)) {
+    u1[0] /= s1; u1[1] /= s1; u1[2] /= s1;
+  } else {
+    T ax = fabs(u0[0]), ay = fabs(u0[1]), az = fabs(u0[2]);
+    T t[3] = {0, 0, 0};
+    t[(ax < ay) ? (ax < az ? 0 : 2) : (ay < az ? 1 : 2)] = 1;
+    u1[0] = u0[1] * t[2] - u0[2] * t[1];
+    u1[1] = u0[2] * t[0] - u0[0] * t[2];
+    u1[2] = u0[0] * t[1] - u0[1] * t[0];
+    T n = dsqrt<T>(u1[0] * u1[0] + u1[1] * u1[1] + u1[2] * u1[2]);
+    u1[0] /= n; u1[1] /= n; u1[2] /= n;
+    s1 = 0;
+  }
+  u2[0] = u0[1] * u1[2] - u0[2] * u1[1];
+  u2[1] = u0[2] * u1[0] - u0[0] * u1[2];
+  u2[2] = u0[0] * u1[1] - u0[1] * u1[0];
+  T s2 = u2[0] * b[2][0] + u2[1] * b[2][1] + u2[2] * b[2][2];
+#pragma unroll
+  for (int r = 0; r < 3; r++) { U[3 * r] = u0[r]; U[3 * r + 1] = u1[r]; U[3 * r + 2] = u2[r]; }
+  sig[0] = s0; sig[1] = s1; sig[2] = s2;
+}
+
+template <class T>
+__global__ void k_cloth_curvature(int nCloth, int ld, const int* __restrict__ cCenter, const int* __restrict__ cOff,
+                                  const int* __restrict__ cNbr, const double* __restrict__ cLbo,
+                                  const double* __restrict__ cH0, const double* __restrict__ X,
+                                  double* __restrict__ A) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nCloth) return;
+  int i = cCenter[c];
+  int o = cOff[c], e = cOff[c + 1];
+  double xi[3] = {X[i], X[ld + i], X[2 * ld + i]};
+  // a = sum_s L_s x_s = sum_{s>0} L_s (x_s - x_i) + (sum_s L_s) x_i  (the row sum is ~0)
+  double lsum = cLbo[o];
+  T a[3] = {0, 0, 0};
+  for (int s = o + 1; s < e; s++) {
+    int j = cNbr[s];
+    T l = (T)cLbo[s];
+    lsum += cLbo[s];
+#pragma unroll
+    for (int k = 0; k < 3; k++) a[k] += l * (T)(X[k * ld + j] - xi[k]);
+  }
+  double ad[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) ad[k] = (double)a[k] + lsum * xi[k];
+  double n = sqrt(ad[0] * ad[0] + ad[1] * ad[1] + ad[2] * ad[2]);
+  if (n >= 1e-10) {
+    double sc = cH0[c] / n;
+#pragma unroll
+    for (int k = 0; k < 3; k++) ad[k] *= sc;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) A[k * ld + i] = ad[k];
+}
+
+template <class T>
+__global__ void k_tet_project(int nT, int ld, T sigLo, T sigHi, const int4* __restrict__ tets,
+                              const double* __restrict__ dminv, const double* __restrict__ w,
+                              const double* __restrict__ X, double* __restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nT) return;
+  int4 q = tets[t];
+  int id[4] = {q.x, q.y, q.z, q.w};
+  T Ds[9];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    double x0 = X[r * ld + id[0]];
+#pragma unroll
+    for (int c = 0; c < 3; c++) Ds[3 * r + c] = (T)(X[r * ld + id[c + 1]] - x0);
+  }
+  T idm[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) idm[k] = (T)dminv[(int64_t)k * nT + t];
+  T F[9];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) F[3 * r + c] = Ds[3 * r] * idm[c] + Ds[3 * r + 1] * idm[3 + c] + Ds[3 * r + 2] * idm[6 + c];
+  T U[9], s[3], V[9];
+  svd3<T>(F, U, s, V);
+  if (s[0] * s[1] * s[2] < 0) {  // inverted: flip the last value and re-sort the values only
+    s[2] = -s[2];
+    T hi = s[0], mid = s[1], lo = s[2], tt;
+    if (mid < lo) { tt = lo; lo = mid; mid = tt; }
+    if (hi < lo) { tt = lo; lo = hi; hi = tt; }
+    if (hi < mid) { tt = mid; mid = hi; hi = tt; }
+    s[0] = hi; s[1] = mid; s[2] = lo;
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    T r = s[k] < sigHi ? s[k] : sigHi;
+    s[k] = r < sigLo ? sigLo : r;
+  }
+  T P[9];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) P[3 * r + c] = U[3 * r] * s[0] * V[3 * c] + U[3 * r + 1] * s[1] * V[3 * c + 1] + U[3 * r + 2] * s[2] * V[3 * c + 2];
+  // result = w * P * [ -Dm^{-T} 1 | Dm^{-T} ]   (PdIpcGpuFunc.cu:554-580)
+  T wt = (T)w[t];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    T g1 = P[3 * r] * idm[0] + P[3 * r + 1] * idm[1] + P[3 * r + 2] * idm[2];
+    T g2 = P[3 * r] * idm[3] + P[3 * r + 1] * idm[4] + P[3 * r + 2] * idm[5];
+    T g3 = P[3 * r] * idm[6] + P[3 * r + 1] * idm[7] + P[3 * r + 2] * idm[8];
+    T g0 = -(g1 + g2 + g3);
+    out[(int64_t)(0 * 3 + r) * nT + t] = (double)(g0 * wt);
+    out[(int64_t)(1 * 3 + r) * nT + t] = (double)(g1 * wt);
+    out[(int64_t)(2 * 3 + r) * nT + t] = (double)(g2 * wt);
+    out[(int64_t)(3 * 3 + r) * nT + t] = (double)(g3 * wt);
+  }
+}
+
+template <class T>
+__global__ void k_rhs(int nSim, int ld, int nT, int withCol, const double* __restrict__ initRhs,
+                      const int* __restrict__ vtOff, const int* __restrict__ vtList, const double* __restrict__ tetOut,
+                      const int* __restrict__ clothOfPoint, const int* __restrict__ cOff, const int* __restrict__ cNbr,
+                      const double* __restrict__ cRest, const double* __restrict__ cStrain,
+                      const double* __restrict__ cBendIn, const double* __restrict__ A, const double* __restrict__ X,
+                      const double* __restrict__ colRhs, const double* __restrict__ fixK,
+                      const double* __restrict__ Xrest, double* __restrict__ rhs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nSim) return;
+  double r[3] = {initRhs[i], initRhs[ld + i], initRhs[2 * ld + i]};
+  for (int q = vtOff[i]; q < vtOff[i + 1]; q++) {
+    int tc = vtList[q];
+    int t = tc >> 2, corner = tc & 3;
+#pragma unroll
+    for (int k = 0; k < 3; k++) r[k] += tetOut[(int64_t)(corner * 3 + k) * nT + t];
+  }
+  int c = clothOfPoint[i];
+  if (c >= 0) {
+    int o = cOff[c], e = cOff[c + 1];
+    double xi[3] = {X[i], X[ld + i], X[2 * ld + i]};
+    T st = (T)cStrain[c];
+    T sp[3] = {0, 0, 0};
+    for (int s = o + 1; s < e; s++) {
+      int j = cNbr[s];
+      T d[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) d[k] = (T)(xi[k] - X[k * ld + j]);
+      T len = dsqrt<T>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      T rc = st * ((T)cRest[s] / len);
+#pragma unroll
+      for (int k = 0; k < 3; k++) sp[k] += rc * d[k];
+    }
+    double bd[3] = {0, 0, 0};
+    for (int s = o; s < e; s++) {
+      int j = cNbr[s];
+      double b = cBendIn[s];
+#pragma unroll
+      for (int k = 0; k < 3; k++) bd[k] += b * A[k * ld + j];
+    }
+#pragma unroll
+    for (int k = 0; k < 3; k++) r[k] += (double)sp[k] + bd[k];
+  }
+  if (withCol) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) r[k] += colRhs[k * ld + i];
+  }
+  double kf = fixK[i];
+  if (kf != 0.0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) r[k] += kf * Xrest[k * ld + i];
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) rhs[k * ld + i] = r[k];
+}
+
+template <class T>
+void local_impl(const DeviceMesh& m, T lo, T hi, int withCol, hipStream_t s) {
+  if (m.nCloth > 0)
+    hipLaunchKernelGGL(k_cloth_curvature<T>, dim3(grid_for(m.nCloth)), dim3(kBlock), 0, s, m.nCloth, m.ld, m.cCenter,
+                       m.cOff, m.cNbr, m.cLbo, m.cH0, m.X, m.clothA);
+  if (m.nTets > 0)
+    hipLaunchKernelGGL(k_tet_project<T>, dim3(grid_for(m.nTets)), dim3(kBlock), 0, s, m.nTets, m.ld, lo, hi,
+                       m.tetIdx, m.tetDminv, m.tetW, m.X, m.tetOut);
+  hipLaunchKernelGGL(k_rhs<T>, dim3(grid_for(m.nSim)), dim3(kBlock), 0, s, m.nSim, m.ld, m.nTets, withCol, m.initRhs,
+                     m.vtOff, m.vtList, m.tetOut, m.clothOfPoint, m.cOff, m.cNbr, m.cRest, m.cStrain, m.cBendIn,
+                     m.clothA, m.X, m.colRhs, m.fixK, m.Xrest, m.sysRhs);
+}
+
+}  // namespace
+
+void launch_local(const DeviceMesh& m, int precision, double sigLo, double sigHi, int withCollision, hipStream_t s) {
+  if (precision == 1) local_impl<float>(m, (float)sigLo, (float)sigHi, withCollision, s);
+  else local_impl<double>(m, sigLo, sigHi, withCollision, s);
+}
+
+}  // namespace pdipc

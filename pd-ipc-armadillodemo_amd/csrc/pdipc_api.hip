@@ -1,0 +1,719 @@
+// C ABI of the MI355X PD-IPC step (include/pdipc.h) and the step driver
+// PdIpcSimulator::advance (Simulator/PD-IPC/PdIpcSimulator.cpp:1683-1796) on one
+// HIP stream.  The reference synchronises after every kernel and reads four
+// scalars per outer iteration; here kernels are queued back to back and the
+// host waits once per outer iteration (for the |d|^2 break test), plus twice in
+// the per-step broad phase (to size the pair and record lists).
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "../../include/pdipc.h"
+#include "device.hpp"
+#include "host_setup.hpp"
+
+using namespace pdipc;
+
+namespace {
+
+thread_local std::string g_err;
+
+int fail(int code, const std::string& msg) {
+  g_err = msg;
+  return code;
+}
+
+#define HIPCHK(expr)                                                                          \
+  do {                                                                                        \
+    hipError_t e_ = (expr);                                                                   \
+    if (e_ != hipSuccess) return fail(PDIPC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
+  } while (0)
+
+}  // namespace
+
+struct pdipc_handle {
+  pdipc_params prm;
+  std::vector<ModelInput> sims, statics;
+  bool hostReady = false, deviceReady = false;
+  HostSetup hs;
+  SellLayout sell;
+  DeviceMesh m;
+  hipStream_t stream = nullptr;
+  std::vector<void*> allocs;
+  // host mirrors
+  void* scalHost = nullptr;  // pinned: residual, toiBits, counters[8]
+  void* scalDev = nullptr;
+  std::vector<double> stage;
+  pdipc_counters cnt;
+  int lastRecs = 0, lastPairs = 0;
+
+  template <class T>
+  int alloc(T** p, size_t n) {
+    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) return fail(PDIPC_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
+    allocs.push_back(*p);
+    return PDIPC_OK;
+  }
+  template <class T>
+  int upload(T** p, const std::vector<T>& v) {
+    int rc = alloc(p, v.size());
+    if (rc) return rc;
+    if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+    return PDIPC_OK;
+  }
+  void release(void* p) {
+    if (!p) return;
+    auto it = std::find(allocs.begin(), allocs.end(), p);
+    if (it != allocs.end()) allocs.erase(it);
+    (void)hipFree(p);
+  }
+  ~pdipc_handle() {
+    for (void* p : allocs) (void)hipFree(p);
+    if (scalHost) (void)hipHostFree(scalHost);
+    if (stream) (void)hipStreamDestroy(stream);
+  }
+  double* hResidual() { return (double*)scalHost; }
+  unsigned long long* hToi() { return (unsigned long long*)((char*)scalHost + 8); }
+  int* hCounters() { return (int*)((char*)scalHost + 16); }
+};
+
+namespace {
+
+// AoS xyz (npoints) -> SoA [3][ld] device
+int put_soa(pdipc_handle* h, double* dst, const double* aos, int npts) {
+  h->stage.assign(3 * (size_t)h->m.ld, 0.0);
+  for (int i = 0; i < npts; i++)
+    for (int k = 0; k < 3; k++) h->stage[(size_t)k * h->m.ld + i] = aos[3 * (size_t)i + k];
+  HIPCHK(hipMemcpyAsync(dst, h->stage.data(), 3 * (size_t)h->m.ld * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PDIPC_OK;
+}
+// partial overwrite: only the first npts points of each component
+int put_soa_prefix(pdipc_handle* h, double* dst, const double* aos, int npts) {
+  h->stage.assign(3 * (size_t)h->m.ld, 0.0);
+  HIPCHK(hipMemcpyAsync(h->stage.data(), dst, 3 * (size_t)h->m.ld * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < npts; i++)
+    for (int k = 0; k < 3; k++) h->stage[(size_t)k * h->m.ld + i] = aos[3 * (size_t)i + k];
+  HIPCHK(hipMemcpyAsync(dst, h->stage.data(), 3 * (size_t)h->m.ld * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PDIPC_OK;
+}
+int get_soa(pdipc_handle* h, const double* src, double* aos, int npts) {
+  h->stage.assign(3 * (size_t)h->m.ld, 0.0);
+  HIPCHK(hipMemcpyAsync(h->stage.data(), src, 3 * (size_t)h->m.ld * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < npts; i++)
+    for (int k = 0; k < 3; k++) aos[3 * (size_t)i + k] = h->stage[(size_t)k * h->m.ld + i];
+  return PDIPC_OK;
+}
+
+int check_launch(pdipc_handle* h) {
+  hipError_t e = hipGetLastError();
+  if (e != hipSuccess) return fail(PDIPC_E_HIP, std::string("kernel launch: ") + hipGetErrorString(e));
+  (void)h;
+  return PDIPC_OK;
+}
+
+int read_scalars(pdipc_handle* h) {
+  HIPCHK(hipMemcpyAsync(h->scalHost, h->scalDev, 64, hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PDIPC_OK;
+}
+
+int upload_all(pdipc_handle* h) {
+  HostSetup& hs = h->hs;
+  DeviceMesh& m = h->m;
+  int lanes = h->prm.sell_lanes;
+  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) lanes = 4;
+  build_sell(hs, lanes, h->sell);
+  m.nTotal = hs.nTotal;
+  m.nSim = hs.nSim;
+  m.nCloth = hs.nCloth;
+  m.nTets = hs.nTets;
+  m.ld = ((hs.nTotal + 63) / 64) * 64;
+  size_t V3 = 3 * (size_t)m.ld;
+  int rc;
+#define UP(field, vec) if ((rc = h->upload(&m.field, vec))) return rc
+#define AL(field, n) if ((rc = h->alloc(&m.field, n))) return rc
+  AL(X, V3); AL(Xold, V3); AL(X0, V3); AL(Dir, V3); AL(Pre, V3); AL(Next, V3); AL(V, V3);
+  AL(initRhs, V3); AL(sysRhs, V3); AL(firstTerm, V3); AL(colRhs, V3); AL(clothA, V3); AL(Xrest, V3);
+  AL(dPre, V3); AL(dCur, V3); AL(dNext, V3); AL(r0, V3);
+  for (double* p : {m.X, m.Xold, m.X0, m.Dir, m.Pre, m.Next, m.V, m.initRhs, m.sysRhs, m.firstTerm, m.colRhs, m.clothA,
+                    m.Xrest})
+    HIPCHK(hipMemset(p, 0, V3 * sizeof(double)));
+  for (float* p : {m.dPre, m.dCur, m.dNext, m.r0}) HIPCHK(hipMemset(p, 0, V3 * sizeof(float)));
+  UP(mass, hs.mass); UP(invMass, hs.invMass); UP(gravY, hs.gravY); UP(fixK, hs.fixK); UP(fixFlag, hs.fixFlag);
+  UP(colMass, hs.colMass); UP(elasticDiag, hs.elasticDiag);
+  AL(inertialDiag, hs.nSim); AL(sysDiag, hs.nSim); AL(colDiag, hs.nTotal);
+  HIPCHK(hipMemset(m.inertialDiag, 0, std::max(1, hs.nSim) * sizeof(double)));
+  HIPCHK(hipMemset(m.colDiag, 0, std::max(1, hs.nTotal) * sizeof(double)));
+  UP(cCenter, hs.cCenter); UP(cOff, hs.cOff.empty() ? std::vector<int>{0} : hs.cOff); UP(cNbr, hs.cNbr);
+  UP(clothOfPoint, hs.clothOfPoint); UP(cRest, hs.cRest); UP(cLbo, hs.cLbo); UP(cBendIn, hs.cBendIn);
+  UP(cStrain, hs.cStrain); UP(cH0, hs.cH0);
+  {
+    std::vector<int4> t(hs.nTets);
+    for (int e = 0; e < hs.nTets; e++) t[e] = make_int4(hs.tetIdx[4 * e], hs.tetIdx[4 * e + 1], hs.tetIdx[4 * e + 2], hs.tetIdx[4 * e + 3]);
+    UP(tetIdx, t);
+    std::vector<double> dm(9 * (size_t)hs.nTets);
+    for (int e = 0; e < hs.nTets; e++)
+      for (int k = 0; k < 9; k++) dm[(size_t)k * hs.nTets + e] = hs.tetDminv[9 * (size_t)e + k];
+    UP(tetDminv, dm);
+  }
+  UP(tetW, hs.tetW); AL(tetOut, 12 * (size_t)hs.nTets);
+  UP(vtOff, hs.vtOff); UP(vtList, hs.vtList);
+  UP(r1Off, hs.r1Off); UP(r1Col, hs.r1Col); UP(r1Val, hs.r1Val);
+  const SellLayout& L = h->sell;
+  m.sellLanes = L.lanes;
+  m.rowsPerSlice = L.rowsPerSlice;
+  m.nSlices = L.nSlices;
+  m.sellEntries = L.sliceOff.back();
+  UP(sliceOff, L.sliceOff); UP(sliceW, L.sliceW); UP(m2Col, L.col); UP(m2RowFlag, L.rowFlag); UP(m2R1, L.r1);
+  UP(p2Off, L.pOff); UP(p2K, L.pK); UP(p2A, L.pA);
+  AL(m2Val, m.sellEntries); AL(m2ValF, m.sellEntries);
+  // collision
+  m.nFaces = (int)hs.bFaces.size() / 3;
+  m.nEdges = (int)hs.bEdges.size() / 2;
+  m.nPatches = (int)hs.patchStatic.size();
+  {
+    std::vector<int3> f(m.nFaces);
+    for (int i = 0; i < m.nFaces; i++) f[i] = make_int3(hs.bFaces[3 * i], hs.bFaces[3 * i + 1], hs.bFaces[3 * i + 2]);
+    UP(bFaces, f);
+    std::vector<int2> e(m.nEdges);
+    for (int i = 0; i < m.nEdges; i++) e[i] = make_int2(hs.bEdges[2 * i], hs.bEdges[2 * i + 1]);
+    UP(bEdges, e);
+  }
+  UP(pfOff, hs.pfOff); UP(pfList, hs.pfList); UP(ppOff, hs.ppOff); UP(ppList, hs.ppList); UP(peOff, hs.peOff);
+  UP(peList, hs.peList); UP(patchStatic, hs.patchStatic);
+  int P = std::max(1, m.nPatches);
+  AL(patchBox, 6 * (size_t)P); AL(sceneBox, 6); AL(sceneBoxPart, 6 * 256);
+  AL(mortonA, P); AL(mortonB, P); AL(idxA, P); AL(idxB, P); AL(keys, P);
+  int nbRadix = (P + 2047) / 2048;
+  AL(radixHist, 256 * (size_t)nbRadix); AL(radixBlockSums, (256 * (size_t)nbRadix + 1023) / 1024 + 2);
+  AL(nodeLeft, P); AL(nodeRight, P); AL(nodeParent, 2 * (size_t)P); AL(nodeFlag, P); AL(nodeBox, 6 * (2 * (size_t)P));
+  AL(pairCount, P); AL(pairOff, P);
+  m.pairCap = std::max(1024, 8 * P);
+  AL(pairs, m.pairCap); AL(recCount, m.pairCap); AL(recOff, m.pairCap);
+  m.recCap = std::max(4096, 32 * P);
+  AL(recs, m.recCap);
+  m.scanBlockCap = (std::max(P, m.pairCap) + 1023) / 1024 + 2;
+  AL(scanBlock, m.scanBlockCap);
+  m.nPartials = (std::max(1, hs.nSim) + 255) / 256;
+  AL(partials, m.nPartials);
+  // scalars: residual (8), toiBits (8), counters (8 ints)
+  void* sd = nullptr;
+  HIPCHK(hipMalloc(&sd, 64));
+  h->allocs.push_back(sd);
+  HIPCHK(hipMemset(sd, 0, 64));
+  h->scalDev = sd;
+  m.residual = (double*)sd;
+  m.toiBits = (unsigned long long*)((char*)sd + 8);
+  m.counters = (int*)((char*)sd + 16);
+  HIPCHK(hipHostMalloc(&h->scalHost, 64, hipHostMallocDefault));
+  memset(h->scalHost, 0, 64);
+  // initial positions
+  {
+    std::vector<double> soa(V3, 0.0);
+    for (int i = 0; i < hs.nTotal; i++)
+      for (int k = 0; k < 3; k++) soa[(size_t)k * m.ld + i] = hs.X[3 * (size_t)i + k];
+    for (double* p : {m.X, m.Xold, m.X0, m.Pre, m.Next, m.Xrest})
+      HIPCHK(hipMemcpy(p, soa.data(), V3 * sizeof(double), hipMemcpyHostToDevice));
+  }
+#undef UP
+#undef AL
+  return PDIPC_OK;
+}
+
+
+// gpuCcdCulling (PdIpcSimulator.cpp:1799-1869) on (x, dx)
+int broad_phase(pdipc_handle* h, const double* x, const double* dx) {
+  DeviceMesh& m = h->m;
+  hipStream_t s = h->stream;
+  h->lastPairs = 0;
+  h->lastRecs = 0;
+  if (m.nPatches <= 0) return PDIPC_OK;
+  double infl = h->prm.inflation >= 0 ? h->prm.inflation : 0.5 * h->prm.dhat;
+  HIPCHK(hipMemsetAsync(m.counters, 0, 8 * sizeof(int), s));
+  launch_patch_boxes(m, x, dx, infl, s);
+  launch_lbvh(m, s);
+  launch_pair_count(m, h->prm.skip_static_pairs, s);
+  int rc = check_launch(h);
+  if (rc) return rc;
+  if ((rc = read_scalars(h))) return rc;
+  int Q = h->hCounters()[0];
+  if (h->hCounters()[3]) return fail(PDIPC_E_CAPACITY, "LBVH traversal stack overflow");
+  if (Q > m.pairCap) {
+    int cap = Q + Q / 2 + 1024;
+    h->release(m.pairs);
+    if ((rc = h->alloc(&m.pairs, cap))) return rc;
+    h->release(m.recCount);
+    if ((rc = h->alloc(&m.recCount, cap))) return rc;
+    h->release(m.recOff);
+    if ((rc = h->alloc(&m.recOff, cap))) return rc;
+    int sb = (cap + 1023) / 1024 + 2;
+    if (sb > m.scanBlockCap) {
+      h->release(m.scanBlock);
+      if ((rc = h->alloc(&m.scanBlock, sb))) return rc;
+      m.scanBlockCap = sb;
+    }
+    m.pairCap = cap;
+  }
+  launch_pair_fill(m, h->prm.skip_static_pairs, s);
+  launch_rec_count(m, Q, s);
+  if ((rc = check_launch(h))) return rc;
+  if ((rc = read_scalars(h))) return rc;
+  int K = Q > 0 ? h->hCounters()[1] : 0;
+  if (K < 0) return fail(PDIPC_E_CAPACITY, "candidate count overflows int32");
+  if (K > m.recCap) {
+    int cap = K + K / 2 + 4096;
+    h->release(m.recs);
+    if ((rc = h->alloc(&m.recs, cap))) return rc;
+    m.recCap = cap;
+  }
+  launch_rec_fill(m, Q, K, s);
+  if ((rc = check_launch(h))) return rc;
+  h->lastPairs = Q;
+  h->lastRecs = K;
+  return PDIPC_OK;
+}
+
+int local_global(pdipc_handle* h, int withCol) {
+  DeviceMesh& m = h->m;
+  const pdipc_params& p = h->prm;
+  hipStream_t s = h->stream;
+  launch_diag(m, withCol, s);
+  launch_operators(m, p.omega, p.precision, s);
+  for (int it = 0; it < p.lg_iters; it++) {
+    launch_local(m, p.precision, p.sigma_lo, p.sigma_hi, withCol, s);
+    launch_solve(m, p.omega, p.rho, p.cheb_iters, p.precision, s);
+  }
+  return check_launch(h);
+}
+
+int advance(pdipc_handle* h) {
+  DeviceMesh& m = h->m;
+  const pdipc_params& p = h->prm;
+  hipStream_t s = h->stream;
+  size_t V3 = 3 * (size_t)m.ld;
+  int rc;
+  launch_copy(m.X0, m.X, (int64_t)V3, s);
+  launch_copy(m.Xold, m.X, (int64_t)V3, s);
+  launch_predict(m, p.dt, s);
+  bool activeCol = false;
+  int outer = 0;
+  double toiMin = 1.0;
+  h->cnt.lg_iters = 0;
+  for (; outer < p.max_outer; outer++) {
+    if ((rc = local_global(h, activeCol ? 1 : 0))) return rc;
+    h->cnt.lg_iters += p.lg_iters;
+    launch_direction(m, s);
+    if (outer == 0) {
+      if ((rc = broad_phase(h, m.Xold, m.Dir))) return rc;
+    }
+    launch_toi_init(m, s);
+    launch_ccd(m, h->lastRecs, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap, p.touch_policy, s);
+    launch_clamp_residual(m, p.toi_scale, s);
+    launch_zero(m.colDiag, m.nTotal, s);
+    launch_zero(m.colRhs, (int64_t)V3, s);
+    HIPCHK(hipMemsetAsync(m.counters + 2, 0, sizeof(int), s));
+    launch_collision(m, h->lastRecs, p.dhat, p.kappa, p.dt, s);
+    activeCol = true;
+    if ((rc = check_launch(h))) return rc;
+    if ((rc = read_scalars(h))) return rc;
+    double res = *h->hResidual();
+    double tmin;
+    unsigned long long b = *h->hToi();
+    memcpy(&tmin, &b, 8);
+    double toi = tmin < 1.0 ? tmin * p.toi_scale : 1.0;
+    if (toi < toiMin) toiMin = toi;
+    h->cnt.residual = res;
+    h->cnt.active_pairs = h->hCounters()[2];
+    if (!std::isfinite(res)) return fail(PDIPC_E_NAN, "non-finite search direction");
+    if (res < p.residual_tol && outer > 0) break;
+    launch_copy(m.Xold, m.X, (int64_t)V3, s);
+  }
+  h->cnt.outer_iters = outer < p.max_outer ? outer + 1 : p.max_outer;
+  h->cnt.toi_min = toiMin;
+  h->cnt.patch_pairs = h->lastPairs;
+  h->cnt.candidates = h->lastRecs;
+  launch_velocity(m, p.dt, s);
+  return check_launch(h);
+}
+
+int require_device(pdipc_handle* h) {
+  if (!h) return fail(PDIPC_E_ARG, "null handle");
+  if (!h->deviceReady) return fail(PDIPC_E_ARG, "pdipc_init has not run");
+  return PDIPC_OK;
+}
+
+ModelInput make_input(int kind, const double* v, int nv, const int* el, int nel, int per, double s0, double s1,
+                      double s2, double density, int patchDim, const int* fixed, int nfixed) {
+  ModelInput m;
+  m.kind = kind;
+  m.verts.assign(v, v + 3 * (size_t)nv);
+  m.elems.assign(el, el + per * (size_t)nel);
+  m.s0 = s0; m.s1 = s1; m.s2 = s2; m.density = density; m.patchDim = patchDim;
+  if (fixed && nfixed > 0) m.fixed.assign(fixed, fixed + nfixed);
+  return m;
+}
+
+}  // namespace
+
+extern "C" {
+
+void pdipc_default_params(pdipc_params* o) {
+  if (!o) return;
+  memset(o, 0, sizeof(*o));
+  o->dt = 0.01; o->gravity = -9.8; o->dhat = 1e-3; o->kappa = 1e4;
+  o->lg_iters = 10; o->cheb_iters = 15; o->max_outer = 30; o->residual_tol = 1e-5;
+  o->omega = 2.0 / 3.0; o->rho = 0.9992; o->sigma_lo = 0.999; o->sigma_hi = 1.001;
+  o->ccd_filter = 5e-3; o->toi_scale = 0.98; o->ccd_iters = 10; o->ccd_step = 0.9; o->ccd_gap = 0.1;
+  o->static_mass = 1e4; o->inflation = -1.0; o->skip_static_pairs = 0; o->touch_policy = 1;
+  o->precision = PDIPC_FP64; o->sell_lanes = 4; o->use_graphs = 0;
+}
+
+const char* pdipc_last_error(void) { return g_err.c_str(); }
+
+int pdipc_create(const pdipc_params* params, pdipc_handle** out) {
+  if (!out) return fail(PDIPC_E_ARG, "null output pointer");
+  auto* h = new pdipc_handle();
+  if (params) h->prm = *params;
+  else pdipc_default_params(&h->prm);
+  if (h->prm.dt <= 0 || h->prm.lg_iters < 1 || h->prm.cheb_iters < 1 || h->prm.max_outer < 1) {
+    delete h;
+    return fail(PDIPC_E_ARG, "invalid step parameters");
+  }
+  memset(&h->cnt, 0, sizeof(h->cnt));
+  *out = h;
+  return PDIPC_OK;
+}
+
+int pdipc_destroy(pdipc_handle* h) {
+  delete h;
+  return PDIPC_OK;
+}
+
+int pdipc_add_cloth(pdipc_handle* h, const double* verts, int nverts, const int* tris, int ntris, double s_strain,
+                    double s_bend, double s_pos, double density, int patch_dim, const int* fixed, int nfixed) {
+  if (!h || !verts || !tris || nverts <= 0 || ntris <= 0) return fail(PDIPC_E_ARG, "bad cloth arguments");
+  if (h->hostReady) return fail(PDIPC_E_ARG, "models must be added before setup");
+  h->sims.push_back(make_input(MODEL_CLOTH, verts, nverts, tris, ntris, 3, s_strain, s_bend, s_pos, density, patch_dim,
+                               fixed, nfixed));
+  return PDIPC_OK;
+}
+
+int pdipc_add_tet(pdipc_handle* h, const double* verts, int nverts, const int* tets, int ntets, double s_strain,
+                  double s_volume, double s_pos, double density, int patch_dim, const int* fixed, int nfixed) {
+  if (!h || !verts || !tets || nverts <= 0 || ntets <= 0) return fail(PDIPC_E_ARG, "bad tet arguments");
+  if (h->hostReady) return fail(PDIPC_E_ARG, "models must be added before setup");
+  h->sims.push_back(make_input(MODEL_TET, verts, nverts, tets, ntets, 4, s_strain, s_volume, s_pos, density, patch_dim,
+                               fixed, nfixed));
+  return PDIPC_OK;
+}
+
+int pdipc_add_static(pdipc_handle* h, const double* verts, int nverts, const int* tris, int ntris, int patch_dim) {
+  if (!h || !verts || !tris || nverts <= 0 || ntris <= 0) return fail(PDIPC_E_ARG, "bad static arguments");
+  if (h->hostReady) return fail(PDIPC_E_ARG, "models must be added before setup");
+  h->statics.push_back(make_input(MODEL_STATIC, verts, nverts, tris, ntris, 3, 0, 0, 0, 1e5, patch_dim, nullptr, 0));
+  return PDIPC_OK;
+}
+
+int pdipc_host_setup(pdipc_handle* h) {
+  if (!h) return fail(PDIPC_E_ARG, "null handle");
+  if (h->hostReady) return PDIPC_OK;
+  if (h->sims.empty()) return fail(PDIPC_E_ARG, "no simulated model");
+  StepConstants k;
+  k.dt = h->prm.dt;
+  k.gravity = h->prm.gravity;
+  k.dhat = h->prm.dhat;
+  std::string err;
+  try {
+    err = build_host_setup(h->sims, h->statics, k, h->hs);
+  } catch (const std::exception& e) {
+    err = std::string("host setup: ") + e.what();
+  }
+  if (!err.empty()) return fail(PDIPC_E_BADMESH, err);
+  for (int i = h->hs.nSim; i < h->hs.nTotal; i++) h->hs.colMass[i] = h->prm.static_mass;
+  h->hostReady = true;
+  HostSetup& hs = h->hs;
+  h->cnt.sim_points = hs.nSim;
+  h->cnt.total_points = hs.nTotal;
+  h->cnt.cloth_points = hs.nCloth;
+  h->cnt.tets = hs.nTets;
+  h->cnt.patches = (int)hs.patchStatic.size();
+  h->cnt.boundary_faces = (int)hs.bFaces.size() / 3;
+  h->cnt.boundary_edges = (int)hs.bEdges.size() / 2;
+  h->cnt.nnz_r1 = hs.nnzR1;
+  h->cnt.nnz_r2 = hs.nnzR2;
+  h->cnt.n_2paths = hs.n2paths;
+  return PDIPC_OK;
+}
+
+int pdipc_init(pdipc_handle* h) {
+  int rc = pdipc_host_setup(h);
+  if (rc) return rc;
+  if (h->deviceReady) return PDIPC_OK;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(PDIPC_E_NODEVICE, "no HIP device");
+  HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
+  if ((rc = upload_all(h))) return rc;
+  h->cnt.sell_entries = h->m.sellEntries;
+  h->deviceReady = true;
+  return PDIPC_OK;
+}
+
+int pdipc_step(pdipc_handle* h, int nsteps) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  for (int i = 0; i < nsteps; i++)
+    if ((rc = advance(h))) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PDIPC_OK;
+}
+
+int pdipc_set_precision(pdipc_handle* h, int precision) {
+  if (!h) return fail(PDIPC_E_ARG, "null handle");
+  if (precision != PDIPC_FP64 && precision != PDIPC_FP32_SPLIT) return fail(PDIPC_E_ARG, "unknown precision");
+  h->prm.precision = precision;
+  return PDIPC_OK;
+}
+
+int pdipc_get_x(pdipc_handle* h, double* out, int npoints) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (!out || npoints < h->m.nTotal) return fail(PDIPC_E_ARG, "output holds fewer points than the scene");
+  return get_soa(h, h->m.X, out, h->m.nTotal);
+}
+
+int pdipc_set_x(pdipc_handle* h, const double* in, int npoints) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (!in || npoints < h->m.nSim || npoints > h->m.nTotal) return fail(PDIPC_E_ARG, "npoints must cover the simulated points");
+  return put_soa_prefix(h, h->m.X, in, h->m.nSim);
+}
+
+int pdipc_get_v(pdipc_handle* h, double* out, int npoints) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (!out || npoints < h->m.nSim) return fail(PDIPC_E_ARG, "output holds fewer points than the simulated set");
+  return get_soa(h, h->m.V, out, h->m.nSim);
+}
+
+int pdipc_set_v(pdipc_handle* h, const double* in, int npoints) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (!in || npoints < h->m.nSim) return fail(PDIPC_E_ARG, "npoints must cover the simulated points");
+  return put_soa(h, h->m.V, in, h->m.nSim);
+}
+
+int pdipc_get_counters(pdipc_handle* h, pdipc_counters* out) {
+  if (!h || !out) return fail(PDIPC_E_ARG, "null argument");
+  *out = h->cnt;
+  return PDIPC_OK;
+}
+
+int pdipc_save_frame(pdipc_handle* h, const char* path, int as_double) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  std::vector<double> x(3 * (size_t)h->m.nTotal);
+  if ((rc = get_soa(h, h->m.X, x.data(), h->m.nTotal))) return rc;
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(PDIPC_E_ARG, std::string("cannot open ") + path);
+  int64_t rows = 3, cols = h->m.nTotal;
+  fwrite(&rows, 8, 1, f);
+  fwrite(&cols, 8, 1, f);
+  if (as_double) fwrite(x.data(), sizeof(double), x.size(), f);
+  else {
+    std::vector<float> xf(x.begin(), x.end());
+    fwrite(xf.data(), sizeof(float), xf.size(), f);
+  }
+  fclose(f);
+  return PDIPC_OK;
+}
+
+int pdipc_get_candidates(pdipc_handle* h, int* out, int cap, int* count) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  int K = h->lastRecs;
+  if (count) *count = K;
+  if (!out || cap <= 0 || K == 0) return PDIPC_OK;
+  int n = std::min(K, cap);
+  std::vector<int4> r(n);
+  HIPCHK(hipMemcpyAsync(r.data(), h->m.recs, n * sizeof(int4), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (int i = 0; i < n; i++) {
+    bool ee = r[i].x < 0;
+    out[5 * i] = ee ? 1 : 0;
+    out[5 * i + 1] = ee ? ~r[i].x : r[i].x;
+    out[5 * i + 2] = r[i].y;
+    out[5 * i + 3] = r[i].z;
+    out[5 * i + 4] = r[i].w;
+  }
+  return PDIPC_OK;
+}
+
+int pdipc_get_patch_pairs(pdipc_handle* h, int* out, int cap, int* count) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  int Q = h->lastPairs;
+  if (count) *count = Q;
+  if (!out || cap <= 0 || Q == 0) return PDIPC_OK;
+  int n = std::min(Q, cap);
+  HIPCHK(hipMemcpyAsync(out, h->m.pairs, n * sizeof(int2), hipMemcpyDeviceToHost, h->stream));
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PDIPC_OK;
+}
+
+int pdipc_get_model_dat(pdipc_handle* h, int model, int which, int* out, int cap, int* count) {
+  if (!h) return fail(PDIPC_E_ARG, "null handle");
+  if (!h->hostReady) return fail(PDIPC_E_ARG, "pdipc_host_setup has not run");
+  if (model < 0 || model >= (int)h->hs.boundaryDat.size()) return fail(PDIPC_E_ARG, "model index out of range");
+  const std::vector<int>& d = which == 0 ? h->hs.boundaryDat[model] : h->hs.patchDat[model];
+  if (count) *count = (int)d.size();
+  if (out && cap > 0) std::copy(d.begin(), d.begin() + std::min<size_t>(cap, d.size()), out);
+  return PDIPC_OK;
+}
+
+int pdipc_debug_broad_phase(pdipc_handle* h, const double* xold, const double* dx, int* count) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if ((rc = put_soa(h, h->m.Xold, xold, h->m.nTotal))) return rc;
+  std::vector<double> d(3 * (size_t)h->m.nTotal, 0.0);
+  std::copy(dx, dx + 3 * (size_t)h->m.nSim, d.begin());  // static points never move
+  if ((rc = put_soa(h, h->m.Dir, d.data(), h->m.nTotal))) return rc;
+  if ((rc = broad_phase(h, h->m.Xold, h->m.Dir))) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  if (count) *count = h->lastRecs;
+  return PDIPC_OK;
+}
+
+int pdipc_debug_toi(pdipc_handle* h, const double* x0, const double* x1, double* toi) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if ((rc = put_soa(h, h->m.Xold, x0, h->m.nTotal))) return rc;
+  if ((rc = put_soa(h, h->m.X, x1, h->m.nTotal))) return rc;
+  launch_toi_init(h->m, h->stream);
+  launch_ccd(h->m, h->lastRecs, h->prm.ccd_filter, h->prm.ccd_iters, h->prm.ccd_step, h->prm.ccd_gap,
+             h->prm.touch_policy, h->stream);
+  if ((rc = check_launch(h))) return rc;
+  if ((rc = read_scalars(h))) return rc;
+  double tmin;
+  unsigned long long b = *h->hToi();
+  memcpy(&tmin, &b, 8);
+  *toi = tmin < 1.0 ? tmin * h->prm.toi_scale : 1.0;
+  return PDIPC_OK;
+}
+
+int pdipc_debug_collision(pdipc_handle* h, const double* x, double* diag, double* rhs, int* active) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  DeviceMesh& m = h->m;
+  if ((rc = put_soa(h, m.X, x, m.nTotal))) return rc;
+  launch_zero(m.colDiag, m.nTotal, h->stream);
+  launch_zero(m.colRhs, 3 * (int64_t)m.ld, h->stream);
+  HIPCHK(hipMemsetAsync(m.counters + 2, 0, sizeof(int), h->stream));
+  launch_collision(m, h->lastRecs, h->prm.dhat, h->prm.kappa, h->prm.dt, h->stream);
+  if ((rc = check_launch(h))) return rc;
+  HIPCHK(hipMemcpyAsync(diag, m.colDiag, m.nSim * sizeof(double), hipMemcpyDeviceToHost, h->stream));
+  if ((rc = get_soa(h, m.colRhs, rhs, m.nSim))) return rc;
+  if ((rc = read_scalars(h))) return rc;
+  if (active) *active = h->hCounters()[2];
+  return PDIPC_OK;
+}
+
+int pdipc_debug_local(pdipc_handle* h, const double* x, const double* init_rhs, double* rhs) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  DeviceMesh& m = h->m;
+  if ((rc = put_soa(h, m.X, x, m.nTotal))) return rc;
+  if ((rc = put_soa(h, m.initRhs, init_rhs, m.nSim))) return rc;
+  launch_local(m, h->prm.precision, h->prm.sigma_lo, h->prm.sigma_hi, 0, h->stream);
+  if ((rc = check_launch(h))) return rc;
+  return get_soa(h, m.sysRhs, rhs, m.nSim);
+}
+
+int pdipc_debug_solve(pdipc_handle* h, const double* diag, const double* rhs, const double* x0, double* xout) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  DeviceMesh& m = h->m;
+  HIPCHK(hipMemcpyAsync(m.sysDiag, diag, m.nSim * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  if ((rc = put_soa(h, m.sysRhs, rhs, m.nSim))) return rc;
+  if ((rc = put_soa(h, m.X, x0, m.nTotal))) return rc;
+  launch_operators(m, h->prm.omega, h->prm.precision, h->stream);
+  launch_solve(m, h->prm.omega, h->prm.rho, h->prm.cheb_iters, h->prm.precision, h->stream);
+  if ((rc = check_launch(h))) return rc;
+  return get_soa(h, m.X, xout, m.nSim);
+}
+
+int pdipc_debug_lbvh(pdipc_handle* h, unsigned long long* keys, int* left, int* right, int* parent, int* npatches) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  int P = h->m.nPatches;
+  if (npatches) *npatches = P;
+  if (P <= 0) return PDIPC_OK;
+  if (keys) HIPCHK(hipMemcpyAsync(keys, h->m.keys, P * sizeof(uint64_t), hipMemcpyDeviceToHost, h->stream));
+  if (P > 1) {
+    if (left) HIPCHK(hipMemcpyAsync(left, h->m.nodeLeft, (P - 1) * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (right) HIPCHK(hipMemcpyAsync(right, h->m.nodeRight, (P - 1) * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+    if (parent) HIPCHK(hipMemcpyAsync(parent, h->m.nodeParent, (2 * P - 1) * sizeof(int), hipMemcpyDeviceToHost, h->stream));
+  }
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return PDIPC_OK;
+}
+
+int pdipc_time_kernel(pdipc_handle* h, int which, int reps, double* avg_us, double* bytes) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (reps < 1 || !avg_us) return fail(PDIPC_E_ARG, "reps must be positive");
+  DeviceMesh& m = h->m;
+  const pdipc_params& p = h->prm;
+  hipStream_t s = h->stream;
+  auto launch = [&]() {
+    switch (which) {
+      case 0: launch_one_sweep(m, 0, s); break;
+      case 1: launch_one_sweep(m, 1, s); break;
+      case 2: launch_local(m, p.precision, p.sigma_lo, p.sigma_hi, 0, s); break;
+      case 3: launch_ccd(m, h->lastRecs, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap, p.touch_policy, s); break;
+      default: launch_operators(m, p.omega, p.precision, s); break;
+    }
+  };
+  if (which < 0 || which > 4) return fail(PDIPC_E_ARG, "unknown kernel id");
+  double N = m.nSim, nnz = (double)h->hs.nnzR2, D3 = 3.0 * N;
+  double b = 0;
+  switch (which) {
+    case 0: b = nnz * 12.0 + D3 * 8.0 * 4.0; break;            // col+val FP64; x_k, x_{k-1}, c, x_{k+1}
+    case 1: b = nnz * 8.0 + D3 * 4.0 * 4.0; break;             // col+val FP32; d_k, d_{k-1}, r0, d_{k+1}
+    case 2: {
+      double slots = (double)h->hs.cNbr.size();
+      b = slots * (4 + 8 + 8 + 8) + h->hs.nCloth * 3 * 8.0 * 2 + D3 * 8.0 * 4;  // ring data; a; x, rhs0, rhs
+      b += h->hs.nTets * (16 + 72 + 8 + 96.0);
+      break;
+    }
+    case 3: b = h->lastRecs * (16.0 + 24 * 8.0); break;          // record + x0, x1 of 4 points
+    default: b = (double)h->hs.n2paths * 12.0 + (double)m.sellEntries * (4 + 8 + 16 + 8 + 4); break;
+  }
+  launch();  // warm
+  hipEvent_t e0, e1;
+  HIPCHK(hipEventCreate(&e0));
+  HIPCHK(hipEventCreate(&e1));
+  HIPCHK(hipEventRecord(e0, s));
+  for (int i = 0; i < reps; i++) launch();
+  HIPCHK(hipEventRecord(e1, s));
+  HIPCHK(hipEventSynchronize(e1));
+  float ms = 0;
+  HIPCHK(hipEventElapsedTime(&ms, e0, e1));
+  (void)hipEventDestroy(e0);
+  (void)hipEventDestroy(e1);
+  if ((rc = check_launch(h))) return rc;
+  *avg_us = 1000.0 * ms / reps;
+  if (bytes) *bytes = b;
+  return PDIPC_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,320 @@
+"""ctypes binding of libpdipc.so (include/pdipc.h).
+
+`Simulator` mirrors the reference's PD_IPC::PdIpcSimulator life cycle
+(Simulator/PD-IPC/PdIpcSimulator.h): add models, initialization(), then run()
+per frame, postRun() to read positions back, saveSimulator() to dump a frame.
+It raises when the HIP library is missing or a call fails; there is no
+fallback path.
+"""
+
+import ctypes
+import os
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "libpdipc.so")
+HEADER = os.path.join(os.path.dirname(HERE), "include", "pdipc.h")
+
+FP64 = 0
+FP32_SPLIT = 1
+
+
+class Params(ctypes.Structure):
+    _fields_ = [("dt", ctypes.c_double), ("gravity", ctypes.c_double), ("dhat", ctypes.c_double),
+                ("kappa", ctypes.c_double), ("lg_iters", ctypes.c_int), ("cheb_iters", ctypes.c_int),
+                ("max_outer", ctypes.c_int), ("residual_tol", ctypes.c_double), ("omega", ctypes.c_double),
+                ("rho", ctypes.c_double), ("sigma_lo", ctypes.c_double), ("sigma_hi", ctypes.c_double),
+                ("ccd_filter", ctypes.c_double), ("toi_scale", ctypes.c_double), ("ccd_iters", ctypes.c_int),
+                ("ccd_step", ctypes.c_double), ("ccd_gap", ctypes.c_double), ("static_mass", ctypes.c_double),
+                ("inflation", ctypes.c_double), ("skip_static_pairs", ctypes.c_int), ("touch_policy", ctypes.c_int),
+                ("precision", ctypes.c_int), ("sell_lanes", ctypes.c_int), ("use_graphs", ctypes.c_int)]
+
+
+class Counters(ctypes.Structure):
+    _fields_ = [("outer_iters", ctypes.c_int), ("lg_iters", ctypes.c_int), ("patch_pairs", ctypes.c_int),
+                ("candidates", ctypes.c_int), ("active_pairs", ctypes.c_int), ("toi_min", ctypes.c_double),
+                ("residual", ctypes.c_double), ("nnz_r1", ctypes.c_longlong), ("nnz_r2", ctypes.c_longlong),
+                ("n_2paths", ctypes.c_longlong), ("sell_entries", ctypes.c_longlong), ("sim_points", ctypes.c_int),
+                ("total_points", ctypes.c_int), ("cloth_points", ctypes.c_int), ("tets", ctypes.c_int),
+                ("patches", ctypes.c_int), ("boundary_faces", ctypes.c_int), ("boundary_edges", ctypes.c_int)]
+
+    def as_dict(self):
+        return {name: getattr(self, name) for name, _ in self._fields_}
+
+
+_D = ctypes.POINTER(ctypes.c_double)
+_I = ctypes.POINTER(ctypes.c_int)
+_H = ctypes.c_void_p
+
+# every entry point declared in include/pdipc.h: name -> (restype, argtypes)
+SIGNATURES = {
+    "pdipc_default_params": (None, [ctypes.POINTER(Params)]),
+    "pdipc_create": (ctypes.c_int, [ctypes.POINTER(Params), ctypes.POINTER(_H)]),
+    "pdipc_destroy": (ctypes.c_int, [_H]),
+    "pdipc_last_error": (ctypes.c_char_p, []),
+    "pdipc_add_cloth": (ctypes.c_int, [_H, _D, ctypes.c_int, _I, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                       ctypes.c_double, ctypes.c_double, ctypes.c_int, _I, ctypes.c_int]),
+    "pdipc_add_tet": (ctypes.c_int, [_H, _D, ctypes.c_int, _I, ctypes.c_int, ctypes.c_double, ctypes.c_double,
+                                     ctypes.c_double, ctypes.c_double, ctypes.c_int, _I, ctypes.c_int]),
+    "pdipc_add_static": (ctypes.c_int, [_H, _D, ctypes.c_int, _I, ctypes.c_int, ctypes.c_int]),
+    "pdipc_host_setup": (ctypes.c_int, [_H]),
+    "pdipc_init": (ctypes.c_int, [_H]),
+    "pdipc_step": (ctypes.c_int, [_H, ctypes.c_int]),
+    "pdipc_set_precision": (ctypes.c_int, [_H, ctypes.c_int]),
+    "pdipc_get_x": (ctypes.c_int, [_H, _D, ctypes.c_int]),
+    "pdipc_set_x": (ctypes.c_int, [_H, _D, ctypes.c_int]),
+    "pdipc_get_v": (ctypes.c_int, [_H, _D, ctypes.c_int]),
+    "pdipc_set_v": (ctypes.c_int, [_H, _D, ctypes.c_int]),
+    "pdipc_get_counters": (ctypes.c_int, [_H, ctypes.POINTER(Counters)]),
+    "pdipc_save_frame": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
+    "pdipc_get_candidates": (ctypes.c_int, [_H, _I, ctypes.c_int, _I]),
+    "pdipc_get_patch_pairs": (ctypes.c_int, [_H, _I, ctypes.c_int, _I]),
+    "pdipc_get_model_dat": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int, _I]),
+    "pdipc_debug_broad_phase": (ctypes.c_int, [_H, _D, _D, _I]),
+    "pdipc_debug_toi": (ctypes.c_int, [_H, _D, _D, _D]),
+    "pdipc_debug_collision": (ctypes.c_int, [_H, _D, _D, _D, _I]),
+    "pdipc_debug_local": (ctypes.c_int, [_H, _D, _D, _D]),
+    "pdipc_debug_solve": (ctypes.c_int, [_H, _D, _D, _D, _D]),
+    "pdipc_debug_lbvh": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_ulonglong), _I, _I, _I, _I]),
+    "pdipc_time_kernel": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _D, _D]),
+}
+
+KERNEL_IDS = {"sweep_fp64": 0, "sweep_fp32": 1, "local": 2, "ccd": 3, "operators": 4}
+
+_lib = None
+
+
+class PdipcError(RuntimeError):
+    pass
+
+
+def lib():
+    """Loads the in-tree HIP library; raises if it was not built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise PdipcError(f"{LIB_PATH} is missing: build it with __graft_entry__.build()")
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+        _lib = L
+    return _lib
+
+
+def _check(rc):
+    if rc != 0:
+        msg = lib().pdipc_last_error()
+        raise PdipcError(f"pdipc error {rc}: {msg.decode() if msg else ''}")
+
+
+def _dp(a):
+    return a.ctypes.data_as(_D)
+
+
+def _ip(a):
+    return a.ctypes.data_as(_I)
+
+
+def default_params(**overrides):
+    p = Params()
+    lib().pdipc_default_params(ctypes.byref(p))
+    for k, v in overrides.items():
+        setattr(p, k, v)
+    return p
+
+
+def params_from_scene(scene, **extra):
+    p = default_params()
+    for k, v in scene["params"].items():
+        if hasattr(p, k):
+            setattr(p, k, type(getattr(p, k))(v))
+    for k, v in extra.items():
+        setattr(p, k, v)
+    return p
+
+
+class Simulator:
+    """One simulator handle (PdIpcSimulator)."""
+
+    def __init__(self, params=None):
+        self.h = _H()
+        self.params = params if params is not None else default_params()
+        _check(lib().pdipc_create(ctypes.byref(self.params), ctypes.byref(self.h)))
+        self._keep = []
+        self.total = self.sim = 0
+
+    @classmethod
+    def from_scene(cls, scene, init=True, **extra):
+        s = cls(params_from_scene(scene, **extra))
+        for c in scene["cloths"]:
+            s.add_cloth(c["v"], c["f"], c["s0"], c["s1"], c["s2"], c["density"], c["patch_dim"], c["fixed"])
+        for t in scene["tets"]:
+            s.add_tet(t["v"], t["t"], t["s0"], t["s1"], t["s2"], t["density"], t["patch_dim"], t["fixed"])
+        for st in scene["statics"]:
+            s.add_static(st["v"], st["f"], st["patch_dim"])
+        if init:
+            s.initialization()
+        return s
+
+    def close(self):
+        if self.h:
+            lib().pdipc_destroy(self.h)
+            self.h = _H()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    # --- addModelFromConfigFile / addStaticModelFromConfigFile
+    def add_cloth(self, v, f, strain, bend, pos, density, patch_dim, fixed=()):
+        v = np.ascontiguousarray(v, np.float64)
+        f = np.ascontiguousarray(f, np.int32)
+        fx = np.ascontiguousarray(fixed, np.int32)
+        self._keep += [v, f, fx]
+        _check(lib().pdipc_add_cloth(self.h, _dp(v), len(v), _ip(f), len(f), strain, bend, pos, density,
+                                     int(patch_dim), _ip(fx), len(fx)))
+
+    def add_tet(self, v, t, strain, volume, pos, density, patch_dim, fixed=()):
+        v = np.ascontiguousarray(v, np.float64)
+        t = np.ascontiguousarray(t, np.int32)
+        fx = np.ascontiguousarray(fixed, np.int32)
+        self._keep += [v, t, fx]
+        _check(lib().pdipc_add_tet(self.h, _dp(v), len(v), _ip(t), len(t), strain, volume, pos, density,
+                                   int(patch_dim), _ip(fx), len(fx)))
+
+    def add_static(self, v, f, patch_dim):
+        v = np.ascontiguousarray(v, np.float64)
+        f = np.ascontiguousarray(f, np.int32)
+        self._keep += [v, f]
+        _check(lib().pdipc_add_static(self.h, _dp(v), len(v), _ip(f), len(f), int(patch_dim)))
+
+    # --- initialization()
+    def host_setup(self):
+        _check(lib().pdipc_host_setup(self.h))
+        c = self.counters()
+        self.total, self.sim = c["total_points"], c["sim_points"]
+
+    def initialization(self):
+        _check(lib().pdipc_init(self.h))
+        c = self.counters()
+        self.total, self.sim = c["total_points"], c["sim_points"]
+
+    # --- run(frame) -> advance
+    def run(self, nsteps=1):
+        _check(lib().pdipc_step(self.h, int(nsteps)))
+
+    step = run
+
+    def set_precision(self, precision):
+        _check(lib().pdipc_set_precision(self.h, int(precision)))
+
+    # --- postRun / state
+    def x(self):
+        out = np.zeros((self.total, 3))
+        _check(lib().pdipc_get_x(self.h, _dp(out), self.total))
+        return out
+
+    post_run = x
+
+    def set_x(self, x):
+        x = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1, 3)[: self.sim])
+        _check(lib().pdipc_set_x(self.h, _dp(x), len(x)))
+
+    def v(self):
+        out = np.zeros((self.sim, 3))
+        _check(lib().pdipc_get_v(self.h, _dp(out), self.sim))
+        return out
+
+    def set_v(self, v):
+        v = np.ascontiguousarray(np.asarray(v, np.float64).reshape(-1, 3)[: self.sim])
+        _check(lib().pdipc_set_v(self.h, _dp(v), len(v)))
+
+    def counters(self):
+        c = Counters()
+        _check(lib().pdipc_get_counters(self.h, ctypes.byref(c)))
+        return c.as_dict()
+
+    def save_simulator(self, path, as_double=False):
+        _check(lib().pdipc_save_frame(self.h, path.encode(), 1 if as_double else 0))
+
+    def candidates(self):
+        n = ctypes.c_int()
+        _check(lib().pdipc_get_candidates(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros(5 * max(n.value, 1), np.int32)
+        _check(lib().pdipc_get_candidates(self.h, _ip(out), n.value, ctypes.byref(n)))
+        return out[: 5 * n.value].reshape(-1, 5)
+
+    def patch_pairs(self):
+        n = ctypes.c_int()
+        _check(lib().pdipc_get_patch_pairs(self.h, None, 0, ctypes.byref(n)))
+        out = np.zeros(2 * max(n.value, 1), np.int32)
+        _check(lib().pdipc_get_patch_pairs(self.h, _ip(out), n.value, ctypes.byref(n)))
+        return out[: 2 * n.value].reshape(-1, 2)
+
+    def model_dat(self, model, which):
+        n = ctypes.c_int()
+        _check(lib().pdipc_get_model_dat(self.h, model, which, None, 0, ctypes.byref(n)))
+        out = np.zeros(max(n.value, 1), np.int32)
+        _check(lib().pdipc_get_model_dat(self.h, model, which, _ip(out), n.value, ctypes.byref(n)))
+        return out[: n.value]
+
+    # --- teacher-forced phases
+    def debug_broad_phase(self, xold, dx):
+        a = np.ascontiguousarray(np.asarray(xold, np.float64).reshape(-1))
+        b = np.ascontiguousarray(np.asarray(dx, np.float64).reshape(-1))
+        n = ctypes.c_int()
+        _check(lib().pdipc_debug_broad_phase(self.h, _dp(a), _dp(b), ctypes.byref(n)))
+        return self.candidates()
+
+    def debug_toi(self, x0, x1):
+        a = np.ascontiguousarray(np.asarray(x0, np.float64).reshape(-1))
+        b = np.ascontiguousarray(np.asarray(x1, np.float64).reshape(-1))
+        t = ctypes.c_double()
+        _check(lib().pdipc_debug_toi(self.h, _dp(a), _dp(b), ctypes.byref(t)))
+        return t.value
+
+    def debug_collision(self, x):
+        a = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
+        d = np.zeros(self.sim)
+        r = np.zeros((self.sim, 3))
+        n = ctypes.c_int()
+        _check(lib().pdipc_debug_collision(self.h, _dp(a), _dp(d), _dp(r), ctypes.byref(n)))
+        return d, r, n.value
+
+    def debug_local(self, x, init_rhs):
+        a = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
+        b = np.ascontiguousarray(np.asarray(init_rhs, np.float64).reshape(-1))
+        r = np.zeros((self.sim, 3))
+        _check(lib().pdipc_debug_local(self.h, _dp(a), _dp(b), _dp(r)))
+        return r
+
+    def debug_solve(self, diag, rhs, x0):
+        d = np.ascontiguousarray(diag, np.float64)
+        r = np.ascontiguousarray(np.asarray(rhs, np.float64).reshape(-1))
+        a = np.ascontiguousarray(np.asarray(x0, np.float64).reshape(-1))
+        out = np.zeros((self.sim, 3))
+        _check(lib().pdipc_debug_solve(self.h, _dp(d), _dp(r), _dp(a), _dp(out)))
+        return out
+
+    def time_kernel(self, name, reps=50):
+        """(mean microseconds per launch, algorithmic bytes per launch) of one kernel."""
+        us, b = ctypes.c_double(), ctypes.c_double()
+        _check(lib().pdipc_time_kernel(self.h, KERNEL_IDS[name], int(reps), ctypes.byref(us), ctypes.byref(b)))
+        return us.value, b.value
+
+    def debug_lbvh(self):
+        n = ctypes.c_int()
+        _check(lib().pdipc_debug_lbvh(self.h, None, None, None, None, ctypes.byref(n)))
+        P = n.value
+        keys = np.zeros(max(P, 1), np.uint64)
+        left = np.zeros(max(P - 1, 1), np.int32)
+        right = np.zeros(max(P - 1, 1), np.int32)
+        parent = np.zeros(max(2 * P - 1, 1), np.int32)
+        _check(lib().pdipc_debug_lbvh(self.h, keys.ctypes.data_as(ctypes.POINTER(ctypes.c_ulonglong)), _ip(left),
+                                      _ip(right), _ip(parent), ctypes.byref(n)))
+        return keys[:P], left[: max(P - 1, 0)], right[: max(P - 1, 0)], parent[: max(2 * P - 1, 0)]
