@@ -59,6 +59,9 @@ def lib():
         L.orc_debug_local.argtypes = [vp, _D, _D, _D]
         L.orc_debug_solve.argtypes = [vp, _D, _D, _D, _D]
         L.orc_model_dat.argtypes = [vp, ctypes.c_int, ctypes.c_int, _I]
+        L.orc_sys_matrix.argtypes = [vp, _I, _I, _D]
+        L.orc_boundary_faces.argtypes = [vp, _I]
+        L.orc_patches.argtypes = [vp, _I]
         L.orc_math_strain_projection.argtypes = [_D, ctypes.c_double, ctypes.c_double, _D]
         L.orc_math_svd.argtypes = [_D, _D, _D, _D]
         L.orc_math_pt_dist2.argtypes = [_D, _D, _D, _D, _D]
@@ -199,6 +202,35 @@ class Oracle:
         out = np.zeros(self.sim)
         lib().orc_mass(self.h, _dp(out))
         return out
+
+    def sys_matrix(self):
+        """sysHess = elastic + M/dt^2 as a dense (sim x sim) array (small scenes only)."""
+        nnz = lib().orc_sys_matrix(self.h, None, None, None)
+        off = np.zeros(self.sim + 1, np.int32)
+        cols = np.zeros(max(nnz, 1), np.int32)
+        vals = np.zeros(max(nnz, 1))
+        lib().orc_sys_matrix(self.h, _ip(off), _ip(cols), _dp(vals))
+        A = np.zeros((self.sim, self.sim))
+        for i in range(self.sim):
+            A[i, cols[off[i]:off[i + 1]]] = vals[off[i]:off[i + 1]]
+        return A
+
+    def boundary_faces(self):
+        n = lib().orc_boundary_faces(self.h, None)
+        out = np.zeros(max(n, 1), np.int32)
+        lib().orc_boundary_faces(self.h, _ip(out))
+        return out[:n].reshape(-1, 3)
+
+    def patch_lists(self):
+        n = lib().orc_patches(self.h, None)
+        out = np.zeros(max(n, 1), np.int32)
+        lib().orc_patches(self.h, _ip(out))
+        P, pos, res = int(out[0]), 1, []
+        for _ in range(P):
+            k = int(out[pos])
+            res.append(out[pos + 1: pos + 1 + k].copy())
+            pos += 1 + k
+        return res
 
     def model_dat(self, model_index, which):
         """int32 image of the model's boundary.dat (which=0) or patchInfo.dat (which=1)."""

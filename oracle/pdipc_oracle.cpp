@@ -1674,6 +1674,55 @@ int orc_mass(void* hp, double* out) {
   return with_sim(h, [&](auto& s) { for (int i = 0; i < s.simPoints; i++) out[i] = s.mass[i]; return 0; });
 }
 
+// sysHess = elastic + M/dt^2 as CSR (rows ascending, columns ascending);
+// returns nnz (call with null to size).  Rebuilt from R1 and the diagonal.
+int orc_sys_matrix(void* hp, int* rowOff, int* cols, double* vals) {
+  auto* h = (OracleHandle*)hp;
+  return with_sim(h, [&](auto& s) {
+    int nnz = 0;
+    const double dt = s.prm.v[P_DT];
+    for (int i = 0; i < s.simPoints; i++) {
+      if (rowOff) rowOff[i] = nnz;
+      // R1 row: slot 0 is the diagonal (value 0), the rest are -offdiag
+      std::vector<std::pair<int, double>> row;
+      row.push_back({i, (double)s.elasticDiag[i] + (double)s.mass[i] / (dt * dt)});
+      for (size_t a = 1; a < s.r1Idx[i].size(); a++) row.push_back({s.r1Idx[i][a], -(double)s.r1Val[i][a]});
+      std::sort(row.begin(), row.end());
+      for (auto& e : row) {
+        if (cols) cols[nnz] = e.first;
+        if (vals) vals[nnz] = e.second;
+        nnz++;
+      }
+    }
+    if (rowOff) rowOff[s.simPoints] = nnz;
+    return nnz;
+  });
+}
+// global boundary faces (3 point ids each) and patches (count, then per patch
+// its face count and global face ids); return the int counts
+int orc_boundary_faces(void* hp, int* out) {
+  auto* h = (OracleHandle*)hp;
+  return with_sim(h, [&](auto& s) {
+    if (out)
+      for (size_t i = 0; i < s.bFaces.size(); i++)
+        for (int k = 0; k < 3; k++) out[3 * i + k] = s.bFaces[i][k];
+    return (int)(3 * s.bFaces.size());
+  });
+}
+int orc_patches(void* hp, int* out) {
+  auto* h = (OracleHandle*)hp;
+  return with_sim(h, [&](auto& s) {
+    std::vector<int> b;
+    b.push_back((int)s.patchFaces.size());
+    for (size_t p = 0; p < s.patchFaces.size(); p++) {
+      b.push_back((int)s.patchFaces[p].size());
+      for (int f : s.patchFaces[p]) b.push_back(f);
+    }
+    if (out) std::copy(b.begin(), b.end(), out);
+    return (int)b.size();
+  });
+}
+
 // model-local boundary.dat (PdIpcModel.h:219-251) and patchInfo.dat (:291-318)
 // images for the simulated model `mi`; returns the byte count (call with null to size)
 static int serialize_model(const Model& m, int which, int* out) {
