@@ -75,6 +75,9 @@ def main():
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
     ap.add_argument("--sell-lanes", type=int, default=4)
+    ap.add_argument("--graphs", type=int, default=1, help="replay the local/global loop from a HIP graph")
+    ap.add_argument("--keep-static-pairs", action="store_true",
+                    help="keep static-static patch pairs (the reference's full candidate set)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs the PD-IPC step on one GPU; --gpus {args.gpus} is not supported",
@@ -86,7 +89,9 @@ def main():
     t0 = time.perf_counter()
     sims = {}
     for name, prec in (("fp64", pdipc.FP64), ("fp32_split", pdipc.FP32_SPLIT)):
-        sims[name] = pdipc.Simulator.from_scene(scene, precision=prec, sell_lanes=args.sell_lanes)
+        sims[name] = pdipc.Simulator.from_scene(scene, precision=prec, sell_lanes=args.sell_lanes,
+                                                use_graphs=args.graphs,
+                                                skip_static_pairs=0 if args.keep_static_pairs else 1)
     log(f"setup {time.perf_counter() - t0:.1f} s; counters {sims['fp64'].counters()}")
 
     # warm-up on the FP64 handle, then both precisions time K steps from that state
@@ -158,7 +163,11 @@ def main():
             "fp32_split_speedup_vs_fp64": res["fp64"]["seconds"] / t32,
             "fp32_split_step_error_vs_fp64": max(errs),
             "outer_iters_per_step": res["fp32_split"]["outer"] / steps,
-            "nnz_r2": c["nnz_r2"], "candidates": c["candidates"], "patches": c["patches"],
+            "nnz_r2": c["nnz_r2"], "candidates": c["candidates"], "patch_pairs": c["patch_pairs"],
+            "patches": c["patches"],
+            "static_static_pairs": "kept" if args.keep_static_pairs else
+                                   "skipped (inert: they only write rows the step never reads)",
+            "graphs": bool(args.graphs),
             "parallelism": "single GPU",
         },
         "roofline": {

@@ -72,8 +72,16 @@ def lib():
         L.orc_math_ccd.restype = ctypes.c_double
         L.orc_math_barrier.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_math_barrier.restype = ctypes.c_double
+        L.orc_set_tri_stiffness.argtypes = [vp, ctypes.c_int, _D, ctypes.c_int, _D]
+        L.orc_dual_dirs.restype = ctypes.c_int
+        L.orc_dual_set.argtypes = [vp, ctypes.c_int, _D, _D]
+        L.orc_dual_get.argtypes = [vp, ctypes.c_int, _D, _D]
         _lib = L
     return _lib
+
+
+def dual_dirs():
+    return int(lib().orc_dual_dirs())
 
 
 def _dp(a):
@@ -96,21 +104,36 @@ class Oracle:
     """The CPU restatement of the reference step; `use_float` mimics the
     reference's FP32 build."""
 
-    def __init__(self, scene, use_float=False):
+    def __init__(self, scene, use_float=False, dual=False, tri_k_tangents=None):
+        """`dual` runs the FP64 step on forward-mode dual numbers carrying
+        `dual_dirs()` directional derivatives; `tri_k_tangents` maps a cloth's
+        model index to its (dirs, ntri) stiffness seeds."""
         L = lib()
         prm = np.array([float(scene["params"][k]) for k in PARAM_ORDER])
-        self.h = L.orc_create(_dp(prm), len(prm), 1 if use_float else 0)
+        self.dual = bool(dual)
+        self.h = L.orc_create(_dp(prm), len(prm), 2 if dual else (1 if use_float else 0))
         self._keep = []
+        mi = 0
         for c in scene["cloths"]:
             v, f, fx = _f64(c["v"]), _i32(c["f"]), _i32(c["fixed"])
             self._keep += [v, f, fx]
             L.orc_add_cloth(self.h, _dp(v), len(v), _ip(f), len(f), c["s0"], c["s1"], c["s2"], c["density"],
                             int(c["patch_dim"]), _ip(fx), len(fx))
+            if c.get("tri_k") is not None:
+                k = _f64(c["tri_k"])
+                tg = None
+                if tri_k_tangents is not None and mi in tri_k_tangents:
+                    tg = _f64(tri_k_tangents[mi])
+                    assert tg.shape == (dual_dirs(), len(f))
+                if L.orc_set_tri_stiffness(self.h, mi, _dp(k), len(k), _dp(tg) if tg is not None else None) != 0:
+                    raise ValueError("per-triangle stiffness does not match the cloth")
+            mi += 1
         for t in scene["tets"]:
             v, tt, fx = _f64(t["v"]), _i32(t["t"]), _i32(t["fixed"])
             self._keep += [v, tt, fx]
             L.orc_add_tet(self.h, _dp(v), len(v), _ip(tt), len(tt), t["s0"], t["s1"], t["s2"], t["density"],
                           int(t["patch_dim"]), _ip(fx), len(fx))
+            mi += 1
         for s in scene["statics"]:
             v, f = _f64(s["v"]), _i32(s["f"])
             self._keep += [v, f]
@@ -129,6 +152,20 @@ class Oracle:
 
     def step(self, n=1):
         lib().orc_step(self.h, n)
+
+    # dual mode: values and tangents of the simulated points, tangents (dirs, sim, 3)
+    def dual_set(self, which, val, tangents=None):
+        val = _f64(np.asarray(val).reshape(-1)[: 3 * self.sim])
+        tg = None
+        if tangents is not None:
+            tg = _f64(np.asarray(tangents).reshape(dual_dirs(), -1)[:, : 3 * self.sim])
+        lib().orc_dual_set(self.h, 0 if which == "x" else 1, _dp(val), _dp(tg) if tg is not None else None)
+
+    def dual_get(self, which):
+        val = np.zeros(3 * self.sim)
+        tg = np.zeros((dual_dirs(), 3 * self.sim))
+        lib().orc_dual_get(self.h, 0 if which == "x" else 1, _dp(val), _dp(tg))
+        return val.reshape(-1, 3), tg.reshape(dual_dirs(), -1, 3)
 
     def x(self):
         out = np.zeros(3 * self.total)

@@ -51,6 +51,94 @@
 
 namespace orc {
 
+using std::acos;
+using std::fabs;
+using std::log;
+using std::sqrt;
+using std::tan;
+
+// ---------------------------------------------------------------------------
+// Forward-mode dual numbers: a value and K tangents.  Sim<Dual<K>> runs the
+// same step as Sim<double> -- every value is computed by the same operations in
+// the same order, so the value part is bit-identical -- and carries K
+// directional derivatives of every quantity along.  This is the CPU reference
+// for the derivative of the step: the chain rule is applied mechanically to
+// the restatement, independently of the hand-written adjoint it checks.
+// ---------------------------------------------------------------------------
+template <int K>
+struct Dual {
+  double v;
+  double d[K];
+  Dual() : v(0) { for (int k = 0; k < K; k++) d[k] = 0; }
+  Dual(double a) : v(a) { for (int k = 0; k < K; k++) d[k] = 0; }
+  explicit operator double() const { return v; }
+  explicit operator float() const { return (float)v; }
+  Dual& operator+=(const Dual& o) { v += o.v; for (int k = 0; k < K; k++) d[k] += o.d[k]; return *this; }
+  Dual& operator-=(const Dual& o) { v -= o.v; for (int k = 0; k < K; k++) d[k] -= o.d[k]; return *this; }
+  Dual& operator*=(const Dual& o) { *this = *this * o; return *this; }
+  Dual& operator/=(const Dual& o) { *this = *this / o; return *this; }
+  friend Dual operator+(Dual a, const Dual& b) { return a += b; }
+  friend Dual operator-(Dual a, const Dual& b) { return a -= b; }
+  friend Dual operator-(const Dual& a) { Dual r; r.v = -a.v; for (int k = 0; k < K; k++) r.d[k] = -a.d[k]; return r; }
+  friend Dual operator*(const Dual& a, const Dual& b) {
+    Dual r;
+    r.v = a.v * b.v;
+    for (int k = 0; k < K; k++) r.d[k] = a.d[k] * b.v + a.v * b.d[k];
+    return r;
+  }
+  friend Dual operator/(const Dual& a, const Dual& b) {
+    Dual r;
+    r.v = a.v / b.v;
+    for (int k = 0; k < K; k++) r.d[k] = (a.d[k] - r.v * b.d[k]) / b.v;
+    return r;
+  }
+  friend bool operator<(const Dual& a, const Dual& b) { return a.v < b.v; }
+  friend bool operator>(const Dual& a, const Dual& b) { return a.v > b.v; }
+  friend bool operator<=(const Dual& a, const Dual& b) { return a.v <= b.v; }
+  friend bool operator>=(const Dual& a, const Dual& b) { return a.v >= b.v; }
+  friend bool operator==(const Dual& a, const Dual& b) { return a.v == b.v; }
+  friend bool operator!=(const Dual& a, const Dual& b) { return a.v != b.v; }
+  friend Dual sqrt(const Dual& a) {
+    Dual r;
+    r.v = std::sqrt(a.v);
+    for (int k = 0; k < K; k++) r.d[k] = r.v > 0 ? a.d[k] / (2 * r.v) : 0.0;
+    return r;
+  }
+  friend Dual fabs(const Dual& a) { return a.v < 0 ? -a : a; }
+  friend Dual log(const Dual& a) {
+    Dual r;
+    r.v = std::log(a.v);
+    for (int k = 0; k < K; k++) r.d[k] = a.d[k] / a.v;
+    return r;
+  }
+  friend Dual acos(const Dual& a) {
+    Dual r;
+    r.v = std::acos(a.v);
+    double g = -1.0 / std::sqrt(1.0 - a.v * a.v);
+    for (int k = 0; k < K; k++) r.d[k] = g * a.d[k];
+    return r;
+  }
+  friend Dual tan(const Dual& a) {
+    Dual r;
+    r.v = std::tan(a.v);
+    for (int k = 0; k < K; k++) r.d[k] = (1.0 + r.v * r.v) * a.d[k];
+    return r;
+  }
+};
+constexpr int kDualDirs = 8;
+using DualD = Dual<kDualDirs>;
+
+}  // namespace orc
+
+namespace std {
+template <int K>
+struct numeric_limits<orc::Dual<K>> {
+  static orc::Dual<K> max() { return orc::Dual<K>(numeric_limits<double>::max()); }
+};
+}  // namespace std
+
+namespace orc {
+
 // ---------------------------------------------------------------------------
 // parameters (flat double array; order shared with oracle/oracle.py)
 // ---------------------------------------------------------------------------
@@ -142,6 +230,10 @@ struct Model {
   std::vector<int> T;      // tets
   double s0 = 0, s1 = 0, s2 = 0, density = 1;
   int patchDim = 10;
+  // optional per-triangle strain stiffness (cloth): an edge's spring weight is
+  // the mean over its adjacent triangles (a documented extension; the
+  // reference uses one weight per model)
+  std::vector<double> triK;
   std::vector<int> fixedIds;
   // model-local boundary (PdIpcModel.h:22-37)
   std::vector<int> bPoints;
@@ -222,7 +314,7 @@ static void cloth_set_patches(Model& m) {
       mn[k] = min2(mn[k], P(i, k));
       mx[k] = max2(mx[k], P(i, k));
     }
-  float sq = std::sqrt((float)m.nv);
+  float sq = sqrt((float)m.nv);
   int len = sq < (float)m.patchDim ? (int)sq : m.patchDim;
   float size[3];
   for (int k = 0; k < 3; k++) size[k] = mx[k] - mn[k];
@@ -370,8 +462,8 @@ static void svd3(const R* F, R* U, R* sig, R* V) {
       R apq = A[p][q];
       if (apq == 0) continue;
       R tau = (A[q][q] - A[p][p]) / (2 * apq);
-      R t = (tau >= 0 ? (R)1 : (R)-1) / (std::fabs(tau) + std::sqrt((R)1 + tau * tau));
-      R c = (R)1 / std::sqrt((R)1 + t * t), s = t * c;
+      R t = (tau >= 0 ? (R)1 : (R)-1) / (fabs(tau) + sqrt((R)1 + tau * tau));
+      R c = (R)1 / sqrt((R)1 + t * t), s = t * c;
       for (int k = 0; k < 3; k++) {  // A <- A J  (columns p, q)
         R akp = A[k][p], akq = A[k][q];
         A[k][p] = c * akp - s * akq;
@@ -406,7 +498,7 @@ static void svd3(const R* F, R* U, R* sig, R* V) {
   for (int c = 0; c < 3; c++)
     for (int r = 0; r < 3; r++) b[c][r] = F[r * 3 + 0] * Vm[0][c] + F[r * 3 + 1] * Vm[1][c] + F[r * 3 + 2] * Vm[2][c];
   R u0[3], u1[3], u2[3];
-  R s0 = std::sqrt(dot3(b[0], b[0]));
+  R s0 = sqrt(dot3(b[0], b[0]));
   if (s0 > (R)1e-30) {
     for (int k = 0; k < 3; k++) u0[k] = b[0][k] / s0;
   } else {
@@ -414,16 +506,16 @@ static void svd3(const R* F, R* U, R* sig, R* V) {
   }
   R pr = dot3(u0, b[1]);
   for (int k = 0; k < 3; k++) u1[k] = b[1][k] - pr * u0[k];
-  R s1 = std::sqrt(dot3(u1, u1));
+  R s1 = sqrt(dot3(u1, u1));
   if (s1 > (R)1e-30 * (s0 > 1 ? s0 : (R)1)) {
     for (int k = 0; k < 3; k++) u1[k] /= s1;
   } else {
     // any unit vector orthogonal to u0
     R t[3] = {0, 0, 0};
-    int mi = std::fabs(u0[0]) < std::fabs(u0[1]) ? (std::fabs(u0[0]) < std::fabs(u0[2]) ? 0 : 2) : (std::fabs(u0[1]) < std::fabs(u0[2]) ? 1 : 2);
+    int mi = fabs(u0[0]) < fabs(u0[1]) ? (fabs(u0[0]) < fabs(u0[2]) ? 0 : 2) : (fabs(u0[1]) < fabs(u0[2]) ? 1 : 2);
     t[mi] = 1;
     cross3(u0, t, u1);
-    R n = std::sqrt(dot3(u1, u1));
+    R n = sqrt(dot3(u1, u1));
     for (int k = 0; k < 3; k++) u1[k] /= n;
     s1 = 0;
   }
@@ -594,11 +686,11 @@ static R ccd_toi(int type, R x[4][3], R dx[4][3], int iters, R step, R gapf, int
   if (type == 0) {
     R m2 = max2(dot3(dx[1], dx[1]), dot3(dx[2], dx[2]));
     m2 = max2(m2, dot3(dx[3], dx[3]));
-    maxdisp = std::sqrt(dot3(dx[0], dx[0])) + std::sqrt(m2);
+    maxdisp = sqrt(dot3(dx[0], dx[0])) + sqrt(m2);
   } else {
     R m0 = max2(dot3(dx[0], dx[0]), dot3(dx[1], dx[1]));
     R m1 = max2(dot3(dx[2], dx[2]), dot3(dx[3], dx[3]));
-    maxdisp = std::sqrt(m0) + std::sqrt(m1);
+    maxdisp = sqrt(m0) + sqrt(m1);
   }
   if (maxdisp == 0) return 1;
   auto dfun = [&](R* dist2out) -> R {  // returns dFunc; *dist2out = dist2 used for sqrt
@@ -621,7 +713,7 @@ static R ccd_toi(int type, R x[4][3], R dx[4][3], int iters, R step, R gapf, int
   };
   R d2;
   R dF = dfun(&d2);
-  R dcur = std::sqrt(d2);
+  R dcur = sqrt(d2);
   if (touch_policy == 1 && !(dcur > 0)) return 0;
   R gap = gapf * dF / dcur;  // = gapf * dcur
   R toc = 0;
@@ -630,7 +722,7 @@ static R ccd_toi(int type, R x[4][3], R dx[4][3], int iters, R step, R gapf, int
     for (int j = 0; j < 4; j++)
       for (int k = 0; k < 3; k++) x[j][k] += tlb * dx[j][k];
     dF = dfun(&d2);
-    dcur = std::sqrt(d2);
+    dcur = sqrt(d2);
     if (touch_policy == 1 && !(dcur > 0)) return toc;
     if (toc != 0 && (dF / dcur) < gap) break;
     toc += tlb;
@@ -645,7 +737,7 @@ static R barrier_value(R d2, R dh2, R kappa) {
   R e = 0;
   if (d2 < dh2) {
     R t2 = d2 - dh2;
-    e = -kappa * t2 * t2 * std::log(d2 / dh2) / (dh2 * dh2);
+    e = -kappa * t2 * t2 * log(d2 / dh2) / (dh2 * dh2);
   }
   return e;
 }
@@ -664,6 +756,8 @@ struct Sim {
   Params prm;
   std::vector<Model> models;   // simulated, in add order
   std::vector<Model> statics;  // static obstacles
+  // per simulated model: per-triangle strain stiffness (empty = the model's s0)
+  std::vector<std::vector<R>> triStiffness;
   bool initialized = false;
 
   int totalPoints = 0, simPoints = 0, clothPoints = 0, nTets = 0;
@@ -712,14 +806,15 @@ struct Sim {
     totalPoints = 0;
     clothPoints = 0;
     // per-model init, simulated models first (PdIpcSimulator.cpp:69-124)
-    for (auto& m : models) {
+    for (size_t mi = 0; mi < models.size(); mi++) {
+      Model& m = models[mi];
       int base = totalPoints;
       if (m.kind == KIND_CLOTH) {
         m.topo = build_surface_topo(m.nv, m.F);
         cloth_register_boundary(m);
         cloth_set_patches(m);
         post_process_boundary(m);
-        init_cloth(m, base, posCons, posK);
+        init_cloth(m, (int)mi, base, posCons, posK);
         clothPoints += m.nv;
       } else {
         tet_register_boundary(m);
@@ -779,7 +874,7 @@ struct Sim {
         inv[2][0] = (E[1][0] * E[2][1] - E[1][1] * E[2][0]) / det;
         inv[2][1] = (E[0][1] * E[2][0] - E[0][0] * E[2][1]) / det;
         inv[2][2] = (E[0][0] * E[1][1] - E[0][1] * E[1][0]) / det;
-        R vol = std::fabs(det / (R)6.0);
+        R vol = fabs(det / (R)6.0);
         R w = tetK[i] * vol;
         for (int k = 0; k < 4; k++) tets.push_back(ele[k]);
         tetW.push_back(w);
@@ -804,8 +899,8 @@ struct Sim {
     nTets = (int)tetW.size();
     for (size_t i = 0; i < cNbr.size(); i++) {
       int pi = cNbr[i][0];
-      R st = cStrain[i][0];
       for (size_t k = 1; k < cNbr[i].size(); k++) {
+        R st = cStrain[i][k];
         H[pi][pi] += st;
         H[pi][cNbr[i][k]] += -st;
       }
@@ -834,7 +929,7 @@ struct Sim {
   }
 
   // PdIpcClothModel::init (PdIpcClothModel.h:142-343)
-  void init_cloth(Model& m, int base, std::vector<std::vector<int>>& posCons, std::vector<R>& posK) {
+  void init_cloth(Model& m, int modelIndex, int base, std::vector<std::vector<int>>& posCons, std::vector<R>& posK) {
     int nv = m.nv, nf = (int)m.F.size() / 3, ne = (int)m.topo.edges.size();
     auto P = [&](int i, R* p) { p[0] = (R)m.V[3 * i]; p[1] = (R)m.V[3 * i + 1]; p[2] = (R)m.V[3 * i + 2]; };
     std::vector<R> pArea(nv, 0);
@@ -842,7 +937,7 @@ struct Sim {
       R a[3], b[3], c[3], e1[3], e2[3], n[3];
       P(m.F[3 * f], a); P(m.F[3 * f + 1], b); P(m.F[3 * f + 2], c);
       sub3(b, a, e1); sub3(c, a, e2); cross3(e1, e2, n);
-      R area = (R)0.5 * std::fabs(std::sqrt(dot3(n, n)));
+      R area = (R)0.5 * fabs(sqrt(dot3(n, n)));
       for (int k = 0; k < 3; k++) pArea[m.F[3 * f + k]] += area / 3;
     }
     for (int v = 0; v < nv; v++) {
@@ -866,9 +961,9 @@ struct Sim {
         R p[3], x10[3], x20[3];
         P(opp, p);
         sub3(p1, p, x10); sub3(p2, p, x20);
-        R l10 = std::sqrt(dot3(x10, x10)), l20 = std::sqrt(dot3(x20, x20));
-        R th = std::acos(dot3(x10, x20) / (l10 * l20));
-        R ct = (R)1.0 / std::tan(th);
+        R l10 = sqrt(dot3(x10, x10)), l20 = sqrt(dot3(x20, x20));
+        R th = acos(dot3(x10, x20) / (l10 * l20));
+        R ct = (R)1.0 / tan(th);
         cot[e] += (R)-0.5 * ct;
       }
     }
@@ -890,7 +985,13 @@ struct Sim {
         R pe[3], d[3];
         P(ve, pe);
         sub3(pv, pe, d);
-        rest[k + 1] = std::sqrt(dot3(d, d));
+        rest[k + 1] = sqrt(dot3(d, d));
+        if (!triStiffness.empty() && !triStiffness[modelIndex].empty()) {
+          const auto& fs = m.topo.edgeFaces[e];
+          R acc = 0;
+          for (int fid : fs) acc += triStiffness[modelIndex][fid];
+          st[k + 1] = acc / (R)(double)fs.size();
+        }
         R ew = cot[e] / pArea[v];
         sum += ew;
         lbo[k + 1] = ew;
@@ -907,7 +1008,7 @@ struct Sim {
       cBend.push_back(bd);
       cRest.push_back(rest);
       cLbo.push_back(lbo);
-      cH0.push_back(std::sqrt(dot3(curv, curv)));
+      cH0.push_back(sqrt(dot3(curv, curv)));
     }
     std::vector<int> pc;
     for (int f : m.fixedIds) pc.push_back(f + base);
@@ -928,7 +1029,7 @@ struct Sim {
         for (int k = 0; k < 3; k++) E[r][k] = (R)m.V[3 * m.T[4 * e + r + 1] + k] - (R)m.V[3 * m.T[4 * e] + k];
       R det = E[0][0] * (E[1][1] * E[2][2] - E[1][2] * E[2][1]) - E[0][1] * (E[1][0] * E[2][2] - E[1][2] * E[2][0]) +
               E[0][2] * (E[1][0] * E[2][1] - E[1][1] * E[2][0]);
-      R tv = std::fabs(det) / (R)6.0;
+      R tv = fabs(det) / (R)6.0;
       for (int k = 0; k < 4; k++) vol[m.T[4 * e + k]] += tv;
       list[e] = {m.T[4 * e] + base, m.T[4 * e + 1] + base, m.T[4 * e + 2] + base, m.T[4 * e + 3] + base};
     }
@@ -960,7 +1061,7 @@ struct Sim {
     // so row i's off-diagonal columns arrive in ascending order.
     for (int i = 0; i < n; i++)
       for (auto& kv : H[i]) {
-        if (std::fabs(kv.second) < IG) continue;
+        if (fabs(kv.second) < IG) continue;
         if (kv.first != i) {
           r1Idx[i].push_back(kv.first);
           r1Val[i].push_back(-kv.second);
@@ -997,7 +1098,7 @@ struct Sim {
       for (int j : touched) {
         R value = acc[j];
         seen[j] = 0;
-        if (std::fabs(value) < IG) continue;
+        if (fabs(value) < IG) continue;
         // 2-path list for (i, j) (PdIpcSimulator.cpp:732-765)
         std::vector<int> ks;
         std::vector<R> as;
@@ -1006,7 +1107,7 @@ struct Sim {
           for (size_t c = 0; c < r1Idx[j].size(); c++) {
             if (rid != r1Idx[j][c]) continue;
             R ajk = r1Val[i][r] * r1Val[j][c];
-            if (std::fabs(ajk) < IG) continue;
+            if (fabs(ajk) < IG) continue;
             ks.push_back(rid);
             as.push_back(ajk);
           }
@@ -1142,17 +1243,16 @@ struct Sim {
       R tx[3] = {0, 0, 0};
       R lbo = cLbo[tid][0];
       R aq[3] = {xi[0] * lbo, xi[1] * lbo, xi[2] * lbo};
-      R strain = cStrain[tid][0];
       for (size_t i = 1; i < nb.size(); i++) {
         const R* nx = &x[3 * nb[i]];
         R d[3] = {xi[0] - nx[0], xi[1] - nx[1], xi[2] - nx[2]};
-        R cur = std::sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        R rc = strain * (cRest[tid][i] / cur);
+        R cur = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        R rc = cStrain[tid][i] * (cRest[tid][i] / cur);
         for (int k = 0; k < 3; k++) tx[k] += rc * d[k];
         lbo = cLbo[tid][i];
         for (int k = 0; k < 3; k++) aq[k] += nx[k] * lbo;
       }
-      R an = std::sqrt(dot3(aq, aq));
+      R an = sqrt(dot3(aq, aq));
       if (an >= (R)1e-10) {
         R c = cH0[tid] / an;
         for (int k = 0; k < 3; k++) aq[k] *= c;
@@ -1375,7 +1475,7 @@ struct Sim {
         R a, b, cc;
         d2 = vertex_triangle_sqd(x0, x1, x2, x3, &a, &b, &cc, dir);
         if (d2 >= dHat2) continue;
-        dist = std::sqrt(d2);
+        dist = sqrt(d2);
         mp = colMass[t0];
         mq = a * colMass[t1] + b * colMass[t2] + cc * colMass[t3];
         w = barrier_value(d2, dHat2, kappa) + min2(mp, mq) / (dt * dt);
@@ -1385,7 +1485,7 @@ struct Sim {
         dist = dHat - dist;
         dp = dist * rp;
         dq = dist * rq;
-        R len = std::sqrt(dot3(dir, dir));
+        R len = sqrt(dot3(dir, dir));
         for (int k = 0; k < 3; k++) dir[k] /= len;
         for (int k = 0; k < 3; k++) {
           colRhs[3 * t0 + k] += w * (x0[k] + dp * dir[k]);
@@ -1399,14 +1499,14 @@ struct Sim {
         R r, s;
         d2 = edge_edge_sqd(x0, x1, x2, x3, &r, &s, dir);
         if (d2 >= dHat2) continue;
-        dist = std::sqrt(d2);
+        dist = sqrt(d2);
         mp = (1 - r) * colMass[t0] + r * colMass[t1];
         mq = (1 - s) * colMass[t2] + s * colMass[t3];
         w = barrier_value(d2, dHat2, kappa) + min2(mp, mq) / (dt * dt);
         colDiag[t0] += w; colDiag[t1] += w; colDiag[t2] += w; colDiag[t3] += w;
         rp = mq / (mp + mq);
         rq = mp / (mp + mq);
-        R len = std::sqrt(dot3(dir, dir));
+        R len = sqrt(dot3(dir, dir));
         for (int k = 0; k < 3; k++) dir[k] /= len;
         dist = dHat - dist;
         dp = dist * rp;
@@ -1437,7 +1537,7 @@ struct Sim {
       for (int i = 0; i < 3 * simPoints; i++) Dir[i] = X[i] - Xold[i];
       if (outer == 0) broad_phase(Xold, Dir);
       R toi = compute_toi(Xold, X);
-      if (toi < cnt.toiMin) cnt.toiMin = toi;
+      if (toi < cnt.toiMin) cnt.toiMin = (double)toi;
       for (int i = 0; i < 3 * simPoints; i++) X[i] = Xold[i] + toi * Dir[i];
       for (int i = 0; i < simPoints; i++) colDiag[i] = 0;
       for (int i = 0; i < 3 * simPoints; i++) colRhs[i] = 0;
@@ -1463,16 +1563,19 @@ struct Sim {
 // ---------------------------------------------------------------------------
 using namespace orc;
 
+// mode 0: FP64 oracle; 1: FP32 mimic; 2: FP64 with kDualDirs forward-mode tangents
 struct OracleHandle {
   int useFloat = 0;
   Sim<double> d;
   Sim<float> f;
+  Sim<DualD> z;
   std::string err;
 };
 
 template <class F>
 static auto with_sim(OracleHandle* h, F&& fn) {
-  if (h->useFloat) return fn(h->f);
+  if (h->useFloat == 1) return fn(h->f);
+  if (h->useFloat == 2) return fn(h->z);
   return fn(h->d);
 }
 
@@ -1485,6 +1588,7 @@ void* orc_create(const double* params, int nparams, int useFloat) {
   for (int i = 0; i < nparams && i < P_COUNT; i++) p.v[i] = params[i];
   h->d.prm = p;
   h->f.prm = p;
+  h->z.prm = p;
   return h;
 }
 
@@ -1509,6 +1613,7 @@ int orc_add_cloth(void* hp, const double* v, int nv, const int* f, int nf, doubl
   Model m = make_model(KIND_CLOTH, v, nv, f, nf, 3, s0, s1, s2, density, patchDim, fixed, nfixed);
   h->d.models.push_back(m);
   h->f.models.push_back(m);
+  h->z.models.push_back(m);
   return 0;
 }
 
@@ -1518,6 +1623,7 @@ int orc_add_tet(void* hp, const double* v, int nv, const int* t, int nt, double 
   Model m = make_model(KIND_TET, v, nv, t, nt, 4, s0, s1, s2, density, patchDim, fixed, nfixed);
   h->d.models.push_back(m);
   h->f.models.push_back(m);
+  h->z.models.push_back(m);
   return 0;
 }
 
@@ -1526,6 +1632,56 @@ int orc_add_static(void* hp, const double* v, int nv, const int* f, int nf, int 
   Model m = make_model(KIND_STATIC, v, nv, f, nf, 3, 0, 0, 0, 1e5, patchDim, nullptr, 0);
   h->d.statics.push_back(m);
   h->f.statics.push_back(m);
+  h->z.statics.push_back(m);
+  return 0;
+}
+
+// per-triangle strain stiffness of simulated model `mi` (a cloth), before
+// orc_init; `tangents` (kDualDirs x ntri, row per direction) seeds the dual mode
+int orc_set_tri_stiffness(void* hp, int mi, const double* k, int ntri, const double* tangents) {
+  auto* h = (OracleHandle*)hp;
+  if (mi < 0 || mi >= (int)h->d.models.size() || h->d.models[mi].kind != KIND_CLOTH) return -1;
+  if (ntri * 3 != (int)h->d.models[mi].F.size()) return -2;
+  auto put = [&](auto& s) {
+    using R = typename std::remove_reference<decltype(s.X)>::type::value_type;
+    s.triStiffness.resize(s.models.size());
+    s.triStiffness[mi].assign(ntri, (R)0);
+    for (int t = 0; t < ntri; t++) s.triStiffness[mi][t] = (R)k[t];
+  };
+  put(h->d);
+  put(h->f);
+  put(h->z);
+  if (tangents)
+    for (int t = 0; t < ntri; t++)
+      for (int q = 0; q < kDualDirs; q++) h->z.triStiffness[mi][t].d[q] = tangents[(size_t)q * ntri + t];
+  return 0;
+}
+
+int orc_dual_dirs(void) { return kDualDirs; }
+
+// dual mode: values and tangents (kDualDirs x 3*sim, row per direction) of x or v
+int orc_dual_set(void* hp, int which, const double* val, const double* tangents) {
+  auto* h = (OracleHandle*)hp;
+  auto& s = h->z;
+  std::vector<DualD>& a = which == 0 ? s.X : s.Vel;
+  int n = 3 * s.simPoints;
+  for (int i = 0; i < n; i++) {
+    a[i] = DualD(val[i]);
+    if (tangents)
+      for (int q = 0; q < kDualDirs; q++) a[i].d[q] = tangents[(size_t)q * n + i];
+  }
+  return 0;
+}
+int orc_dual_get(void* hp, int which, double* val, double* tangents) {
+  auto* h = (OracleHandle*)hp;
+  auto& s = h->z;
+  const std::vector<DualD>& a = which == 0 ? s.X : s.Vel;
+  int n = 3 * s.simPoints;
+  for (int i = 0; i < n; i++) {
+    val[i] = a[i].v;
+    if (tangents)
+      for (int q = 0; q < kDualDirs; q++) tangents[(size_t)q * n + i] = a[i].d[q];
+  }
   return 0;
 }
 
@@ -1628,8 +1784,8 @@ int orc_debug_collision(void* hp, const double* x, double* diag, double* rhs) {
     std::fill(s.colDiag.begin(), s.colDiag.end(), (R)0);
     std::fill(s.colRhs.begin(), s.colRhs.end(), (R)0);
     s.collision_constraints(a, (R)s.prm.v[P_DT]);
-    for (int i = 0; i < s.simPoints; i++) diag[i] = s.colDiag[i];
-    for (int i = 0; i < 3 * s.simPoints; i++) rhs[i] = s.colRhs[i];
+    for (int i = 0; i < s.simPoints; i++) diag[i] = (double)s.colDiag[i];
+    for (int i = 0; i < 3 * s.simPoints; i++) rhs[i] = (double)s.colRhs[i];
     return s.cnt.activePairs;
   });
 }
@@ -1644,7 +1800,7 @@ int orc_debug_local(void* hp, const double* x, const double* initRhs, double* rh
     s.activeCollision = false;
     s.local_projection(a, out);
     s.activeCollision = act;
-    for (int i = 0; i < 3 * s.simPoints; i++) rhs[i] = out[i];
+    for (int i = 0; i < 3 * s.simPoints; i++) rhs[i] = (double)out[i];
     return 0;
   });
 }
@@ -1661,17 +1817,17 @@ int orc_debug_solve(void* hp, const double* diag, const double* rhs, const doubl
     std::vector<R>* cur = &A;
     std::vector<R>* next = &C;
     s.cheb_solve(pre, cur, next);
-    for (int i = 0; i < 3 * s.simPoints; i++) xout[i] = (*cur)[i];
+    for (int i = 0; i < 3 * s.simPoints; i++) xout[i] = (double)(*cur)[i];
     return 0;
   });
 }
 int orc_elastic_diag(void* hp, double* out) {
   auto* h = (OracleHandle*)hp;
-  return with_sim(h, [&](auto& s) { for (int i = 0; i < s.simPoints; i++) out[i] = s.elasticDiag[i]; return 0; });
+  return with_sim(h, [&](auto& s) { for (int i = 0; i < s.simPoints; i++) out[i] = (double)s.elasticDiag[i]; return 0; });
 }
 int orc_mass(void* hp, double* out) {
   auto* h = (OracleHandle*)hp;
-  return with_sim(h, [&](auto& s) { for (int i = 0; i < s.simPoints; i++) out[i] = s.mass[i]; return 0; });
+  return with_sim(h, [&](auto& s) { for (int i = 0; i < s.simPoints; i++) out[i] = (double)s.mass[i]; return 0; });
 }
 
 // sysHess = elastic + M/dt^2 as CSR (rows ascending, columns ascending);

@@ -23,6 +23,10 @@ struct DeviceMesh {
   double *mass = nullptr, *invMass = nullptr, *gravY = nullptr, *fixK = nullptr, *colMass = nullptr;
   int* fixFlag = nullptr;
   double *inertialDiag = nullptr, *elasticDiag = nullptr, *sysDiag = nullptr, *colDiag = nullptr;
+  // rows whose diagonal changed since the last M^2 update; dStale forces rows
+  // (all, after the host wrote sysDiag directly); dirty = changed rows and
+  // their R1 neighbours (the rows whose M^2 entries depend on a changed D)
+  uint8_t *dStale = nullptr, *dChanged = nullptr, *dDirty = nullptr;
   // cloth
   int *cCenter = nullptr, *cOff = nullptr, *cNbr = nullptr, *clothOfPoint = nullptr;
   double *cRest = nullptr, *cLbo = nullptr, *cBendIn = nullptr, *cStrain = nullptr, *cH0 = nullptr;
@@ -60,9 +64,10 @@ struct DeviceMesh {
   int *idxA = nullptr, *idxB = nullptr;
   uint64_t* keys = nullptr;        // sorted (code << 32 | patch)
   int *radixHist = nullptr, *radixBlockSums = nullptr;
-  int *nodeLeft = nullptr, *nodeRight = nullptr, *nodeParent = nullptr, *nodeFlag = nullptr;
+  int *nodeLeft = nullptr, *nodeRight = nullptr, *nodeParent = nullptr;
+  int2* nodeRange = nullptr;       // leaf range [first, last] of each internal node
+  double* nodeTable = nullptr;     // sparse table of leaf-box unions, levels x [6][P]
   double* nodeBox = nullptr;       // SoA [6][2P-1]
-  int *pairCount = nullptr, *pairOff = nullptr;
   int2* pairs = nullptr;
   int pairCap = 0;
   int *recCount = nullptr, *recOff = nullptr;
@@ -104,8 +109,8 @@ void launch_collision(const DeviceMesh& m, int nRecs, double dhat, double kappa,
 // pair and record totals (host sync) and whether the capacities sufficed.
 void launch_patch_boxes(const DeviceMesh& m, const double* x, const double* dx, double inflation, hipStream_t s);
 void launch_lbvh(DeviceMesh& m, hipStream_t s);
-void launch_pair_count(const DeviceMesh& m, int skipStatic, hipStream_t s);
-void launch_pair_fill(const DeviceMesh& m, int skipStatic, hipStream_t s);
+// appends overlapping patch pairs, count in counters[0]
+void launch_pairs(const DeviceMesh& m, int skipStatic, hipStream_t s);
 void launch_rec_count(const DeviceMesh& m, int nPairs, hipStream_t s);
 void launch_rec_fill(const DeviceMesh& m, int nPairs, int nRecs, hipStream_t s);
 void launch_exclusive_scan(int* data_in, int* data_out, int n, int* total_out, int* blockScratch, hipStream_t s);

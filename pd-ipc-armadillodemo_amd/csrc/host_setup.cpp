@@ -291,6 +291,7 @@ struct Triplet { int r, c; double v; };
 std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vector<ModelInput>& statics,
                              const StepConstants& k, HostSetup& hs) {
   hs = HostSetup();
+  hs.clothTriBase.assign(sims.size(), -1);
   const double dt = k.dt;
   std::vector<LocalTopo> topo(sims.size() + statics.size());
   std::vector<int> base(sims.size() + statics.size());
@@ -336,6 +337,23 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
           if (efCount[e] < 2) efList[e][efCount[e]] = f;
           efCount[e]++;
         }
+      // spring weight per edge: s0, or the mean stiffness of the adjacent triangles
+      std::vector<double> springW(ne, m.s0);
+      int triBase = hs.nClothTris;
+      int edgeBase = (int)hs.edgeTriOff.size();
+      if (!m.triK.empty() && (int)m.triK.size() != nf) return "per-triangle stiffness does not match the cloth";
+      for (int e = 0; e < ne; e++) {
+        int c = std::min(efCount[e], 2);
+        if (!m.triK.empty()) {
+          double acc = 0;
+          for (int q = 0; q < c; q++) acc += m.triK[efList[e][q]];
+          springW[e] = acc / c;
+        }
+        hs.edgeTriOff.push_back((int)hs.edgeTriList.size());
+        for (int q = 0; q < c; q++) hs.edgeTriList.push_back(triBase + efList[e][q]);
+      }
+      hs.clothTriBase[mi] = triBase;
+      hs.nClothTris += nf;
       std::vector<double> cotw(ne, 0.0);
       for (int e = 0; e < ne; e++) {
         int a = (int)(edgeKeys[e] >> 32), b = (int)(edgeKeys[e] & 0xffffffffu);
@@ -377,6 +395,8 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
         double sum = 0, curv[3] = {0, 0, 0};
         hs.cNbr.push_back(base[mi] + v);
         hs.cRest.push_back(0.0);
+        hs.cStrain.push_back(m.s0);
+        hs.cSlotEdge.push_back(-1);
         for (int q = 0; q < nb; q++) {
           int e = ringE[ringOff[v] + q];
           int a = (int)(edgeKeys[e] >> 32), b = (int)(edgeKeys[e] & 0xffffffffu);
@@ -385,6 +405,8 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
           sub3(V + 3 * v, V + 3 * ve, d);
           hs.cNbr.push_back(base[mi] + ve);
           hs.cRest.push_back(std::sqrt(dot3(d, d)));
+          hs.cStrain.push_back(springW[e]);
+          hs.cSlotEdge.push_back(edgeBase + e);
           double ew = cotw[e] / area[v];
           sum += ew;
           lbo[q + 1] = ew;
@@ -395,15 +417,16 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
           for (int j = 0; j < 3; j++) curv[j] += lbo[q] * V[3 * pid + j];
         }
         hs.cLbo.insert(hs.cLbo.end(), lbo.begin(), lbo.end());
-        hs.cStrain.push_back(m.s0);
+        for (int q = 0; q <= nb; q++) hs.cBendOut.push_back(lbo[q] * m.s1 * area[v]);
         hs.cH0.push_back(std::sqrt(dot3(curv, curv)));
         hs.cOff.push_back((int)hs.cNbr.size());
         lboRows[v] = lbo;
         // PD system: springs (PdIpcSimulator.cpp:358-369) and bending area*bend * l l^T (:371-396)
         int p0 = base[mi] + v;
         for (int q = 0; q < nb; q++) {
-          trip.push_back({p0, p0, m.s0});
-          trip.push_back({p0, hs.cNbr[hs.cOff[hs.cOff.size() - 2] + q + 1], -m.s0});
+          double w = springW[ringE[ringOff[v] + q]];
+          trip.push_back({p0, p0, w});
+          trip.push_back({p0, hs.cNbr[hs.cOff[hs.cOff.size() - 2] + q + 1], -w});
         }
         double ba = area[v] * m.s1;
         int o = hs.cOff[hs.cOff.size() - 2];
@@ -416,6 +439,7 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
         int o = hs.cOff[cbase + v];
         int nb = ringOff[v + 1] - ringOff[v];
         hs.cBendIn.push_back(lboRows[v][0] * m.s1 * area[v]);
+        hs.cLboIn.push_back(lboRows[v][0]);
         for (int q = 0; q < nb; q++) {
           int kv = hs.cNbr[o + q + 1] - base[mi];
           // slot of v in k's ring
@@ -426,6 +450,7 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
             if ((a == v && b == kv) || (b == v && a == kv)) { slot = z + 1; break; }
           }
           hs.cBendIn.push_back(lboRows[kv][slot] * m.s1 * area[kv]);
+          hs.cLboIn.push_back(lboRows[kv][slot]);
         }
       }
       for (int f : m.fixed) {
@@ -574,6 +599,13 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
     hs.r1Off.push_back((int)hs.r1Col.size());
   }
   hs.nnzR1 = (int64_t)hs.r1Col.size() + hs.nSim;
+  hs.cSlotR1.assign(hs.cNbr.size(), -1);
+  for (size_t c = 0; c < hs.cCenter.size(); c++) {
+    int i = hs.cCenter[c];
+    for (int sl = hs.cOff[c] + 1; sl < hs.cOff[c + 1]; sl++)
+      for (int q = hs.r1Off[i]; q < hs.r1Off[i + 1]; q++)
+        if (hs.r1Col[q] == hs.cNbr[sl]) { hs.cSlotR1[sl] = q; break; }
+  }
   // ---- R2 = R1 * R1 value pattern, then the 2-path lists (PdIpcSimulator.cpp:704-796)
   {
     int n = hs.nSim;
@@ -613,6 +645,8 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
             if (!(std::fabs(ajk) < IG)) {
               hs.p2K.push_back(hs.r1Col[a]);
               hs.p2A.push_back(ajk);
+              hs.p2Ia.push_back(a);
+              hs.p2Ib.push_back(b);
             }
             a++; b++;
           }
@@ -620,9 +654,11 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
         if ((int64_t)hs.p2K.size() == before) continue;  // no valid 2-path: entry dropped
         hs.m2Col.push_back(j);
         double r1v = 0.0;
+        int r1q = -1;
         for (int q = hs.r1Off[i]; q < hs.r1Off[i + 1]; q++)
-          if (hs.r1Col[q] == j) { r1v = hs.r1Val[q]; break; }
+          if (hs.r1Col[q] == j) { r1v = hs.r1Val[q]; r1q = q; break; }
         hs.m2R1.push_back(r1v);
+        hs.m2R1Idx.push_back(r1q);
         hs.p2Off.push_back((int64_t)hs.p2K.size());
       }
       hs.m2Off.push_back((int)hs.m2Col.size());
@@ -701,6 +737,7 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
   L.col.assign(total, 0);
   L.rowFlag.assign(total, 0);
   L.r1.assign(total, 0.0);
+  L.r1Idx.assign(total, -1);
   std::vector<int64_t> src(total, -1);  // CSR entry feeding each SELL slot
   for (int s = 0; s < L.nSlices; s++)
     for (int lr = 0; lr < L.rowsPerSlice; lr++) {
@@ -715,6 +752,7 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
           L.col[e] = hs.m2Col[q];
           L.rowFlag[e] = r | (hs.m2Col[q] == r ? (int)0x80000000 : 0);
           L.r1[e] = hs.m2R1[q];
+          L.r1Idx[e] = hs.m2R1Idx[q];
           src[e] = q;
         }
       }
@@ -726,11 +764,15 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
   }
   L.pK.resize(L.pOff.back());
   L.pA.resize(L.pOff.back());
+  L.pIa.resize(L.pOff.back());
+  L.pIb.resize(L.pOff.back());
   for (int64_t e = 0; e < total; e++) {
     int64_t q = src[e];
     if (q < 0) continue;
     std::copy(hs.p2K.begin() + hs.p2Off[q], hs.p2K.begin() + hs.p2Off[q + 1], L.pK.begin() + L.pOff[e]);
     std::copy(hs.p2A.begin() + hs.p2Off[q], hs.p2A.begin() + hs.p2Off[q + 1], L.pA.begin() + L.pOff[e]);
+    std::copy(hs.p2Ia.begin() + hs.p2Off[q], hs.p2Ia.begin() + hs.p2Off[q + 1], L.pIa.begin() + L.pOff[e]);
+    std::copy(hs.p2Ib.begin() + hs.p2Off[q], hs.p2Ib.begin() + hs.p2Off[q + 1], L.pIb.begin() + L.pOff[e]);
   }
 }
 

@@ -22,6 +22,9 @@ struct ModelInput {
   double s0 = 0, s1 = 0, s2 = 0, density = 1;
   int patchDim = 10;
   std::vector<int> fixed;
+  // optional per-triangle strain stiffness of a cloth: an edge's spring weight
+  // is the mean over its adjacent triangles (empty: s0 on every edge)
+  std::vector<double> triK;
   int nv() const { return (int)verts.size() / 3; }
 };
 
@@ -42,8 +45,17 @@ struct HostSetup {
   std::vector<int> cCenter;              // nCloth: point id
   std::vector<int> cOff, cNbr;           // CSR over slots
   std::vector<double> cRest, cLbo, cBendIn;  // per slot
-  std::vector<double> cStrain, cH0;      // per cloth vertex
+  std::vector<double> cStrain;           // per slot: spring weight of the slot's edge
+  std::vector<double> cH0;               // per cloth vertex
   std::vector<int> clothOfPoint;         // nSim: cloth index or -1
+  // derivative bookkeeping (used by the adjoint)
+  std::vector<double> cBendOut;          // per slot: bend(i -> slot) = L_i,slot * bend * area_i
+  std::vector<double> cLboIn;            // per slot s of i (k = nbr): L_k,i (the gather form of L^T)
+  std::vector<int> cSlotEdge;            // per slot: cloth edge id (-1 for the vertex itself)
+  std::vector<int> cSlotR1;              // per slot: R1 CSR index of (i, nbr) (-1 for the vertex itself)
+  std::vector<int> edgeTriOff, edgeTriList;  // cloth edge -> cloth triangles (ids over all cloths)
+  std::vector<int> clothTriBase;         // per simulated model: first cloth-triangle id (-1: not a cloth)
+  int nClothTris = 0;
   // tets
   std::vector<int> tetIdx;               // 4*T
   std::vector<double> tetDminv;          // 9*T row-major Dm^{-1}
@@ -58,6 +70,8 @@ struct HostSetup {
   std::vector<int64_t> p2Off;            // per M^2 entry
   std::vector<int> p2K;                  // middle index k
   std::vector<double> p2A;               // R1(i,k) * R1(k,j)
+  std::vector<int> p2Ia, p2Ib;           // R1 CSR indices of (i,k) and (j,k) of each 2-path
+  std::vector<int> m2R1Idx;              // R1 CSR index of (i,j), -1 if none
   // collision
   std::vector<int> bFaces, bEdges;       // 3*F, 2*E point ids
   std::vector<int> pfOff, pfList;        // patch -> boundary faces
@@ -89,6 +103,8 @@ struct SellLayout {
   std::vector<int64_t> pOff;       // per SELL entry + 1
   std::vector<int> pK;
   std::vector<double> pA;
+  std::vector<int> r1Idx;          // per SELL entry: R1 index of (i,j), -1 if none or padding
+  std::vector<int> pIa, pIb;       // per 2-path: R1 indices of its two factors
 };
 void build_sell(const HostSetup& hs, int lanes, SellLayout& out);
 

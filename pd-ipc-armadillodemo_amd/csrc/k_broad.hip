@@ -106,13 +106,35 @@ __device__ inline uint32_t expand_bits(uint32_t v) {
 }
 
 __global__ void k_scene_final(int nPart, const double* __restrict__ part, double* __restrict__ scene) {
-  if (threadIdx.x != 0) return;
-  for (int k = 0; k < 3; k++) {
-    double lo = part[k], hi = part[3 + k];
-    for (int q = 1; q < nPart; q++) { lo = dmin(lo, part[6 * q + k]); hi = dmax(hi, part[6 * q + 3 + k]); }
-    scene[k] = lo;
-    scene[3 + k] = hi;
-  }
+  __shared__ double red[6][kBlock / kWave];
+  double v[6];
+#pragma unroll
+  for (int k = 0; k < 3; k++) { v[k] = 1.7976931348623157e308; v[3 + k] = -1.7976931348623157e308; }
+  for (int q = threadIdx.x; q < nPart; q += blockDim.x)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      v[k] = dmin(v[k], part[6 * q + k]);
+      v[3 + k] = dmax(v[3 + k], part[6 * q + 3 + k]);
+    }
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      v[k] = dmin(v[k], __shfl_xor(v[k], o, kWave));
+      v[3 + k] = dmax(v[3 + k], __shfl_xor(v[3 + k], o, kWave));
+    }
+  int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  if (lane == 0)
+#pragma unroll
+    for (int k = 0; k < 6; k++) red[k][w] = v[k];
+  __syncthreads();
+  if (threadIdx.x == 0)
+    for (int k = 0; k < 3; k++) {
+      double a = red[k][0], b = red[3 + k][0];
+      for (int q = 1; q < kBlock / kWave; q++) { a = dmin(a, red[k][q]); b = dmax(b, red[3 + k][q]); }
+      scene[k] = a;
+      scene[3 + k] = b;
+    }
 }
 
 // morton_code(double4) of lbvh/morton_code.cuh:34-44 applied to the box
@@ -284,10 +306,9 @@ __global__ void k_make_keys(int P, const uint32_t* __restrict__ code, const int*
 }
 
 __global__ void k_karras(int P, const uint64_t* __restrict__ keys, int* __restrict__ left, int* __restrict__ right,
-                         int* __restrict__ parent, int* __restrict__ flags) {
+                         int* __restrict__ parent, int2* __restrict__ range) {
   int idx = blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= P - 1) return;
-  flags[idx] = 0;
   int first, last;
   if (idx == 0) {
     first = 0;
@@ -342,38 +363,56 @@ __global__ void k_karras(int P, const uint64_t* __restrict__ keys, int* __restri
   if (last == gamma + 1) rc_ += P - 1;
   left[idx] = lc_;
   right[idx] = rc_;
+  range[idx] = make_int2(first, last);
   parent[lc_] = idx;
   parent[rc_] = idx;
   if (idx == 0) parent[0] = -1;
 }
 
-__global__ void k_refit(int P, const int* __restrict__ sortedIdx, const double* __restrict__ pbox,
-                        const int* __restrict__ left, const int* __restrict__ right, const int* __restrict__ parent,
-                        int* __restrict__ flags, double* __restrict__ nbox) {
+// Node boxes without inter-block synchronisation: every internal node covers a
+// contiguous leaf range [first, last] (Karras), so its box is the union of the
+// leaf boxes over that range, read from a sparse table of power-of-two window
+// unions (level j holds the union of leaves [i, i + 2^j)).  Min / max are exact
+// and order-free, so the result equals the bottom-up refit bit for bit.
+__global__ void k_table_level0(int P, const int* __restrict__ sortedIdx, const double* __restrict__ pbox,
+                               double* __restrict__ tab) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= P) return;
+  int obj = sortedIdx[i];
+#pragma unroll
+  for (int k = 0; k < 6; k++) tab[(int64_t)k * P + i] = pbox[(int64_t)k * P + obj];
+}
+
+__global__ void k_table_level(int P, int half, const double* __restrict__ prev, double* __restrict__ cur) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 2 * half > P) return;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    cur[(int64_t)k * P + i] = dmin(prev[(int64_t)k * P + i], prev[(int64_t)k * P + i + half]);
+    cur[(int64_t)(3 + k) * P + i] = dmax(prev[(int64_t)(3 + k) * P + i], prev[(int64_t)(3 + k) * P + i + half]);
+  }
+}
+
+__global__ void k_node_boxes(int P, const int2* __restrict__ range, const double* __restrict__ tab,
+                             double* __restrict__ nbox) {
+  int n = blockIdx.x * blockDim.x + threadIdx.x;
   const int NN = 2 * P - 1;
-  int leaf = P - 1 + i, obj = sortedIdx[i];
+  if (n >= NN) return;
+  if (n >= P - 1) {
+    int i = n - (P - 1);
 #pragma unroll
-  for (int k = 0; k < 6; k++) nbox[(int64_t)k * NN + leaf] = pbox[(int64_t)k * P + obj];
-  if (P == 1) return;
-  int node = parent[leaf];
-  while (node >= 0) {
-    __threadfence();
-    int old = atomicAdd(&flags[node], 1);
-    if (old == 0) return;  // the sibling will finish this node
-    __threadfence();
-    int a = left[node], b = right[node];
+    for (int k = 0; k < 6; k++) nbox[(int64_t)k * NN + n] = tab[(int64_t)k * P + i];
+    return;
+  }
+  int2 r = range[n];
+  int len = r.y - r.x + 1;
+  int j = 31 - __clz(len);
+  const double* t = tab + (int64_t)j * 6 * P;
+  int a = r.x, b = r.y - (1 << j) + 1;
 #pragma unroll
-    for (int k = 0; k < 3; k++) {
-      double la = __hip_atomic_load(&nbox[(int64_t)k * NN + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      double lb = __hip_atomic_load(&nbox[(int64_t)k * NN + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      double ha = __hip_atomic_load(&nbox[(int64_t)(3 + k) * NN + a], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      double hb = __hip_atomic_load(&nbox[(int64_t)(3 + k) * NN + b], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-      nbox[(int64_t)k * NN + node] = dmin(la, lb);
-      nbox[(int64_t)(3 + k) * NN + node] = dmax(ha, hb);
-    }
-    node = parent[node];
+  for (int k = 0; k < 3; k++) {
+    nbox[(int64_t)k * NN + n] = dmin(t[(int64_t)k * P + a], t[(int64_t)k * P + b]);
+    nbox[(int64_t)(3 + k) * NN + n] = dmax(t[(int64_t)(3 + k) * P + a], t[(int64_t)(3 + k) * P + b]);
   }
 }
 
@@ -384,49 +423,61 @@ __device__ inline bool overlap(const double* a, const double* b) {
   return true;
 }
 
-// FILL = 0 counts, 1 writes.  Leaves report object q (> p) when boxes overlap.
-template <int FILL>
-__global__ void k_traverse(int P, const double* __restrict__ pbox, const double* __restrict__ nbox,
-                           const int* __restrict__ left, const int* __restrict__ right,
-                           const int* __restrict__ sortedIdx, const uint8_t* __restrict__ patchStatic, int skipStatic,
-                           int* __restrict__ count, const int* __restrict__ off, int2* __restrict__ pairs, int cap,
-                           int* __restrict__ overflow) {
+// One lane per patch p walks the tree and appends every patch q > p whose box
+// overlaps; appends are aggregated per wavefront (one atomic per wave per hit
+// round).  Pair order is therefore not fixed; the pair set is.  Static patches
+// follow all simulated ones in patch order, so with skipStatic a static query
+// has nothing to report and exits at once.
+__global__ void __launch_bounds__(kBlock) k_traverse(int P, const double* __restrict__ pbox,
+                                                     const double* __restrict__ nbox, const int* __restrict__ left,
+                                                     const int* __restrict__ right, const int* __restrict__ sortedIdx,
+                                                     const uint8_t* __restrict__ patchStatic, int skipStatic,
+                                                     int* __restrict__ counter, int2* __restrict__ pairs, int cap,
+                                                     int* __restrict__ overflow) {
   int p = blockIdx.x * blockDim.x + threadIdx.x;
-  if (p >= P) return;
+  int lane = threadIdx.x % kWave;
+  const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
+  bool alive = p < P && !(skipStatic && patchStatic[p]);
+  if (!alive) return;
   const int NN = 2 * P - 1;
   double q[6];
 #pragma unroll
   for (int k = 0; k < 6; k++) q[k] = pbox[(int64_t)k * P + p];
-  bool pStatic = skipStatic && patchStatic[p];
-  int n = 0, base = FILL ? off[p] : 0;
   int stack[kStack];
   int sp = 0;
-  stack[sp++] = 0;  // root: node 0 (or the single leaf when P == 1)
+  stack[sp++] = 0;  // root: node 0 (the single leaf when P == 1)
   while (sp > 0) {
     int node = stack[--sp];
     double b[6];
 #pragma unroll
     for (int k = 0; k < 6; k++) b[k] = nbox[(int64_t)k * NN + node];
-    if (!overlap(q, b)) continue;
-    if (node >= P - 1) {
-      int obj = sortedIdx[node - (P - 1)];
-      if (obj > p && !(pStatic && patchStatic[obj])) {
-        if (FILL) {
-          if (base + n < cap) pairs[base + n] = make_int2(p, obj);
-        }
-        n++;
-      }
-    } else {
-      // push right first so the left subtree is visited first
-      if (sp + 2 <= kStack) {
-        stack[sp++] = right[node];
+    bool hit = false;
+    int obj = -1;
+    if (overlap(q, b)) {
+      if (node >= P - 1) {
+        obj = sortedIdx[node - (P - 1)];
+        // every pair p < q is emitted once: by p, or -- when static queries are
+        // skipped -- by the simulated side of a static/simulated pair
+        hit = obj > p || (skipStatic && obj < p && patchStatic[obj]);
+      } else if (sp + 2 <= kStack) {
+        stack[sp++] = right[node];  // right first: the left subtree is visited first
         stack[sp++] = left[node];
       } else {
         atomicOr(overflow, 1);  // reported to the host as an error, never silently dropped
       }
     }
+    unsigned long long m = __ballot(hit);
+    if (m) {
+      int leader = __ffsll((long long)m) - 1;
+      int base = 0;
+      if (lane == leader) base = atomicAdd(counter, (int)__popcll(m));
+      base = __shfl(base, leader, kWave);
+      if (hit) {
+        int pos = base + (int)__popcll(m & lt);
+        if (pos < cap) pairs[pos] = make_int2(min(p, obj), max(p, obj));
+      }
+    }
   }
-  if (!FILL) count[p] = n;
 }
 
 __global__ void k_rec_count(int Q, const int2* __restrict__ pairs, const int* __restrict__ ppOff,
@@ -439,24 +490,39 @@ __global__ void k_rec_count(int Q, const int2* __restrict__ pairs, const int* __
   count[i] = pa * fb + pb * fa + ea * eb;
 }
 
-__global__ void k_rec_fill(int Q, int K, const int2* __restrict__ pairs, const int* __restrict__ recOff,
-                           const int* __restrict__ ppOff, const int* __restrict__ ppList,
-                           const int* __restrict__ pfOff, const int* __restrict__ pfList,
-                           const int* __restrict__ peOff, const int* __restrict__ peList,
-                           const int3* __restrict__ faces, const int2* __restrict__ edges, int4* __restrict__ recs) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= K) return;
-  // the pair whose range holds r: last i with recOff[i] <= r
-  int lo = 0, hi = Q - 1;
-  while (lo < hi) {
+__device__ inline int last_le(const int* __restrict__ off, int lo, int hi, int r) {
+  while (lo < hi) {  // last i in [lo, hi] with off[i] <= r
     int mid = (lo + hi + 1) >> 1;
-    if (recOff[mid] <= r) lo = mid;
+    if (off[mid] <= r) lo = mid;
     else hi = mid - 1;
   }
-  int2 pr = pairs[lo];
-  int id = r - recOff[lo];
+  return lo;
+}
+
+__global__ void __launch_bounds__(kBlock) k_rec_fill(int Q, int K, const int2* __restrict__ pairs,
+                                                     const int* __restrict__ recOff, const int* __restrict__ ppOff,
+                                                     const int* __restrict__ ppList, const int* __restrict__ pfOff,
+                                                     const int* __restrict__ pfList, const int* __restrict__ peOff,
+                                                     const int* __restrict__ peList, const int3* __restrict__ faces,
+                                                     const int2* __restrict__ edges, int4* __restrict__ recs) {
+  __shared__ int sRange[2];
+  __shared__ int sOff[kBlock + 1];
+  int r0 = blockIdx.x * kBlock;
+  int r1 = min(r0 + kBlock, K) - 1;
+  if (threadIdx.x == 0) sRange[0] = last_le(recOff, 0, Q - 1, r0);
+  if (threadIdx.x == 1) sRange[1] = last_le(recOff, 0, Q - 1, r1);
+  __syncthreads();
+  int lo = sRange[0], hi = sRange[1];
+  bool local = hi - lo + 1 <= kBlock;
+  if (local && threadIdx.x <= hi - lo) sOff[threadIdx.x] = recOff[lo + threadIdx.x];
+  __syncthreads();
+  int r = r0 + threadIdx.x;
+  if (r >= K) return;
+  int pi = local ? lo + last_le(sOff, 0, hi - lo, r) : last_le(recOff, lo, hi, r);
+  int2 pr = pairs[pi];
+  int id = r - recOff[pi];
   int A = pr.x, B = pr.y;
-  int pa = ppOff[A + 1] - ppOff[A], fa = pfOff[A + 1] - pfOff[A], ea = peOff[A + 1] - peOff[A];
+  int pa = ppOff[A + 1] - ppOff[A], fa = pfOff[A + 1] - pfOff[A];
   int pb = ppOff[B + 1] - ppOff[B], fb = pfOff[B + 1] - pfOff[B], eb = peOff[B + 1] - peOff[B];
   int VaFb = pa * fb, VbFa = pb * fa;
   int4 out;
@@ -497,7 +563,7 @@ void launch_lbvh(DeviceMesh& m, hipStream_t s) {
   if (P <= 0) return;
   int nPart = std::min(256, (P + kBlock - 1) / kBlock);
   hipLaunchKernelGGL(k_scene_partial, dim3(nPart), dim3(kBlock), 0, s, P, m.patchBox, m.sceneBoxPart);
-  hipLaunchKernelGGL(k_scene_final, dim3(1), dim3(kWave), 0, s, nPart, m.sceneBoxPart, m.sceneBox);
+  hipLaunchKernelGGL(k_scene_final, dim3(1), dim3(kBlock), 0, s, nPart, m.sceneBoxPart, m.sceneBox);
   hipLaunchKernelGGL(k_morton, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, m.sceneBox, m.patchBox,
                      m.mortonA, m.idxA);
   int nb = (P + kRadixTile - 1) / kRadixTile;
@@ -514,26 +580,22 @@ void launch_lbvh(DeviceMesh& m, hipStream_t s) {
   hipLaunchKernelGGL(k_make_keys, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, kin, iin, m.keys);
   if (P > 1)
     hipLaunchKernelGGL(k_karras, dim3((P - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, m.keys, m.nodeLeft,
-                       m.nodeRight, m.nodeParent, m.nodeFlag);
-  hipLaunchKernelGGL(k_refit, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, iin, m.patchBox, m.nodeLeft,
-                     m.nodeRight, m.nodeParent, m.nodeFlag, m.nodeBox);
+                       m.nodeRight, m.nodeParent, m.nodeRange);
+  hipLaunchKernelGGL(k_table_level0, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, iin, m.patchBox,
+                     m.nodeTable);
+  for (int j = 1; (1 << j) <= P; j++)
+    hipLaunchKernelGGL(k_table_level, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, 1 << (j - 1),
+                       m.nodeTable + (int64_t)(j - 1) * 6 * P, m.nodeTable + (int64_t)j * 6 * P);
+  hipLaunchKernelGGL(k_node_boxes, dim3((2 * P - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, m.nodeRange,
+                     m.nodeTable, m.nodeBox);
 }
 
-void launch_pair_count(const DeviceMesh& m, int skipStatic, hipStream_t s) {
+void launch_pairs(const DeviceMesh& m, int skipStatic, hipStream_t s) {
   int P = m.nPatches;
   if (P <= 0) return;
-  hipLaunchKernelGGL((k_traverse<0>), dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, m.patchBox, m.nodeBox,
-                     m.nodeLeft, m.nodeRight, m.idxA, m.patchStatic, skipStatic, m.pairCount, nullptr, nullptr, 0,
+  hipLaunchKernelGGL(k_traverse, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, m.patchBox, m.nodeBox,
+                     m.nodeLeft, m.nodeRight, m.idxA, m.patchStatic, skipStatic, m.counters + 0, m.pairs, m.pairCap,
                      m.counters + 3);
-  launch_exclusive_scan(m.pairCount, m.pairOff, P, m.counters + 0, m.scanBlock, s);
-}
-
-void launch_pair_fill(const DeviceMesh& m, int skipStatic, hipStream_t s) {
-  int P = m.nPatches;
-  if (P <= 0) return;
-  hipLaunchKernelGGL((k_traverse<1>), dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, m.patchBox, m.nodeBox,
-                     m.nodeLeft, m.nodeRight, m.idxA, m.patchStatic, skipStatic, m.pairCount, m.pairOff, m.pairs,
-                     m.pairCap, m.counters + 3);
 }
 
 void launch_rec_count(const DeviceMesh& m, int nPairs, hipStream_t s) {

@@ -229,7 +229,6 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, const double* __res
   if (c >= 0) {
     int o = cOff[c], e = cOff[c + 1];
     double xi[3] = {X[i], X[ld + i], X[2 * ld + i]};
-    T st = (T)cStrain[c];
     T sp[3] = {0, 0, 0};
     for (int s = o + 1; s < e; s++) {
       int j = cNbr[s];
@@ -237,7 +236,7 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, const double* __res
 #pragma unroll
       for (int k = 0; k < 3; k++) d[k] = (T)(xi[k] - X[k * ld + j]);
       T len = dsqrt<T>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-      T rc = st * ((T)cRest[s] / len);
+      T rc = (T)cStrain[s] * ((T)cRest[s] / len);
 #pragma unroll
       for (int k = 0; k < 3; k++) sp[k] += rc * d[k];
     }

@@ -48,19 +48,25 @@ void launch_predict(const DeviceMesh& m, double dt, hipStream_t s) {
                      m.inertialDiag, m.gravY, m.invMass, m.fixFlag);
 }
 
-// LocalGlobalIteration diagonal (PdIpcSimulator.cpp:1557-1560): D = elastic + inertial (+ collision)
+// LocalGlobalIteration diagonal (PdIpcSimulator.cpp:1557-1560): D = elastic + inertial
+// (+ collision).  Also records which rows changed since the last operator update
+// (or were marked stale by the host), so the M^2 update can skip untouched rows.
 __global__ void k_diag(int n, const double* __restrict__ e, const double* __restrict__ in,
-                       const double* __restrict__ col, int withCol, double* __restrict__ D) {
+                       const double* __restrict__ col, int withCol, double* __restrict__ D,
+                       uint8_t* __restrict__ stale, uint8_t* __restrict__ changed) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   double d = e[i] + in[i];
   if (withCol) d += col[i];
+  uint8_t c = stale[i] | (uint8_t)(d != D[i]);
+  changed[i] = c;
+  stale[i] = 0;
   D[i] = d;
 }
 
 void launch_diag(const DeviceMesh& m, int withCollision, hipStream_t s) {
   hipLaunchKernelGGL(k_diag, dim3(grid_for(m.nSim)), dim3(kBlock), 0, s, m.nSim, m.elasticDiag, m.inertialDiag,
-                     m.colDiag, withCollision, m.sysDiag);
+                     m.colDiag, withCollision, m.sysDiag, m.dStale, m.dChanged);
 }
 
 // computeDirectionKernel (PdIpcGpuFunc.cu:185-200)

@@ -44,6 +44,8 @@ struct pdipc_handle {
   SellLayout sell;
   DeviceMesh m;
   hipStream_t stream = nullptr;
+  // local/global loop captured once per (collision flag, precision) and replayed
+  hipGraphExec_t lgGraph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   std::vector<void*> allocs;
   // host mirrors
   void* scalHost = nullptr;  // pinned: residual, toiBits, counters[8]
@@ -76,7 +78,16 @@ struct pdipc_handle {
   ~pdipc_handle() {
     for (void* p : allocs) (void)hipFree(p);
     if (scalHost) (void)hipHostFree(scalHost);
+    drop_graphs();
     if (stream) (void)hipStreamDestroy(stream);
+  }
+  void drop_graphs() {
+    for (auto& row : lgGraph)
+      for (auto& g : row)
+        if (g) {
+          (void)hipGraphExecDestroy(g);
+          g = nullptr;
+        }
   }
   double* hResidual() { return (double*)scalHost; }
   unsigned long long* hToi() { return (unsigned long long*)((char*)scalHost + 8); }
@@ -152,6 +163,9 @@ int upload_all(pdipc_handle* h) {
   UP(mass, hs.mass); UP(invMass, hs.invMass); UP(gravY, hs.gravY); UP(fixK, hs.fixK); UP(fixFlag, hs.fixFlag);
   UP(colMass, hs.colMass); UP(elasticDiag, hs.elasticDiag);
   AL(inertialDiag, hs.nSim); AL(sysDiag, hs.nSim); AL(colDiag, hs.nTotal);
+  AL(dStale, hs.nSim); AL(dChanged, hs.nSim); AL(dDirty, hs.nSim);
+  HIPCHK(hipMemset(m.dStale, 1, std::max(1, hs.nSim)));  // first update recomputes every row
+  HIPCHK(hipMemset(m.sysDiag, 0, std::max(1, hs.nSim) * sizeof(double)));
   HIPCHK(hipMemset(m.inertialDiag, 0, std::max(1, hs.nSim) * sizeof(double)));
   HIPCHK(hipMemset(m.colDiag, 0, std::max(1, hs.nTotal) * sizeof(double)));
   UP(cCenter, hs.cCenter); UP(cOff, hs.cOff.empty() ? std::vector<int>{0} : hs.cOff); UP(cNbr, hs.cNbr);
@@ -196,8 +210,13 @@ int upload_all(pdipc_handle* h) {
   AL(mortonA, P); AL(mortonB, P); AL(idxA, P); AL(idxB, P); AL(keys, P);
   int nbRadix = (P + 2047) / 2048;
   AL(radixHist, 256 * (size_t)nbRadix); AL(radixBlockSums, (256 * (size_t)nbRadix + 1023) / 1024 + 2);
-  AL(nodeLeft, P); AL(nodeRight, P); AL(nodeParent, 2 * (size_t)P); AL(nodeFlag, P); AL(nodeBox, 6 * (2 * (size_t)P));
-  AL(pairCount, P); AL(pairOff, P);
+  AL(nodeLeft, P); AL(nodeRight, P); AL(nodeParent, 2 * (size_t)P); AL(nodeRange, P);
+  AL(nodeBox, 6 * (2 * (size_t)P));
+  {
+    int levels = 1;
+    while ((1 << levels) <= P) levels++;
+    AL(nodeTable, (size_t)levels * 6 * P);
+  }
   m.pairCap = std::max(1024, 8 * P);
   AL(pairs, m.pairCap); AL(recCount, m.pairCap); AL(recOff, m.pairCap);
   m.recCap = std::max(4096, 32 * P);
@@ -242,13 +261,13 @@ int broad_phase(pdipc_handle* h, const double* x, const double* dx) {
   HIPCHK(hipMemsetAsync(m.counters, 0, 8 * sizeof(int), s));
   launch_patch_boxes(m, x, dx, infl, s);
   launch_lbvh(m, s);
-  launch_pair_count(m, h->prm.skip_static_pairs, s);
+  launch_pairs(m, h->prm.skip_static_pairs, s);
   int rc = check_launch(h);
   if (rc) return rc;
   if ((rc = read_scalars(h))) return rc;
   int Q = h->hCounters()[0];
   if (h->hCounters()[3]) return fail(PDIPC_E_CAPACITY, "LBVH traversal stack overflow");
-  if (Q > m.pairCap) {
+  if (Q > m.pairCap) {  // grow and search again
     int cap = Q + Q / 2 + 1024;
     h->release(m.pairs);
     if ((rc = h->alloc(&m.pairs, cap))) return rc;
@@ -263,8 +282,9 @@ int broad_phase(pdipc_handle* h, const double* x, const double* dx) {
       m.scanBlockCap = sb;
     }
     m.pairCap = cap;
+    HIPCHK(hipMemsetAsync(m.counters, 0, sizeof(int), s));
+    launch_pairs(m, h->prm.skip_static_pairs, s);
   }
-  launch_pair_fill(m, h->prm.skip_static_pairs, s);
   launch_rec_count(m, Q, s);
   if ((rc = check_launch(h))) return rc;
   if ((rc = read_scalars(h))) return rc;
@@ -283,7 +303,7 @@ int broad_phase(pdipc_handle* h, const double* x, const double* dx) {
   return PDIPC_OK;
 }
 
-int local_global(pdipc_handle* h, int withCol) {
+void enqueue_local_global(pdipc_handle* h, int withCol) {
   DeviceMesh& m = h->m;
   const pdipc_params& p = h->prm;
   hipStream_t s = h->stream;
@@ -293,6 +313,30 @@ int local_global(pdipc_handle* h, int withCol) {
     launch_local(m, p.precision, p.sigma_lo, p.sigma_hi, withCol, s);
     launch_solve(m, p.omega, p.rho, p.cheb_iters, p.precision, s);
   }
+}
+
+// LocalGlobalIteration (PdIpcSimulator.cpp:1554-1622).  Its kernel sequence and
+// arguments are identical every outer iteration (the solver's buffer rotation
+// is deterministic), so with use_graphs it is captured once and replayed.
+int local_global(pdipc_handle* h, int withCol) {
+  if (!h->prm.use_graphs) {
+    enqueue_local_global(h, withCol);
+    return check_launch(h);
+  }
+  hipGraphExec_t& g = h->lgGraph[withCol ? 1 : 0][h->prm.precision == 1 ? 1 : 0];
+  if (!g) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    enqueue_local_global(h, withCol);
+    HIPCHK(hipStreamEndCapture(h->stream, &graph));
+    hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      g = nullptr;
+      return fail(PDIPC_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+  HIPCHK(hipGraphLaunch(g, h->stream));
   return check_launch(h);
 }
 
@@ -480,6 +524,7 @@ int pdipc_step(pdipc_handle* h, int nsteps) {
 int pdipc_set_precision(pdipc_handle* h, int precision) {
   if (!h) return fail(PDIPC_E_ARG, "null handle");
   if (precision != PDIPC_FP64 && precision != PDIPC_FP32_SPLIT) return fail(PDIPC_E_ARG, "unknown precision");
+  if (h->prm.precision != precision) h->drop_graphs();
   h->prm.precision = precision;
   return PDIPC_OK;
 }
@@ -643,6 +688,8 @@ int pdipc_debug_solve(pdipc_handle* h, const double* diag, const double* rhs, co
   if (rc) return rc;
   DeviceMesh& m = h->m;
   HIPCHK(hipMemcpyAsync(m.sysDiag, diag, m.nSim * sizeof(double), hipMemcpyHostToDevice, h->stream));
+  HIPCHK(hipMemsetAsync(m.dChanged, 1, m.nSim, h->stream));  // the host wrote D: every row is stale
+  HIPCHK(hipMemsetAsync(m.dStale, 1, m.nSim, h->stream));    // and stays stale for the next step
   if ((rc = put_soa(h, m.sysRhs, rhs, m.nSim))) return rc;
   if ((rc = put_soa(h, m.X, x0, m.nTotal))) return rc;
   launch_operators(m, h->prm.omega, h->prm.precision, h->stream);
@@ -698,6 +745,7 @@ int pdipc_time_kernel(pdipc_handle* h, int which, int reps, double* avg_us, doub
     case 3: b = h->lastRecs * (16.0 + 24 * 8.0); break;          // record + x0, x1 of 4 points
     default: b = (double)h->hs.n2paths * 12.0 + (double)m.sellEntries * (4 + 8 + 16 + 8 + 4); break;
   }
+  if (which == 4) HIPCHK(hipMemsetAsync(m.dChanged, 1, m.nSim, s));  // time the full recompute
   launch();  // warm
   hipEvent_t e0, e1;
   HIPCHK(hipEventCreate(&e0));

@@ -223,6 +223,24 @@ def config2(seed=0, cells=40, cloth_n=128):
     return s
 
 
+def config3(seed=0, n_cloth=128, steps=50):
+    """The differentiable run: a 128 x 128 cloth (32,258 triangles) over the
+    config-0 obstacle (icosphere subdivision 4), FP64, `steps` steps.  Returns
+    (scene, v0, target): per-triangle strain stiffness log-uniform in
+    [1e3, 1e5] (an edge's spring weight is the mean of its triangles), initial
+    velocity N(0, 0.05^2) per component, and a seeded target drape (the rest
+    grid lowered by 5 cm plus N(0, 1 cm^2) noise) for the loss
+    sum |x_T - target|^2."""
+    s = cloth_over_sphere(n_cloth, 4, ground=False, seed=seed, patch_dim=64, sphere_patch_dim=64)
+    rng = np.random.default_rng(1_000_003 + seed)
+    c = s["cloths"][0]
+    c["tri_k"] = np.exp(rng.uniform(np.log(1e3), np.log(1e5), len(c["f"])))
+    v0 = rng.normal(0.0, 0.05, c["v"].shape)
+    target = c["v"] + np.array([0.0, -0.05, 0.0]) + rng.normal(0.0, 0.01, c["v"].shape)
+    s["steps"] = steps
+    return s, v0, target
+
+
 def scene_sizes(scene):
     nv = sum(len(c["v"]) for c in scene["cloths"]) + sum(len(t["v"]) for t in scene["tets"])
     nf = sum(len(c["f"]) for c in scene["cloths"]) + sum(len(t["f"]) for t in scene["statics"])
