@@ -158,6 +158,36 @@ int pdipc_debug_lbvh(pdipc_handle* h, unsigned long long* keys, int* left, int* 
  * the algorithmic HBM bytes one launch moves (matrix entries once, vectors once). */
 int pdipc_time_kernel(pdipc_handle* h, int which, int reps, double* avg_us, double* bytes);
 
+/* --- per-triangle strain stiffness of a cloth (an extension: the reference
+ * takes one strain weight per cloth model, PdIpcClothModel.h:282,300).  Each
+ * edge's spring weight becomes the mean stiffness of its adjacent triangles.
+ * `model` indexes the simulated models in add order; one value per triangle.
+ * Before pdipc_host_setup it is recorded; afterwards the matrix values are
+ * recomputed and uploaded (the sparsity pattern must not change). */
+int pdipc_set_cloth_stiffness(pdipc_handle* h, int model, const double* k, int ntris);
+
+/* --- reverse-mode derivative of the FP64 step (an extension: the reference
+ * has no derivative).  pdipc_rollout runs `nsteps` steps from the current
+ * state keeping (x, v) every `checkpoint_every` steps; pdipc_backward takes
+ * dL/dx and dL/dv of the final state (AoS over the simulated points; NULL =
+ * zero) and returns dL/dx and dL/dv of the initial state and dL/dk for every
+ * cloth triangle (all cloth models in add order, nk >= their total), treating
+ * candidate records, barrier active sets, minimum-TOI records and iteration
+ * counts as fixed.  It re-runs one checkpoint segment at a time with every
+ * solver iterate recorded; afterwards the handle is left at the final state. */
+int pdipc_rollout(pdipc_handle* h, int nsteps, int checkpoint_every);
+int pdipc_backward(pdipc_handle* h, const double* dL_dx, const double* dL_dv, double* dL_dx0, double* dL_dv0,
+                   double* dL_dk, int nk);
+/* outer iterations recorded by the last backward pass, peak tape bytes,
+ * segments, steps */
+int pdipc_adjoint_stats(pdipc_handle* h, double* out4);
+/* phase-level adjoints for parity tests: the vector-Jacobian products of one
+ * Chebyshev solve (diag, rhs, x0 -> x_out; returns dL/dx0 and dL/drhs for a
+ * given dL/dx_out) and of the local step at x (dL/dx for a given dL/drhs) */
+int pdipc_debug_solve_vjp(pdipc_handle* h, const double* diag, const double* rhs, const double* x0,
+                          const double* xbar_out, double* xbar0, double* bbar);
+int pdipc_debug_local_vjp(pdipc_handle* h, const double* x, const double* bbar, double* xbar);
+
 #ifdef __cplusplus
 }
 #endif

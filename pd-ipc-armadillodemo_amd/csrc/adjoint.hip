@@ -1,0 +1,512 @@
+// Reverse mode of the PD-IPC step (an extension: the reference has no
+// derivative).  The step is differentiated as computed, in FP64:
+//
+//   * the step count, the outer-iteration count of every step, the candidate
+//     records, the set of active barrier pairs and the record attaining the
+//     minimum TOI are frozen (piecewise-constant choices; the derivative is
+//     exact wherever they do not switch);
+//   * the Chebyshev solve is reversed sweep by sweep on the same SELL matrix:
+//     with u = D^{-1} xbar, M^T xbar = D M u (R1 is symmetric), so the adjoint
+//     sweep is the forward SpMV shape; the M^2 entries' adjoints are gathered
+//     on the fly and pulled back through the operator update to D and R1;
+//   * the local step is reversed per element (spring Jacobians, the bending
+//     rescale, the SVD strain projection's differential), the barrier terms
+//     and the ACCD time of impact by forward-mode dual numbers over the four
+//     points of a record.
+//
+// Memory: pdipc_rollout keeps (x, v) every `checkpoint_every` steps;
+// pdipc_backward re-runs one segment at a time recording every Chebyshev
+// iterate (15 x 10 x 3N doubles per outer iteration), then walks it backwards.
+#include <hip/hip_runtime.h>
+
+#include <climits>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+#include "handle.hpp"
+
+namespace pdipc {
+
+namespace {
+
+// bump allocator over large device chunks, reset per segment
+struct Pool {
+  struct Chunk {
+    char* p;
+    size_t size;
+  };
+  std::vector<Chunk> chunks;
+  size_t ci = 0, off = 0;
+  int alloc(size_t bytes, void** out) {
+    bytes = (bytes + 255) & ~(size_t)255;
+    while (ci < chunks.size() && off + bytes > chunks[ci].size) {
+      ci++;
+      off = 0;
+    }
+    if (ci == chunks.size()) {
+      size_t sz = std::max(bytes, (size_t)1 << 29);
+      char* p = nullptr;
+      HIPCHK(hipMalloc((void**)&p, sz));
+      chunks.push_back({p, sz});
+      off = 0;
+    }
+    *out = chunks[ci].p + off;
+    off += bytes;
+    return PDIPC_OK;
+  }
+  void reset() { ci = 0; off = 0; }
+  void release() {
+    for (auto& c : chunks) (void)hipFree(c.p);
+    chunks.clear();
+    reset();
+  }
+};
+
+struct OuterTape {
+  double* xold = nullptr;    // Xold of this outer iteration
+  double* xstart = nullptr;  // X when the local/global loop starts
+  double* D = nullptr;       // system diagonal
+  std::vector<double*> b;    // rhs of each local/global pass
+  std::vector<std::vector<double*>> it;  // it[l][k] = k-th Chebyshev iterate of pass l (k = 0: its input)
+  double toi = 1.0;
+  int clamped = 0;
+  int argmin = -1;
+  int withCol = 0;
+};
+
+struct StepTape {
+  double *x0 = nullptr, *v0 = nullptr;
+  int4* recs = nullptr;
+  int nRecs = 0;
+  std::vector<OuterTape> outers;
+};
+
+}  // namespace
+
+struct AdjointState {
+  AdjMesh am;
+  std::vector<void*> allocs;
+  int nSteps = 0, every = 1;
+  std::vector<double*> ckX, ckV;
+  double *endX = nullptr, *endV = nullptr;
+  Pool tape;
+  std::vector<StepTape> seg;
+  double *ax = nullptr, *axo = nullptr, *asol = nullptr, *uA = nullptr, *uB = nullptr, *uC = nullptr, *uc = nullptr;
+  double *bbar = nullptr, *aA = nullptr, *tetAdj = nullptr, *rhs0bar = nullptr, *axn = nullptr;
+  double* colR[2] = {nullptr, nullptr};
+  double* colD[2] = {nullptr, nullptr};
+  double *xbarA = nullptr, *vbarA = nullptr, *xbarB = nullptr, *vbarB = nullptr;
+  double *Dbar = nullptr, *DbarEl = nullptr, *Gbar = nullptr, *R1bar = nullptr, *sbar = nullptr;
+  double* recToi = nullptr;
+  int recToiCap = 0;
+  double *partials = nullptr, *gscale = nullptr;
+  double stats[4] = {0, 0, 0, 0};  // recorded outer iterations, tape bytes, segments, steps
+};
+
+void adjoint_free(AdjointState* a) {
+  if (!a) return;
+  for (void* p : a->allocs) (void)hipFree(p);
+  for (double* p : a->ckX) (void)hipFree(p);
+  for (double* p : a->ckV) (void)hipFree(p);
+  a->tape.release();
+  delete a;
+}
+
+namespace {
+
+template <class T>
+int dalloc(AdjointState* A, T** p, size_t n) {
+  HIPCHK(hipMalloc((void**)p, std::max<size_t>(n, 1) * sizeof(T)));
+  A->allocs.push_back(*p);
+  HIPCHK(hipMemset(*p, 0, std::max<size_t>(n, 1) * sizeof(T)));
+  return PDIPC_OK;
+}
+template <class T>
+int dupload(AdjointState* A, T** p, const std::vector<T>& v) {
+  int rc = dalloc(A, p, v.size());
+  if (rc) return rc;
+  if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
+  return PDIPC_OK;
+}
+
+int ensure_state(pdipc_handle* h) {
+  if (h->adj) return PDIPC_OK;
+  auto* A = new AdjointState();
+  h->adj = A;
+  const HostSetup& hs = h->hs;
+  const DeviceMesh& m = h->m;
+  size_t V3 = 3 * (size_t)m.ld;
+  int rc;
+  if ((rc = dupload(A, &A->am.sellR1Idx, h->sell.r1Idx)) || (rc = dupload(A, &A->am.pIa, h->sell.pIa)) ||
+      (rc = dupload(A, &A->am.pIb, h->sell.pIb)) || (rc = dupload(A, &A->am.cSlotR1, hs.cSlotR1)) ||
+      (rc = dupload(A, &A->am.cBendOut, hs.cBendOut)) || (rc = dupload(A, &A->am.cLboIn, hs.cLboIn)))
+    return rc;
+  for (double** p : {&A->ax, &A->axo, &A->asol, &A->uA, &A->uB, &A->uC, &A->uc, &A->bbar, &A->aA, &A->rhs0bar,
+                     &A->axn, &A->colR[0], &A->colR[1], &A->xbarA, &A->vbarA, &A->xbarB, &A->vbarB, &A->endX,
+                     &A->endV})
+    if ((rc = dalloc(A, p, V3))) return rc;
+  for (double** p : {&A->Dbar, &A->DbarEl, &A->colD[0], &A->colD[1]})
+    if ((rc = dalloc(A, p, (size_t)m.nSim))) return rc;
+  if ((rc = dalloc(A, &A->tetAdj, 12 * (size_t)std::max(1, m.nTets)))) return rc;
+  if ((rc = dalloc(A, &A->Gbar, (size_t)m.sellEntries))) return rc;
+  if ((rc = dalloc(A, &A->R1bar, hs.r1Col.size()))) return rc;
+  if ((rc = dalloc(A, &A->sbar, hs.cNbr.size()))) return rc;
+  if ((rc = dalloc(A, &A->partials, (size_t)m.nPartials + 1))) return rc;
+  if ((rc = dalloc(A, &A->gscale, 1))) return rc;
+  return PDIPC_OK;
+}
+
+int slot(AdjointState* A, size_t n, double** out) { return A->tape.alloc(n * sizeof(double), (void**)out); }
+
+// One FP64 step with every quantity the reverse pass needs recorded.  The
+// kernel sequence is the step's own (advance in pdipc_api.hip) with the
+// Chebyshev iterates written to the tape and the CCD recording per-record
+// TOIs, so the recorded trajectory is bit-identical to the plain step's.
+int taped_step(pdipc_handle* h, StepTape& st) {
+  AdjointState* A = h->adj;
+  DeviceMesh& m = h->m;
+  const pdipc_params& p = h->prm;
+  hipStream_t s = h->stream;
+  size_t V3 = 3 * (size_t)m.ld;
+  int rc;
+  if ((rc = slot(A, V3, &st.x0)) || (rc = slot(A, V3, &st.v0))) return rc;
+  launch_copy(st.x0, m.X, (int64_t)V3, s);
+  launch_copy(st.v0, m.V, (int64_t)V3, s);
+  launch_copy(m.X0, m.X, (int64_t)V3, s);
+  launch_copy(m.Xold, m.X, (int64_t)V3, s);
+  launch_predict(m, p.dt, s);
+  bool activeCol = false;
+  int outer = 0;
+  for (; outer < p.max_outer; outer++) {
+    OuterTape ot;
+    ot.withCol = activeCol ? 1 : 0;
+    if ((rc = slot(A, V3, &ot.xold)) || (rc = slot(A, V3, &ot.xstart)) || (rc = slot(A, (size_t)m.nSim, &ot.D)))
+      return rc;
+    launch_copy(ot.xold, m.Xold, (int64_t)V3, s);
+    launch_copy(ot.xstart, m.X, (int64_t)V3, s);
+    launch_diag(m, ot.withCol, s);
+    launch_copy(ot.D, m.sysDiag, m.nSim, s);
+    launch_operators(m, p.omega, PDIPC_FP64, s);
+    ot.b.assign(p.lg_iters, nullptr);
+    ot.it.assign(p.lg_iters, std::vector<double*>(p.cheb_iters + 1, nullptr));
+    for (int l = 0; l < p.lg_iters; l++) {
+      launch_local(m, PDIPC_FP64, p.sigma_lo, p.sigma_hi, ot.withCol, s);
+      if ((rc = slot(A, V3, &ot.b[l]))) return rc;
+      launch_copy(ot.b[l], m.sysRhs, (int64_t)V3, s);
+      ot.it[l][0] = l == 0 ? ot.xstart : ot.it[l - 1][p.cheb_iters];
+      for (int k = 1; k <= p.cheb_iters; k++)
+        if ((rc = slot(A, V3, &ot.it[l][k]))) return rc;
+      launch_solve_taped(m, p.omega, p.rho, p.cheb_iters, ot.it[l].data(), s);
+    }
+    launch_direction(m, s);
+    if (outer == 0) {
+      if ((rc = broad_phase(h, m.Xold, m.Dir))) return rc;
+      st.nRecs = h->lastRecs;
+      if ((rc = A->tape.alloc((size_t)std::max(1, st.nRecs) * sizeof(int4), (void**)&st.recs))) return rc;
+      if (st.nRecs > 0)
+        HIPCHK(hipMemcpyAsync(st.recs, m.recs, (size_t)st.nRecs * sizeof(int4), hipMemcpyDeviceToDevice, s));
+      if (st.nRecs > A->recToiCap) {
+        if (A->recToi) {
+          (void)hipFree(A->recToi);
+          A->allocs.erase(std::find(A->allocs.begin(), A->allocs.end(), (void*)A->recToi));
+        }
+        A->recToiCap = st.nRecs + st.nRecs / 2 + 1024;
+        if ((rc = dalloc(A, &A->recToi, (size_t)A->recToiCap))) return rc;
+      }
+    }
+    launch_toi_init(m, s);
+    HIPCHK(hipMemsetAsync(m.counters + 4, 0x7f, sizeof(int), s));
+    launch_ccd_records(m, m.recs, h->lastRecs, m.Xold, m.X, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap,
+                       p.touch_policy, A->recToi, m.counters + 4, s);
+    launch_clamp_residual(m, p.toi_scale, s);
+    if ((rc = collision(h))) return rc;  // reads the scalars back
+    activeCol = true;
+    double res = *h->hResidual();
+    double tmin;
+    unsigned long long bits = *h->hToi();
+    memcpy(&tmin, &bits, 8);
+    int am = h->hCounters()[4];
+    ot.clamped = tmin < 1.0 ? 1 : 0;
+    ot.toi = tmin < 1.0 ? tmin * p.toi_scale : 1.0;
+    ot.argmin = (ot.clamped && am != 0x7f7f7f7f) ? am : -1;
+    st.outers.push_back(std::move(ot));
+    if (!std::isfinite(res)) return fail(PDIPC_E_NAN, "non-finite search direction");
+    if (res < p.residual_tol && outer > 0) break;
+    launch_copy(m.Xold, m.X, (int64_t)V3, s);
+  }
+  launch_velocity(m, p.dt, s);
+  A->stats[0] += (double)st.outers.size();
+  return check_launch(h);
+}
+
+// Chebyshev weights of the sweeps: W[0] = 1 (x1 = M^2 x0 + c), W[1] = 2/(2 - r),
+// W[k+1] = 4/(4 - r W[k])
+std::vector<double> cheb_weights(const pdipc_params& p) {
+  std::vector<double> W(p.cheb_iters + 1, 0.0);
+  double r = p.rho * p.rho;
+  W[0] = 1.0;
+  if (p.cheb_iters > 1) W[1] = 2.0 / (2.0 - r);
+  for (int k = 1; k + 1 < p.cheb_iters; k++) W[k + 1] = 4.0 / (4.0 - r * W[k]);
+  return W;
+}
+
+// Reverse of one recorded step: (xbarIn, vbarIn) = adjoints of the state after
+// the step -> (xbarOut, vbarOut) = adjoints of the state before it; the
+// stiffness adjoints accumulate in sbar, R1bar and DbarEl.
+int reverse_step(pdipc_handle* h, StepTape& st, const double* xbarIn, const double* vbarIn, double* xbarOut,
+                 double* vbarOut) {
+  AdjointState* A = h->adj;
+  DeviceMesh& m = h->m;
+  const pdipc_params& p = h->prm;
+  hipStream_t s = h->stream;
+  size_t V3 = 3 * (size_t)m.ld;
+  const std::vector<double> W = cheb_weights(p);
+  const int K = p.cheb_iters, LG = p.lg_iters;
+  double invt = 1.0 / p.dt * (1.0 - p.dt);
+  // v = (x - x0) (1 - dt) / dt
+  launch_axpby(m, 1.0, xbarIn, invt, vbarIn, A->ax, s);
+  launch_axpby(m, -invt, vbarIn, 0.0, nullptr, A->axn, s);
+  launch_zero(A->rhs0bar, (int64_t)V3, s);
+  int cur = 0;
+  launch_zero(A->colD[cur], m.nSim, s);
+  launch_zero(A->colR[cur], (int64_t)V3, s);
+  const int O = (int)st.outers.size();
+  for (int o = O - 1; o >= 0; o--) {
+    OuterTape& ot = st.outers[o];
+    const double* xsol = ot.it[LG - 1][K];
+    // collision terms computed at the end of this outer iteration (read by the next)
+    if (o < O - 1)
+      launch_adj_collision(m, st.recs, st.nRecs, st.outers[o + 1].xold, p.dhat, p.kappa, p.dt, A->colD[cur],
+                           A->colR[cur], A->ax, s);
+    // x = xold + toi (xsol - xold)
+    launch_axpby(m, ot.toi, A->ax, 0.0, nullptr, A->asol, s);
+    launch_axpby(m, 1.0 - ot.toi, A->ax, 0.0, nullptr, A->axo, s);
+    if (ot.clamped && ot.argmin >= 0) {
+      launch_dot_dir(m, A->ax, xsol, ot.xold, p.toi_scale, A->partials, A->gscale, s);
+      launch_adj_ccd(m, st.recs, ot.argmin, ot.xold, xsol, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap,
+                     p.touch_policy, A->gscale, A->axo, A->asol, s);
+    }
+    int nxt = 1 - cur;
+    launch_zero(A->Gbar, m.sellEntries, s);
+    launch_zero(A->Dbar, m.nSim, s);
+    if (o > 0) {
+      launch_zero(A->colD[nxt], m.nSim, s);
+      launch_zero(A->colR[nxt], (int64_t)V3, s);
+    }
+    // the operators of this outer iteration, from its recorded diagonal
+    launch_copy(m.sysDiag, ot.D, m.nSim, s);
+    HIPCHK(hipMemsetAsync(m.dChanged, 1, m.nSim, s));
+    HIPCHK(hipMemsetAsync(m.dStale, 1, m.nSim, s));  // the next plain step recomputes every row
+    launch_operators(m, p.omega, PDIPC_FP64, s);
+    for (int l = LG - 1; l >= 0; l--) {
+      const std::vector<double*>& it = ot.it[l];
+      launch_div_rows(m, A->asol, m.sysDiag, A->uB, s);  // u_K
+      launch_zero(A->uA, (int64_t)V3, s);                // u_{K+1}
+      launch_zero(A->uc, (int64_t)V3, s);
+      for (int k = K - 1; k >= 0; k--) {
+        double b = k + 1 <= K - 1 ? 1.0 - W[k + 1] : 0.0;
+        launch_adj_sweep(m, W[k], b, A->uB, A->uA, it[k], A->uC, A->Gbar, A->uc, s);
+        double* t = A->uA;
+        A->uA = A->uB;
+        A->uB = A->uC;
+        A->uC = t;
+      }
+      launch_adj_first(m, p.omega, ot.b[l], A->uc, A->bbar, A->Dbar, A->R1bar, s);
+      launch_adj_local(m, A->am, p.sigma_lo, p.sigma_hi, it[0], A->bbar, A->uB, A->aA, A->tetAdj, A->asol, A->sbar,
+                       A->rhs0bar, ot.withCol ? A->colR[nxt] : nullptr, s);
+    }
+    launch_adj_operators(m, A->am, p.omega, A->Gbar, A->Dbar, A->R1bar, s);
+    launch_add_rows(A->DbarEl, A->Dbar, m.nSim, s);
+    if (o > 0) {
+      launch_copy(A->colD[nxt], A->Dbar, m.nSim, s);
+      launch_axpby(m, 1.0, A->axo, 1.0, A->asol, A->ax, s);  // both read x_clamp of outer o-1
+      cur = nxt;
+    } else {
+      launch_axpby(m, 1.0, A->axn, 1.0, A->axo, A->uC, s);  // xold of the first outer iteration is x_n
+      launch_predict_adj(m, p.dt, A->asol, A->rhs0bar, A->uC, xbarOut, vbarOut, s);
+    }
+    if (int rc = check_launch(h)) return rc;
+  }
+  return PDIPC_OK;
+}
+
+}  // namespace
+}  // namespace pdipc
+
+using namespace pdipc;
+
+extern "C" {
+
+int pdipc_rollout(pdipc_handle* h, int nsteps, int every) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (nsteps < 1 || every < 1) return fail(PDIPC_E_ARG, "nsteps and checkpoint_every must be positive");
+  if (h->prm.precision != PDIPC_FP64) return fail(PDIPC_E_ARG, "the adjoint runs on FP64 steps");
+  if ((rc = ensure_state(h))) return rc;
+  AdjointState* A = h->adj;
+  DeviceMesh& m = h->m;
+  size_t V3 = 3 * (size_t)m.ld;
+  int nck = (nsteps + every - 1) / every;
+  while ((int)A->ckX.size() < nck) {
+    double *x = nullptr, *v = nullptr;
+    HIPCHK(hipMalloc((void**)&x, V3 * sizeof(double)));
+    A->ckX.push_back(x);
+    HIPCHK(hipMalloc((void**)&v, V3 * sizeof(double)));
+    A->ckV.push_back(v);
+  }
+  A->nSteps = nsteps;
+  A->every = every;
+  for (int n = 0; n < nsteps; n++) {
+    if (n % every == 0) {
+      launch_copy(A->ckX[n / every], m.X, (int64_t)V3, h->stream);
+      launch_copy(A->ckV[n / every], m.V, (int64_t)V3, h->stream);
+    }
+    if ((rc = advance(h))) return rc;
+  }
+  launch_copy(A->endX, m.X, (int64_t)V3, h->stream);
+  launch_copy(A->endV, m.V, (int64_t)V3, h->stream);
+  HIPCHK(hipStreamSynchronize(h->stream));
+  return check_launch(h);
+}
+
+int pdipc_backward(pdipc_handle* h, const double* dx, const double* dv, double* dx0, double* dv0, double* dk, int nk) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  AdjointState* A = h->adj;
+  if (!A || A->nSteps < 1) return fail(PDIPC_E_ARG, "pdipc_backward needs a pdipc_rollout first");
+  if (h->prm.precision != PDIPC_FP64) return fail(PDIPC_E_ARG, "the adjoint runs on FP64 steps");
+  if (dk && nk < h->hs.nClothTris) return fail(PDIPC_E_ARG, "dL/dk holds fewer entries than the cloth triangles");
+  DeviceMesh& m = h->m;
+  const HostSetup& hs = h->hs;
+  hipStream_t s = h->stream;
+  size_t V3 = 3 * (size_t)m.ld;
+  // seeds (AoS over the simulated points; null = zero)
+  std::vector<double> zero(3 * (size_t)m.nSim, 0.0);
+  if ((rc = put_soa(h, A->xbarA, dx ? dx : zero.data(), m.nSim))) return rc;
+  if ((rc = put_soa(h, A->vbarA, dv ? dv : zero.data(), m.nSim))) return rc;
+  launch_zero(A->sbar, (int64_t)hs.cNbr.size(), s);
+  launch_zero(A->R1bar, (int64_t)hs.r1Col.size(), s);
+  launch_zero(A->DbarEl, m.nSim, s);
+  double *xin = A->xbarA, *vin = A->vbarA, *xout = A->xbarB, *vout = A->vbarB;
+  A->stats[0] = 0;
+  A->stats[2] = 0;
+  double tapeBytes = 0;
+  int nseg = (A->nSteps + A->every - 1) / A->every;
+  for (int sg = nseg - 1; sg >= 0; sg--) {
+    int first = sg * A->every, last = std::min(A->nSteps, first + A->every);
+    launch_copy(m.X, A->ckX[sg], (int64_t)V3, s);
+    launch_copy(m.V, A->ckV[sg], (int64_t)V3, s);
+    A->tape.reset();
+    A->seg.assign(last - first, StepTape());
+    for (int n = first; n < last; n++)
+      if ((rc = taped_step(h, A->seg[n - first]))) return rc;
+    double used = 0;
+    for (size_t c = 0; c < A->tape.ci && c < A->tape.chunks.size(); c++) used += (double)A->tape.chunks[c].size;
+    used += (double)A->tape.off;
+    tapeBytes = std::max(tapeBytes, used);
+    for (int n = last - 1; n >= first; n--) {
+      if ((rc = reverse_step(h, A->seg[n - first], xin, vin, xout, vout))) return rc;
+      std::swap(xin, xout);
+      std::swap(vin, vout);
+    }
+    A->stats[2] += 1;
+  }
+  A->stats[1] = tapeBytes;
+  A->stats[3] = A->nSteps;
+  HIPCHK(hipStreamSynchronize(s));
+  if ((rc = check_launch(h))) return rc;
+  if (dx0 && (rc = get_soa(h, xin, dx0, m.nSim))) return rc;
+  if (dv0 && (rc = get_soa(h, vin, dv0, m.nSim))) return rc;
+  if (dk) {
+    std::vector<double> sb(hs.cNbr.size()), rb(hs.r1Col.size()), de(m.nSim);
+    HIPCHK(hipMemcpy(sb.data(), A->sbar, sb.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(rb.data(), A->R1bar, rb.size() * sizeof(double), hipMemcpyDeviceToHost));
+    HIPCHK(hipMemcpy(de.data(), A->DbarEl, de.size() * sizeof(double), hipMemcpyDeviceToHost));
+    // spring weight of a slot enters: the local step (sbar), R1(i, nbr) and the diagonal D_i
+    std::vector<double> eg(hs.edgeTriOff.size(), 0.0);
+    for (size_t c = 0; c < hs.cCenter.size(); c++) {
+      int i = hs.cCenter[c];
+      for (int sl = hs.cOff[c] + 1; sl < hs.cOff[c + 1]; sl++) {
+        double g = sb[sl] + de[i] + (hs.cSlotR1[sl] >= 0 ? rb[hs.cSlotR1[sl]] : 0.0);
+        eg[hs.cSlotEdge[sl]] += g;
+      }
+    }
+    std::fill(dk, dk + hs.nClothTris, 0.0);
+    for (size_t e = 0; e < eg.size(); e++) {
+      int o = hs.edgeTriOff[e];
+      int n = (e + 1 < eg.size() ? hs.edgeTriOff[e + 1] : (int)hs.edgeTriList.size()) - o;
+      for (int q = 0; q < n; q++) dk[hs.edgeTriList[o + q]] += eg[e] / n;
+    }
+  }
+  // leave the handle at the end of the rollout
+  launch_copy(m.X, A->endX, (int64_t)V3, s);
+  launch_copy(m.V, A->endV, (int64_t)V3, s);
+  HIPCHK(hipStreamSynchronize(s));
+  return check_launch(h);
+}
+
+int pdipc_debug_solve_vjp(pdipc_handle* h, const double* diag, const double* rhs, const double* x0,
+                          const double* xbar_out, double* xbar0, double* bbar) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if ((rc = ensure_state(h))) return rc;
+  AdjointState* A = h->adj;
+  DeviceMesh& m = h->m;
+  const pdipc_params& p = h->prm;
+  hipStream_t s = h->stream;
+  size_t V3 = 3 * (size_t)m.ld;
+  HIPCHK(hipMemcpyAsync(m.sysDiag, diag, m.nSim * sizeof(double), hipMemcpyHostToDevice, s));
+  HIPCHK(hipMemsetAsync(m.dChanged, 1, m.nSim, s));
+  HIPCHK(hipMemsetAsync(m.dStale, 1, m.nSim, s));
+  if ((rc = put_soa(h, A->bbar, rhs, m.nSim))) return rc;
+  if ((rc = put_soa(h, A->axo, x0, m.nTotal))) return rc;
+  if ((rc = put_soa(h, A->asol, xbar_out, m.nSim))) return rc;
+  launch_operators(m, p.omega, PDIPC_FP64, s);
+  const std::vector<double> W = cheb_weights(p);
+  const int K = p.cheb_iters;
+  launch_div_rows(m, A->asol, m.sysDiag, A->uB, s);
+  launch_zero(A->uA, (int64_t)V3, s);
+  launch_zero(A->uc, (int64_t)V3, s);
+  launch_zero(A->Gbar, m.sellEntries, s);
+  for (int k = K - 1; k >= 0; k--) {
+    double b = k + 1 <= K - 1 ? 1.0 - W[k + 1] : 0.0;
+    launch_adj_sweep(m, W[k], b, A->uB, A->uA, A->axo, A->uC, A->Gbar, A->uc, s);
+    double* t = A->uA;
+    A->uA = A->uB;
+    A->uB = A->uC;
+    A->uC = t;
+  }
+  launch_zero(A->Dbar, m.nSim, s);
+  launch_adj_first(m, p.omega, A->bbar, A->uc, A->ax, A->Dbar, A->R1bar, s);
+  launch_mul_rows(m, A->uB, m.sysDiag, A->asol, s);
+  if ((rc = check_launch(h))) return rc;
+  if ((rc = get_soa(h, A->asol, xbar0, m.nSim))) return rc;
+  return get_soa(h, A->ax, bbar, m.nSim);
+}
+
+int pdipc_debug_local_vjp(pdipc_handle* h, const double* x, const double* bbar, double* xbar) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if ((rc = ensure_state(h))) return rc;
+  AdjointState* A = h->adj;
+  DeviceMesh& m = h->m;
+  hipStream_t s = h->stream;
+  size_t V3 = 3 * (size_t)m.ld;
+  if ((rc = put_soa(h, A->axo, x, m.nTotal))) return rc;
+  if ((rc = put_soa(h, A->bbar, bbar, m.nSim))) return rc;
+  launch_zero(A->uB, (int64_t)V3, s);
+  launch_zero(A->rhs0bar, (int64_t)V3, s);
+  launch_adj_local(m, A->am, h->prm.sigma_lo, h->prm.sigma_hi, A->axo, A->bbar, A->uB, A->aA, A->tetAdj, A->asol,
+                   A->sbar, A->rhs0bar, nullptr, s);
+  if ((rc = check_launch(h))) return rc;
+  return get_soa(h, A->asol, xbar, m.nSim);
+}
+
+int pdipc_adjoint_stats(pdipc_handle* h, double* out4) {
+  if (!h || !out4) return fail(PDIPC_E_ARG, "null argument");
+  for (int i = 0; i < 4; i++) out4[i] = h->adj ? h->adj->stats[i] : 0.0;
+  return PDIPC_OK;
+}
+
+}  // extern "C"

@@ -75,11 +75,20 @@ struct DeviceMesh {
   int recCap = 0;
   int* scanBlock = nullptr;        // scan scratch
   int scanBlockCap = 0;
+  // collision terms, summed per point in record order
+  int actCap = 0;                  // capacity of the active-record list
+  int* actRec = nullptr;           // active record ids
+  double *actW = nullptr, *actT = nullptr;  // weight, 4 targets x 3 per active record
+  int *ptCount = nullptr, *ptOff = nullptr, *ptCursor = nullptr, *ptScan = nullptr;  // nSim + 1
+  unsigned long long* entKey = nullptr;  // 4 * actCap: (record << 2 | corner)
+  int* entA = nullptr;
   // scalars
   unsigned long long* toiBits = nullptr;
   double* partials = nullptr;      // reduction partials
   double* residual = nullptr;
-  int* counters = nullptr;         // [0] = pairs, [1] = records, [2] = active pairs
+  // [0] pairs, [1] records, [2] active records, [3] traversal stack overflow,
+  // [4] TOI argmin (recorded steps), [5] active-list overflow
+  int* counters = nullptr;
   int nPartials = 0;
 };
 
@@ -100,10 +109,53 @@ void launch_solve(const DeviceMesh& m, double omega, double rho, int chebIters, 
 // one Chebyshev sweep of the given precision (measurement support)
 void launch_one_sweep(const DeviceMesh& m, int precision, hipStream_t s);
 // k_contact.hip
+// ---- adjoint (reverse mode of the FP64 step; active sets frozen)
+// index maps and transposed weights uploaded on the first adjoint call
+struct AdjMesh {
+  int* sellR1Idx = nullptr;  // per SELL entry: R1 index of (i,j) or -1
+  int *pIa = nullptr, *pIb = nullptr;  // per 2-path (SELL order): R1 indices of its factors
+  int* cSlotR1 = nullptr;    // per cloth slot: R1 index of (center, nbr) or -1
+  double *cBendOut = nullptr, *cLboIn = nullptr;  // per cloth slot
+};
+void launch_solve_taped(const DeviceMesh& m, double omega, double rho, int iters, double* const* outs, hipStream_t s);
+void launch_adj_sweep(const DeviceMesh& m, double a, double b, const double* uin, const double* uoth, const double* xk,
+                      double* uout, double* Gbar, double* uc, hipStream_t s);
+void launch_adj_first(const DeviceMesh& m, double omega, const double* b, const double* uc, double* bbar, double* Dbar,
+                      double* R1bar, hipStream_t s);
+void launch_adj_operators(const DeviceMesh& m, const AdjMesh& a, double omega, const double* Gbar, double* Dbar,
+                          double* R1bar, hipStream_t s);
+// cloth curvature + tet projection adjoints, then the per-point gather:
+//   ax = D u0 + J_local(x)^T bbar;  sbar += d rhs / d spring weight;  rhs0bar += bbar;
+//   colRbar += bbar (when the collision rhs entered the local step)
+void launch_adj_local(const DeviceMesh& m, const AdjMesh& a, double sigLo, double sigHi, const double* x,
+                      const double* bbar, const double* u0, double* aA, double* tetAdj, double* ax, double* sbar,
+                      double* rhs0bar, double* colRbar, hipStream_t s);
+// per-record TOI (2 where skipped), the minimum into toiBits, the first record
+// attaining it into *argmin (which the caller sets to INT_MAX beforehand)
+void launch_ccd_records(const DeviceMesh& m, const int4* recs, int nRecs, const double* x0, const double* x1,
+                        double filter, int iters, double step, double gap, int touchPolicy, double* recToi,
+                        int* argmin, hipStream_t s);
+void launch_adj_collision(const DeviceMesh& m, const int4* recs, int nRecs, const double* x, double dhat,
+                          double kappa, double dt, const double* colDbar, const double* colRbar, double* ax,
+                          hipStream_t s);
+void launch_adj_ccd(const DeviceMesh& m, const int4* recs, int argmin, const double* x0, const double* x1,
+                    double filter, int iters, double step, double gap, int touchPolicy, const double* gscale,
+                    double* ax0, double* ax1, hipStream_t s);
+// elementwise helpers over the simulated rows of [3][ld] vectors
+void launch_axpby(const DeviceMesh& m, double a, const double* x, double b, const double* y, double* out, hipStream_t s);
+void launch_div_rows(const DeviceMesh& m, const double* x, const double* D, double* out, hipStream_t s);
+void launch_mul_rows(const DeviceMesh& m, const double* x, const double* D, double* out, hipStream_t s);
+void launch_dot_dir(const DeviceMesh& m, const double* a, const double* x1, const double* x0, double scale,
+                    double* partials, double* out, hipStream_t s);
+void launch_predict_adj(const DeviceMesh& m, double dt, const double* as, const double* rhs0bar, const double* axn,
+                        double* xbar, double* vbar, hipStream_t s);
+void launch_add_rows(double* dst, const double* src, int64_t n, hipStream_t s);
+
 void launch_toi_init(const DeviceMesh& m, hipStream_t s);
 void launch_ccd(const DeviceMesh& m, int nRecs, double filter, int iters, double step, double gap, int touchPolicy,
                 hipStream_t s);
-void launch_collision(const DeviceMesh& m, int nRecs, double dhat, double kappa, double dt, hipStream_t s);
+// writes colDiag / colRhs of every simulated point (deterministic order)
+int launch_collision(DeviceMesh& m, int nRecs, double dhat, double kappa, double dt, hipStream_t s);
 // k_broad.hip
 // runs patch boxes, LBVH build, pair search and record expansion; returns the
 // pair and record totals (host sync) and whether the capacities sufficed.

@@ -9,88 +9,109 @@
 //                four diagonal entries, weight * target on the four rhs rows.
 // Records are int4: point-triangle (p, t0, t1, t2); edge-edge (~a0, a1, b0, b1).
 #include "device.hpp"
+#include "dual.hpp"
 
 namespace pdipc {
 namespace {
 
 constexpr int kBlock = 256;
 
-struct P3 { double v[3]; };
+template <class T>
+struct P3T { T v[3]; };
+using P3 = P3T<double>;
 
-__device__ inline P3 sub(const P3& a, const P3& b) { return {{a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]}}; }
-__device__ inline double dot(const P3& a, const P3& b) { return a.v[0] * b.v[0] + a.v[1] * b.v[1] + a.v[2] * b.v[2]; }
-__device__ inline P3 cross(const P3& a, const P3& b) {
+template <class T>
+__host__ __device__ inline P3T<T> sub(const P3T<T>& a, const P3T<T>& b) { return {{a.v[0] - b.v[0], a.v[1] - b.v[1], a.v[2] - b.v[2]}}; }
+template <class T>
+__host__ __device__ inline T dot(const P3T<T>& a, const P3T<T>& b) { return a.v[0] * b.v[0] + a.v[1] * b.v[1] + a.v[2] * b.v[2]; }
+template <class T>
+__host__ __device__ inline P3T<T> cross(const P3T<T>& a, const P3T<T>& b) {
   return {{a.v[1] * b.v[2] - a.v[2] * b.v[1], a.v[2] * b.v[0] - a.v[0] * b.v[2], a.v[0] * b.v[1] - a.v[1] * b.v[0]}};
 }
 
+// The primitive distances, ACCD and barrier projection are templated on the
+// scalar: double in the step, Dual<N> in the adjoint (same branches, exact
+// derivatives of the computed function).
+
 // VertexEdgeSqDistance (PdIpcGpuFunc.cu:2155-2174)
-__device__ inline double ve_d2(const P3& v, const P3& ea, const P3& eb, double* r) {
-  P3 va = sub(v, ea), ba = sub(eb, ea);
-  double vb = dot(va, ba), bb = dot(ba, ba);
-  *r = vb < 0 ? 0.0 : (vb > bb ? 1.0 : vb / bb);
-  P3 d;
+template <class T>
+__host__ __device__ inline T ve_d2(const P3T<T>& v, const P3T<T>& ea, const P3T<T>& eb, T* r) {
+  P3T<T> va = sub(v, ea), ba = sub(eb, ea);
+  T vb = dot(va, ba), bb = dot(ba, ba);
+  *r = vb < T(0.0) ? T(0.0) : (vb > bb ? T(1.0) : vb / bb);
+  P3T<T> d;
 #pragma unroll
-  for (int k = 0; k < 3; k++) d.v[k] = v.v[k] - (ea.v[k] * (1.0 - *r) + eb.v[k] * *r);
+  for (int k = 0; k < 3; k++) d.v[k] = v.v[k] - (ea.v[k] * (T(1.0) - *r) + eb.v[k] * *r);
   return dot(d, d);
 }
 
 // EdgeEdgeSqDistance (PdIpcGpuFunc.cu:2176-2236); parallel edges (cross product
 // below 1e-24) take the minimum over all four endpoint-edge distances.
-__device__ inline double ee_d2(const P3& xi, const P3& xj, const P3& xa, const P3& xb, double* r, double* s, P3* dir) {
-  P3 xba = sub(xb, xa), xji = sub(xj, xi), xai = sub(xa, xi);
-  P3 n = cross(xji, xba);
-  double nn = dot(n, n);
-  double waiji = dot(n, cross(xai, xji));
-  double waiba = dot(n, cross(xai, xba));
-  if (nn > 1e-24 && waiji >= 0 && waiji <= nn && waiba >= 0 && waiba <= nn) {
+template <class T>
+__host__ __device__ inline T ee_d2(const P3T<T>& xi, const P3T<T>& xj, const P3T<T>& xa, const P3T<T>& xb, T* r, T* s,
+                          P3T<T>* dir) {
+  P3T<T> xba = sub(xb, xa), xji = sub(xj, xi), xai = sub(xa, xi);
+  P3T<T> n = cross(xji, xba);
+  T nn = dot(n, n);
+  T waiji = dot(n, cross(xai, xji));
+  T waiba = dot(n, cross(xai, xba));
+  if (nn > T(1e-24) && waiji >= T(0.0) && waiji <= nn && waiba >= T(0.0) && waiba <= nn) {
     *r = waiba / nn;
     *s = waiji / nn;
   } else {
-    bool dg = !(nn > 1e-24);
-    double best = 999999999.0, d, v;
-    *r = 0; *s = 0;
-    if ((dg || waiba < 0) && (d = ve_d2(xi, xa, xb, &v)) < best) { best = d; *r = 0; *s = v; }
-    if ((dg || waiba > nn) && (d = ve_d2(xj, xa, xb, &v)) < best) { best = d; *r = 1; *s = v; }
-    if ((dg || waiji < 0) && (d = ve_d2(xa, xi, xj, &v)) < best) { best = d; *r = v; *s = 0; }
-    if ((dg || waiji > nn) && (d = ve_d2(xb, xi, xj, &v)) < best) { best = d; *r = v; *s = 1; }
+    bool dg = !(nn > T(1e-24));
+    T best = T(999999999.0), d, v;
+    *r = T(0.0); *s = T(0.0);
+    if ((dg || waiba < T(0.0)) && (d = ve_d2(xi, xa, xb, &v)) < best) { best = d; *r = T(0.0); *s = v; }
+    if ((dg || waiba > nn) && (d = ve_d2(xj, xa, xb, &v)) < best) { best = d; *r = T(1.0); *s = v; }
+    if ((dg || waiji < T(0.0)) && (d = ve_d2(xa, xi, xj, &v)) < best) { best = d; *r = v; *s = T(0.0); }
+    if ((dg || waiji > nn) && (d = ve_d2(xb, xi, xj, &v)) < best) { best = d; *r = v; *s = T(1.0); }
   }
 #pragma unroll
-  for (int k = 0; k < 3; k++) dir->v[k] = xi.v[k] * (1 - *r) + xj.v[k] * *r - xa.v[k] * (1.0 - *s) - xb.v[k] * *s;
+  for (int k = 0; k < 3; k++)
+    dir->v[k] = xi.v[k] * (T(1.0) - *r) + xj.v[k] * *r - xa.v[k] * (T(1.0) - *s) - xb.v[k] * *s;
   return dot(*dir, *dir);
 }
 
 // VertexTriangleDistance (PdIpcGpuFunc.cu:2238-2292); degenerate triangles take
 // the minimum over the three edges.
-__device__ inline double pt_d2(const P3& xi, const P3& xa, const P3& xb, const P3& xc, double* ba, double* bb,
-                               double* bc, P3* dir) {
-  P3 xba = sub(xb, xa), xca = sub(xc, xa), xia = sub(xi, xa);
-  P3 n = cross(xba, xca);
-  double nn = dot(n, n);
-  double wiaca = dot(n, cross(xia, xca));
-  double wbaia = dot(n, cross(xba, xia));
-  if (nn > 1e-24 && wiaca >= 0 && wbaia >= 0 && nn - wiaca - wbaia >= 0) {
+template <class T>
+__host__ __device__ inline T pt_d2(const P3T<T>& xi, const P3T<T>& xa, const P3T<T>& xb, const P3T<T>& xc, T* ba, T* bb, T* bc,
+                          P3T<T>* dir) {
+  P3T<T> xba = sub(xb, xa), xca = sub(xc, xa), xia = sub(xi, xa);
+  P3T<T> n = cross(xba, xca);
+  T nn = dot(n, n);
+  T wiaca = dot(n, cross(xia, xca));
+  T wbaia = dot(n, cross(xba, xia));
+  if (nn > T(1e-24) && wiaca >= T(0.0) && wbaia >= T(0.0) && nn - wiaca - wbaia >= T(0.0)) {
     *bb = wiaca / nn;
     *bc = wbaia / nn;
-    *ba = 1 - *bb - *bc;
+    *ba = T(1.0) - *bb - *bc;
   } else {
-    bool dg = !(nn > 1e-24);
-    double best = 999999999.0, r, d;
-    *ba = 1; *bb = 0; *bc = 0;
-    if ((dg || nn - wiaca - wbaia < 0) && (d = ve_d2(xi, xb, xc, &r)) < best) { best = d; *bb = 1 - r; *bc = r; *ba = 0; }
-    if ((dg || wiaca < 0) && (d = ve_d2(xi, xa, xc, &r)) < best) { best = d; *bb = 0; *bc = r; *ba = 1 - *bb - *bc; }
-    if ((dg || wbaia < 0) && (d = ve_d2(xi, xa, xb, &r)) < best) { best = d; *bb = r; *bc = 0; *ba = 1 - *bb - *bc; }
+    bool dg = !(nn > T(1e-24));
+    T best = T(999999999.0), r, d;
+    *ba = T(1.0); *bb = T(0.0); *bc = T(0.0);
+    if ((dg || nn - wiaca - wbaia < T(0.0)) && (d = ve_d2(xi, xb, xc, &r)) < best) {
+      best = d; *bb = T(1.0) - r; *bc = r; *ba = T(0.0);
+    }
+    if ((dg || wiaca < T(0.0)) && (d = ve_d2(xi, xa, xc, &r)) < best) {
+      best = d; *bb = T(0.0); *bc = r; *ba = T(1.0) - *bb - *bc;
+    }
+    if ((dg || wbaia < T(0.0)) && (d = ve_d2(xi, xa, xb, &r)) < best) {
+      best = d; *bb = r; *bc = T(0.0); *ba = T(1.0) - *bb - *bc;
+    }
   }
 #pragma unroll
   for (int k = 0; k < 3; k++) dir->v[k] = xi.v[k] - xa.v[k] * *ba - xb.v[k] * *bb - xc.v[k] * *bc;
   return dot(*dir, *dir);
 }
 
-__device__ inline double mn2(double a, double b) { return a < b ? a : b; }
-__device__ inline double mx2(double a, double b) { return a > b ? a : b; }
-__device__ inline double mn3(double a, double b, double c) { return a < b ? (a < c ? a : c) : (b < c ? b : c); }
-__device__ inline double mx3(double a, double b, double c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
+template <class T> __host__ __device__ inline T mn2(T a, T b) { return a < b ? a : b; }
+template <class T> __host__ __device__ inline T mx2(T a, T b) { return a > b ? a : b; }
+__host__ __device__ inline double mn3(double a, double b, double c) { return a < b ? (a < c ? a : c) : (b < c ? b : c); }
+__host__ __device__ inline double mx3(double a, double b, double c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
 
-__device__ inline bool pt_ccd_filter(const P3* x, const P3* dx, double dist) {
+__host__ __device__ inline bool pt_ccd_filter(const P3* x, const P3* dx, double dist) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     double tmin = mn3(x[1].v[k], x[2].v[k], x[3].v[k]);
@@ -105,7 +126,7 @@ __device__ inline bool pt_ccd_filter(const P3* x, const P3* dx, double dist) {
   }
   return true;
 }
-__device__ inline bool ee_ccd_filter(const P3* x, const P3* dx, double dist) {
+__host__ __device__ inline bool ee_ccd_filter(const P3* x, const P3* dx, double dist) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     double mina = mn2(x[0].v[k], x[0].v[k] + dx[0].v[k]);
@@ -125,17 +146,18 @@ __device__ inline bool ee_ccd_filter(const P3* x, const P3* dx, double dist) {
   return true;
 }
 
-__device__ inline double ccd_dfun(int type, const P3* x, double* d2out) {
-  double a, b, c;
-  P3 dir;
+template <class T>
+__host__ __device__ inline T ccd_dfun(int type, const P3T<T>* x, T* d2out) {
+  T a, b, c;
+  P3T<T> dir;
   if (type == 0) {
-    double d2 = pt_d2(x[0], x[1], x[2], x[3], &a, &b, &c, &dir);
+    T d2 = pt_d2(x[0], x[1], x[2], x[3], &a, &b, &c, &dir);
     *d2out = d2;
     return d2;
   }
-  double d2 = ee_d2(x[0], x[1], x[2], x[3], &a, &b, &dir);
-  if (d2 <= 0) {
-    P3 e = sub(x[0], x[2]);
+  T d2 = ee_d2(x[0], x[1], x[2], x[3], &a, &b, &dir);
+  if (d2 <= T(0.0)) {
+    P3T<T> e = sub(x[0], x[2]);
     d2 = dot(e, e);
     e = sub(x[0], x[3]); d2 = mn2(dot(e, e), d2);
     e = sub(x[1], x[2]); d2 = mn2(dot(e, e), d2);
@@ -146,48 +168,49 @@ __device__ inline double ccd_dfun(int type, const P3* x, double* d2out) {
 }
 
 // ACCD with the reference's constants as parameters
-__device__ inline double accd(int type, P3* x, P3* dx, int iters, double step, double gapf, int touchPolicy) {
-  P3 mov;
+template <class T>
+__host__ __device__ inline T accd(int type, P3T<T>* x, P3T<T>* dx, int iters, double step, double gapf, int touchPolicy) {
+  P3T<T> mov;
 #pragma unroll
-  for (int k = 0; k < 3; k++) mov.v[k] = (dx[0].v[k] + dx[1].v[k] + dx[2].v[k] + dx[3].v[k]) / 4.0;
+  for (int k = 0; k < 3; k++) mov.v[k] = (dx[0].v[k] + dx[1].v[k] + dx[2].v[k] + dx[3].v[k]) / T(4.0);
 #pragma unroll
   for (int i = 0; i < 4; i++)
 #pragma unroll
     for (int k = 0; k < 3; k++) dx[i].v[k] -= mov.v[k];
-  double maxdisp;
+  T maxdisp;
   if (type == 0) {
-    double m2 = mx2(dot(dx[1], dx[1]), dot(dx[2], dx[2]));
+    T m2 = mx2(dot(dx[1], dx[1]), dot(dx[2], dx[2]));
     m2 = mx2(m2, dot(dx[3], dx[3]));
     maxdisp = sqrt(dot(dx[0], dx[0])) + sqrt(m2);
   } else {
-    double m0 = mx2(dot(dx[0], dx[0]), dot(dx[1], dx[1]));
-    double m1 = mx2(dot(dx[2], dx[2]), dot(dx[3], dx[3]));
+    T m0 = mx2(dot(dx[0], dx[0]), dot(dx[1], dx[1]));
+    T m1 = mx2(dot(dx[2], dx[2]), dot(dx[3], dx[3]));
     maxdisp = sqrt(m0) + sqrt(m1);
   }
-  if (maxdisp == 0.0) return 1.0;
-  double d2;
-  double dF = ccd_dfun(type, x, &d2);
-  double dcur = sqrt(d2);
-  if (touchPolicy == 1 && !(dcur > 0)) return 0.0;
-  double gap = gapf * dF / dcur;
-  double toc = 0.0;
+  if (maxdisp == T(0.0)) return T(1.0);
+  T d2;
+  T dF = ccd_dfun(type, x, &d2);
+  T dcur = sqrt(d2);
+  if (touchPolicy == 1 && !(dcur > T(0.0))) return T(0.0);
+  T gap = T(gapf) * dF / dcur;
+  T toc = T(0.0);
   for (int i = 0; i < iters; i++) {
-    double tlb = step * dF / (dcur * maxdisp);
+    T tlb = T(step) * dF / (dcur * maxdisp);
 #pragma unroll
     for (int j = 0; j < 4; j++)
 #pragma unroll
       for (int k = 0; k < 3; k++) x[j].v[k] += tlb * dx[j].v[k];
     dF = ccd_dfun(type, x, &d2);
     dcur = sqrt(d2);
-    if (touchPolicy == 1 && !(dcur > 0)) return toc;
-    if (toc != 0.0 && (dF / dcur) < gap) break;
+    if (touchPolicy == 1 && !(dcur > T(0.0))) return toc;
+    if (toc != T(0.0) && (dF / dcur) < gap) break;
     toc += tlb;
-    if (toc > 1.0) { toc = 1.0; break; }
+    if (toc > T(1.0)) { toc = T(1.0); break; }
   }
   return toc;
 }
 
-__device__ inline P3 load3(const double* X, int ld, int i) { return {{X[i], X[ld + i], X[2 * ld + i]}}; }
+__host__ __device__ inline P3 load3(const double* X, int ld, int i) { return {{X[i], X[ld + i], X[2 * ld + i]}}; }
 
 __global__ void k_toi_init(unsigned long long* bits) { *bits = (unsigned long long)__double_as_longlong(1.0); }
 
@@ -226,87 +249,331 @@ __global__ void __launch_bounds__(kBlock) k_ccd(int nRecs, int ld, const int4* _
 }
 
 // barrier (PdIpcGpuFunc.cu:2745-2754)
-__device__ inline double barrier(double d2, double dh2, double kappa) {
-  double e = 0.0;
-  if (d2 < dh2) {
-    double t2 = d2 - dh2;
-    e = -kappa * t2 * t2 * log(d2 / dh2) / (dh2 * dh2);
+template <class T>
+__host__ __device__ inline T barrier(T d2, double dh2, double kappa) {
+  T e = T(0.0);
+  if (d2 < T(dh2)) {
+    T t2 = d2 - T(dh2);
+    e = T(-kappa) * t2 * t2 * log(d2 / T(dh2)) / T(dh2 * dh2);
   }
   return e;
 }
 
-__global__ void __launch_bounds__(kBlock) k_collision(int nRecs, int ld, const int4* __restrict__ recs,
+// the position filter of setCollisionConstraintsKernel (PdIpcGpuFunc.cu:2327-2423)
+__host__ __device__ inline bool cd_filter(int type, const P3* x, double dHat) {
+  bool pass = true;
+  if (type == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double tmin = mn3(x[1].v[k], x[2].v[k], x[3].v[k]), tmax = mx3(x[1].v[k], x[2].v[k], x[3].v[k]);
+      if ((x[0].v[k] - tmax) > dHat || (tmin - x[0].v[k]) > dHat) pass = false;
+    }
+  } else {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double mina = mn2(x[0].v[k], x[1].v[k]), maxa = mx2(x[0].v[k], x[1].v[k]);
+      double minb = mn2(x[2].v[k], x[3].v[k]), maxb = mx2(x[2].v[k], x[3].v[k]);
+      if ((mina - maxb) > dHat || (minb - maxa) > dHat) pass = false;
+    }
+  }
+  return pass;
+}
+
+// One record's barrier projection (setCollisionConstraintsKernel,
+// PdIpcGpuFunc.cu:2756-2892): returns false if inactive (d^2 >= dHat^2), else
+// the weight w and the four targets x_q + shift_q * dir.
+template <class T>
+__host__ __device__ inline bool pair_terms(int type, const P3T<T>* x, const double* cmq, double dHat, double kappa, double dt,
+                                  T* wOut, P3T<T>* tgt) {
+  const double dh2 = dHat * dHat;
+  P3T<T> dir;
+  T d2, mp, mq;
+  double sgn[4];
+  if (type == 0) {
+    T a, b, c;
+    d2 = pt_d2(x[0], x[1], x[2], x[3], &a, &b, &c, &dir);
+    mp = T(cmq[0]);
+    mq = a * T(cmq[1]) + b * T(cmq[2]) + c * T(cmq[3]);
+    sgn[0] = 1; sgn[1] = -1; sgn[2] = -1; sgn[3] = -1;
+  } else {
+    T rr, ss;
+    d2 = ee_d2(x[0], x[1], x[2], x[3], &rr, &ss, &dir);
+    mp = (T(1.0) - rr) * T(cmq[0]) + rr * T(cmq[1]);
+    mq = (T(1.0) - ss) * T(cmq[2]) + ss * T(cmq[3]);
+    sgn[0] = 1; sgn[1] = 1; sgn[2] = -1; sgn[3] = -1;
+  }
+  if (!(d2 < T(dh2))) return false;
+  T dist = sqrt(d2);
+  T w = barrier(d2, dh2, kappa) + mn2(mp, mq) / T(dt * dt);
+  T rp = mq / (mp + mq), rq = mp / (mp + mq);
+  T len = sqrt(dot(dir, dir));
+#pragma unroll
+  for (int k = 0; k < 3; k++) dir.v[k] /= len;
+  T gapd = T(dHat) - dist;
+  T dp = gapd * rp, dq = gapd * rq;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    T sh = sgn[q] > 0 ? dp : -dq;
+#pragma unroll
+    for (int k = 0; k < 3; k++) tgt[q].v[k] = x[q].v[k] + sh * dir.v[k];
+  }
+  *wOut = w;
+  return true;
+}
+
+__host__ __device__ inline bool record_ids(int4 r, int id[4]) {  // false for shared-vertex pairs (skipped)
+  int type = r.x < 0 ? 1 : 0;
+  id[0] = type ? ~r.x : r.x; id[1] = r.y; id[2] = r.z; id[3] = r.w;
+  bool skip = type == 0 ? (id[0] == id[1] || id[0] == id[2] || id[0] == id[3])
+                        : (id[0] == id[2] || id[0] == id[3] || id[1] == id[2] || id[1] == id[3]);
+  return !skip;
+}
+
+// The collision terms are summed per point in a fixed order (ascending record,
+// then corner), so the step is bit-reproducible: no floating-point atomics.
+//   1. k_col_terms: each active record appends (wave-aggregated) its weight and
+//      four targets to a compact list and counts its simulated points;
+//   2. exclusive scan of the per-point counts;
+//   3. k_col_fill: every (point, record, corner) entry goes to its point's range;
+//   4. k_col_gather: per point, the range is ordered by (record, corner) and summed.
+__global__ void __launch_bounds__(kBlock) k_col_terms(int nRecs, int ld, int nSim, const int4* __restrict__ recs,
                                                       const double* __restrict__ X, const double* __restrict__ cm,
-                                                      double dHat, double kappa, double dt, double* __restrict__ diag,
-                                                      double* __restrict__ rhs, int* __restrict__ activeCount) {
+                                                      double dHat, double kappa, double dt, int actCap,
+                                                      int* __restrict__ actRec, double* __restrict__ actW,
+                                                      double* __restrict__ actT, int* __restrict__ ptCount,
+                                                      int* __restrict__ counters) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int lane = threadIdx.x % kWave;
   bool active = false;
+  double w = 0.0;
+  P3 tgt[4];
+  int id[4] = {0, 0, 0, 0};
   if (t < nRecs) {
     int4 r = recs[t];
     int type = r.x < 0 ? 1 : 0;
-    int id[4] = {type ? ~r.x : r.x, r.y, r.z, r.w};
-    bool skip = type == 0 ? (id[0] == id[1] || id[0] == id[2] || id[0] == id[3])
-                          : (id[0] == id[2] || id[0] == id[3] || id[1] == id[2] || id[1] == id[3]);
-    if (!skip) {
+    if (record_ids(r, id)) {
       P3 x[4];
 #pragma unroll
       for (int q = 0; q < 4; q++) x[q] = load3(X, ld, id[q]);
-      bool pass = true;
-      if (type == 0) {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          double tmin = mn3(x[1].v[k], x[2].v[k], x[3].v[k]), tmax = mx3(x[1].v[k], x[2].v[k], x[3].v[k]);
-          if ((x[0].v[k] - tmax) > dHat || (tmin - x[0].v[k]) > dHat) pass = false;
-        }
-      } else {
-#pragma unroll
-        for (int k = 0; k < 3; k++) {
-          double mina = mn2(x[0].v[k], x[1].v[k]), maxa = mx2(x[0].v[k], x[1].v[k]);
-          double minb = mn2(x[2].v[k], x[3].v[k]), maxb = mx2(x[2].v[k], x[3].v[k]);
-          if ((mina - maxb) > dHat || (minb - maxa) > dHat) pass = false;
-        }
-      }
-      if (pass) {
-        const double dh2 = dHat * dHat;
-        P3 dir;
-        double d2, mp, mq;
-        double sgn[4];
-        if (type == 0) {
-          double a, b, c;
-          d2 = pt_d2(x[0], x[1], x[2], x[3], &a, &b, &c, &dir);
-          mp = cm[id[0]];
-          mq = a * cm[id[1]] + b * cm[id[2]] + c * cm[id[3]];
-          sgn[0] = 1; sgn[1] = -1; sgn[2] = -1; sgn[3] = -1;
-        } else {
-          double rr, ss;
-          d2 = ee_d2(x[0], x[1], x[2], x[3], &rr, &ss, &dir);
-          mp = (1 - rr) * cm[id[0]] + rr * cm[id[1]];
-          mq = (1 - ss) * cm[id[2]] + ss * cm[id[3]];
-          sgn[0] = 1; sgn[1] = 1; sgn[2] = -1; sgn[3] = -1;
-        }
-        if (d2 < dh2) {
-          active = true;
-          double dist = sqrt(d2);
-          double w = barrier(d2, dh2, kappa) + mn2(mp, mq) / (dt * dt);
-          double rp = mq / (mp + mq), rq = mp / (mp + mq);
-          double len = sqrt(dot(dir, dir));
-#pragma unroll
-          for (int k = 0; k < 3; k++) dir.v[k] /= len;
-          double gapd = dHat - dist;
-          double dp = gapd * rp, dq = gapd * rq;
-#pragma unroll
-          for (int q = 0; q < 4; q++) {
-            atomicAdd(diag + id[q], w);
-            double sh = sgn[q] > 0 ? dp : -dq;
-#pragma unroll
-            for (int k = 0; k < 3; k++) atomicAdd(rhs + k * ld + id[q], w * (x[q].v[k] + sh * dir.v[k]));
-          }
-        }
+      if (cd_filter(type, x, dHat)) {
+        double cmq[4] = {cm[id[0]], cm[id[1]], cm[id[2]], cm[id[3]]};
+        active = pair_terms(type, x, cmq, dHat, kappa, dt, &w, tgt);
       }
     }
   }
   unsigned long long b = __ballot(active);
-  if ((threadIdx.x % kWave) == 0 && b) atomicAdd(activeCount, (int)__popcll(b));
+  if (!b) return;
+  int leader = __ffsll((long long)b) - 1;
+  int base = 0;
+  if (lane == leader) base = atomicAdd(counters + 2, (int)__popcll(b));
+  base = __shfl(base, leader, kWave);
+  if (!active) return;
+  const unsigned long long lt = lane == 0 ? 0ull : (~0ull >> (64 - lane));
+  int a = base + (int)__popcll(b & lt);
+  if (a >= actCap) {
+    atomicOr(counters + 5, 1);  // the host grows the list and repeats the pass
+    return;
+  }
+  actRec[a] = t;
+  actW[a] = w;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) actT[(int64_t)a * 12 + 3 * q + k] = tgt[q].v[k];
+    if (id[q] < nSim) atomicAdd(ptCount + id[q], 1);
+  }
+}
+
+__global__ void k_col_fill(int ld, int nSim, int actCap, const int4* __restrict__ recs, const int* __restrict__ counters,
+                           const int* __restrict__ actRec, const int* __restrict__ ptOff, int* __restrict__ ptCursor,
+                           unsigned long long* __restrict__ entKey, int* __restrict__ entA) {
+  int A = min(counters[2], actCap);  // an overflowed pass is repeated by the host
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < A; a += gridDim.x * blockDim.x) {
+    int t = actRec[a];
+    int id[4];
+    record_ids(recs[t], id);
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      if (id[q] >= nSim) continue;
+      int slot = ptOff[id[q]] + atomicAdd(ptCursor + id[q], 1);
+      entKey[slot] = ((unsigned long long)t << 2) | (unsigned)q;
+      entA[slot] = a;
+    }
+  }
+}
+
+__global__ void k_col_gather(int nSim, int ld, const int* __restrict__ ptOff, unsigned long long* __restrict__ entKey,
+                             int* __restrict__ entA, const double* __restrict__ actW, const double* __restrict__ actT,
+                             double* __restrict__ diag, double* __restrict__ rhs) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nSim) return;
+  int o = ptOff[i], e = ptOff[i + 1];
+  for (int p = o + 1; p < e; p++) {  // insertion sort: a point has a handful of entries
+    unsigned long long k = entKey[p];
+    int av = entA[p];
+    int q = p - 1;
+    while (q >= o && entKey[q] > k) {
+      entKey[q + 1] = entKey[q];
+      entA[q + 1] = entA[q];
+      q--;
+    }
+    entKey[q + 1] = k;
+    entA[q + 1] = av;
+  }
+  double d = 0.0, r[3] = {0.0, 0.0, 0.0};
+  for (int p = o; p < e; p++) {
+    int a = entA[p];
+    int q = (int)(entKey[p] & 3ull);
+    double w = actW[a];
+    d += w;
+#pragma unroll
+    for (int k = 0; k < 3; k++) r[k] += w * actT[(int64_t)a * 12 + 3 * q + k];
+  }
+  diag[i] = d;
+#pragma unroll
+  for (int k = 0; k < 3; k++) rhs[k * ld + i] = r[k];
+}
+
+// ---- adjoint of the narrow phase (active sets and candidate records frozen)
+
+// For every active record at X: with colDbar / colRbar the adjoints of the
+// collision diagonal and rhs (rows of simulated points), accumulate into ax the
+// gradient of  sum_q colDbar_q w + colRbar_q . (w target_q)  with respect to the
+// simulated points' positions (forward-mode over the 12 coordinates).
+__global__ void __launch_bounds__(64) k_adj_collision(int nRecs, int ld, int nSim, const int4* __restrict__ recs,
+                                                      const double* __restrict__ X, const double* __restrict__ cm,
+                                                      double dHat, double kappa, double dt,
+                                                      const double* __restrict__ colDbar,
+                                                      const double* __restrict__ colRbar, double* __restrict__ ax) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nRecs) return;
+  int4 r = recs[t];
+  int type = r.x < 0 ? 1 : 0;
+  int id[4];
+  if (!record_ids(r, id)) return;
+  P3 x[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) x[q] = load3(X, ld, id[q]);
+  if (!cd_filter(type, x, dHat)) return;
+  // activity in double first (the same test the step made)
+  double cmq[4] = {cm[id[0]], cm[id[1]], cm[id[2]], cm[id[3]]};
+  {
+    double w;
+    P3 tgt[4];
+    if (!pair_terms(type, x, cmq, dHat, kappa, dt, &w, tgt)) return;
+  }
+  double gd[4], gr[4][3];
+  bool any = false;
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    bool simq = id[q] < nSim;
+    gd[q] = simq ? colDbar[id[q]] : 0.0;
+#pragma unroll
+    for (int k = 0; k < 3; k++) gr[q][k] = simq ? colRbar[k * ld + id[q]] : 0.0;
+    any = any || gd[q] != 0.0 || gr[q][0] != 0.0 || gr[q][1] != 0.0 || gr[q][2] != 0.0;
+  }
+  if (!any) return;
+  using D12 = Dual<12>;
+  P3T<D12> xd[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) xd[q].v[k] = D12::seed(x[q].v[k], 3 * q + k);
+  D12 w;
+  P3T<D12> tgt[4];
+  pair_terms(type, xd, cmq, dHat, kappa, dt, &w, tgt);
+  D12 L = D12(0.0);
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    D12 acc = D12(gd[q]);
+#pragma unroll
+    for (int k = 0; k < 3; k++) acc += D12(gr[q][k]) * tgt[q].v[k];
+    L += w * acc;  // sum_q gd_q w + gr_q . (w tgt_q)
+  }
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (id[q] < nSim)
+#pragma unroll
+      for (int k = 0; k < 3; k++) atomicAdd(ax + k * ld + id[q], L.d[3 * q + k]);
+}
+
+// Per-record TOI of the taped step (2.0 where a record is skipped or filtered),
+// so that the record attaining the minimum can be found afterwards.
+__global__ void __launch_bounds__(kBlock) k_ccd_records(int nRecs, int ld, const int4* __restrict__ recs,
+                                                        const double* __restrict__ X0, const double* __restrict__ X1,
+                                                        double filter, int iters, double step, double gapf,
+                                                        int touchPolicy, double* __restrict__ recToi,
+                                                        unsigned long long* __restrict__ toiBits) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int4 r = t < nRecs ? recs[t] : make_int4(0, 0, 0, 0);
+  int type = r.x < 0 ? 1 : 0;
+  int id[4];
+  double toi = 2.0;
+  if (t < nRecs && record_ids(r, id)) {
+    P3 x[4], dx[4];
+#pragma unroll
+    for (int q = 0; q < 4; q++) {
+      x[q] = load3(X0, ld, id[q]);
+      P3 y = load3(X1, ld, id[q]);
+      dx[q] = sub(y, x[q]);
+    }
+    bool pass = type == 0 ? pt_ccd_filter(x, dx, filter) : ee_ccd_filter(x, dx, filter);
+    if (pass) {
+      double v = accd(type, x, dx, iters, step, gapf, touchPolicy);
+      if (v < 2.0) toi = v;  // NaN stays 2
+    }
+  }
+  if (t < nRecs) recToi[t] = toi;
+  double mn = toi;
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    double other = __shfl_xor(mn, o, kWave);
+    mn = other < mn ? other : mn;
+  }
+  if ((threadIdx.x % kWave) == 0 && mn < 1.0) atomicMin(toiBits, (unsigned long long)__double_as_longlong(mn));
+}
+
+// first record whose TOI equals the device minimum (only when the minimum < 1)
+__global__ void k_toi_argmin(int nRecs, const double* __restrict__ recToi, const unsigned long long* __restrict__ toiBits,
+                             int* __restrict__ argmin) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nRecs) return;
+  double tmin = __longlong_as_double((long long)*toiBits);
+  if (tmin < 1.0 && recToi[t] == tmin) atomicMin(argmin, t);
+}
+
+// d toi / d (x0, x1) of one record, scaled by g, added to ax0 / ax1 (simulated points)
+__global__ void k_adj_ccd(int ld, int nSim, const int4* __restrict__ recs, int argmin,
+                          const double* __restrict__ X0, const double* __restrict__ X1, double filter, int iters,
+                          double step, double gapf, int touchPolicy, const double* __restrict__ gscale,
+                          double* __restrict__ ax0, double* __restrict__ ax1) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  int t = argmin;
+  if (t < 0) return;
+  double g = *gscale;
+  if (g == 0.0) return;
+  int4 r = recs[t];
+  int type = r.x < 0 ? 1 : 0;
+  int id[4];
+  record_ids(r, id);
+  using D24 = Dual<24>;
+  P3T<D24> x[4], dx[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      x[q].v[k] = D24::seed(X0[k * ld + id[q]], 3 * q + k);
+      D24 y = D24::seed(X1[k * ld + id[q]], 12 + 3 * q + k);
+      dx[q].v[k] = y - x[q].v[k];
+    }
+  D24 toc = accd(type, x, dx, iters, step, gapf, touchPolicy);
+#pragma unroll
+  for (int q = 0; q < 4; q++)
+    if (id[q] < nSim)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        atomicAdd(ax0 + k * ld + id[q], g * toc.d[3 * q + k]);
+        atomicAdd(ax1 + k * ld + id[q], g * toc.d[12 + 3 * q + k]);
+      }
 }
 
 }  // namespace
@@ -322,10 +589,45 @@ void launch_ccd(const DeviceMesh& m, int nRecs, double filter, int iters, double
                      filter, iters, step, gap, touchPolicy, m.toiBits);
 }
 
-void launch_collision(const DeviceMesh& m, int nRecs, double dhat, double kappa, double dt, hipStream_t s) {
+void launch_ccd_records(const DeviceMesh& m, const int4* recs, int nRecs, const double* x0, const double* x1,
+                        double filter, int iters, double step, double gap, int touchPolicy, double* recToi,
+                        int* argmin, hipStream_t s) {
   if (nRecs <= 0) return;
-  hipLaunchKernelGGL(k_collision, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, m.ld, m.recs, m.X,
-                     m.colMass, dhat, kappa, dt, m.colDiag, m.colRhs, m.counters + 2);
+  hipLaunchKernelGGL(k_ccd_records, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, m.ld, recs, x0, x1,
+                     filter, iters, step, gap, touchPolicy, recToi, m.toiBits);
+  hipLaunchKernelGGL(k_toi_argmin, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, recToi, m.toiBits,
+                     argmin);
+}
+
+void launch_adj_collision(const DeviceMesh& m, const int4* recs, int nRecs, const double* x, double dhat,
+                          double kappa, double dt, const double* colDbar, const double* colRbar, double* ax,
+                          hipStream_t s) {
+  if (nRecs <= 0) return;
+  hipLaunchKernelGGL(k_adj_collision, dim3((nRecs + 63) / 64), dim3(64), 0, s, nRecs, m.ld, m.nSim, recs, x,
+                     m.colMass, dhat, kappa, dt, colDbar, colRbar, ax);
+}
+
+void launch_adj_ccd(const DeviceMesh& m, const int4* recs, int argmin, const double* x0, const double* x1,
+                    double filter, int iters, double step, double gap, int touchPolicy, const double* gscale,
+                    double* ax0, double* ax1, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_ccd, dim3(1), dim3(64), 0, s, m.ld, m.nSim, recs, argmin, x0, x1, filter, iters, step, gap,
+                     touchPolicy, gscale, ax0, ax1);
+}
+
+int launch_collision(DeviceMesh& m, int nRecs, double dhat, double kappa, double dt, hipStream_t s) {
+  // counters[2] (active records) and counters[5] (list overflow) are cleared by the caller
+  hipMemsetAsync(m.ptCount, 0, ((size_t)m.nSim + 1) * sizeof(int), s);
+  hipMemsetAsync(m.ptCursor, 0, (size_t)m.nSim * sizeof(int), s);
+  if (nRecs > 0)
+    hipLaunchKernelGGL(k_col_terms, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, m.ld, m.nSim,
+                       m.recs, m.X, m.colMass, dhat, kappa, dt, m.actCap, m.actRec, m.actW, m.actT, m.ptCount,
+                       m.counters);
+  launch_exclusive_scan(m.ptCount, m.ptOff, m.nSim + 1, nullptr, m.ptScan, s);
+  hipLaunchKernelGGL(k_col_fill, dim3(512), dim3(kBlock), 0, s, m.ld, m.nSim, m.actCap, m.recs, m.counters, m.actRec, m.ptOff,
+                     m.ptCursor, m.entKey, m.entA);
+  hipLaunchKernelGGL(k_col_gather, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, m.ptOff,
+                     m.entKey, m.entA, m.actW, m.actT, m.colDiag, m.colRhs);
+  return 0;
 }
 
 }  // namespace pdipc

@@ -276,7 +276,237 @@ void local_impl(const DeviceMesh& m, T lo, T hi, int withCol, hipStream_t s) {
                      m.clothA, m.X, m.colRhs, m.fixK, m.Xrest, m.sysRhs);
 }
 
+// ---- adjoint of the local step (FP64), evaluated at the recorded x
+
+// cloth: abar_i = sum_slots bend(i -> slot) bbar_slot, pulled back through the
+// rescale a -> a H0 / |a| (Jacobian (H0/|a|)(I - a_hat a_hat^T), symmetric)
+__global__ void k_adj_cloth_a(int nCloth, int ld, const int* __restrict__ cCenter, const int* __restrict__ cOff,
+                              const int* __restrict__ cNbr, const double* __restrict__ cLbo,
+                              const double* __restrict__ cH0, const double* __restrict__ cBendOut,
+                              const double* __restrict__ X, const double* __restrict__ bbar,
+                              double* __restrict__ Aadj) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= nCloth) return;
+  int i = cCenter[c];
+  int o = cOff[c], e = cOff[c + 1];
+  double ab[3] = {0, 0, 0};
+  for (int s = o; s < e; s++) {
+    int j = cNbr[s];
+    double b = cBendOut[s];
+#pragma unroll
+    for (int k = 0; k < 3; k++) ab[k] += b * bbar[k * ld + j];
+  }
+  double xi[3] = {X[i], X[ld + i], X[2 * ld + i]};
+  double lsum = cLbo[o];
+  double a[3] = {0, 0, 0};
+  for (int s = o + 1; s < e; s++) {
+    int j = cNbr[s];
+    double l = cLbo[s];
+    lsum += l;
+#pragma unroll
+    for (int k = 0; k < 3; k++) a[k] += l * (X[k * ld + j] - xi[k]);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) a[k] += lsum * xi[k];
+  double n = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
+  if (n >= 1e-10) {
+    double u[3] = {a[0] / n, a[1] / n, a[2] / n};
+    double pr = u[0] * ab[0] + u[1] * ab[1] + u[2] * ab[2];
+    double sc = cH0[c] / n;
+#pragma unroll
+    for (int k = 0; k < 3; k++) ab[k] = sc * (ab[k] - u[k] * pr);
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) Aadj[k * ld + i] = ab[k];
+}
+
+// Tets: Pbar = w sum_c bbar_c h_c^T, Fbar = dP(F)^T[Pbar], xbar_c = Fbar h_c.
+// P(F) = U g(S) V^T with g the flip / re-sort / clamp of the singular values;
+// with Ft = U^T dF V:  dPt_kk = sum_m dg_k/ds_m Ft_mm, and for i != j
+//   dPt_ij = A (Ft_ij + Ft_ji)/2 + B (Ft_ij - Ft_ji)/2,
+//   A = (g_j - g_i)/(s_j - s_i),  B = (g_i + g_j)/(s_i + s_j)
+// (the differential of U diag(g) V^T through dU = U Omega_U, dV = V Omega_V);
+// the adjoint swaps Pbar_ij, Pbar_ji in the same combination.
+__global__ void k_adj_tet(int nT, int ld, double sigLo, double sigHi, const int4* __restrict__ tets,
+                          const double* __restrict__ dminv, const double* __restrict__ w,
+                          const double* __restrict__ X, const double* __restrict__ bbar, double* __restrict__ out) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nT) return;
+  int4 q = tets[t];
+  int id[4] = {q.x, q.y, q.z, q.w};
+  double Ds[9];
+#pragma unroll
+  for (int r = 0; r < 3; r++) {
+    double x0 = X[r * ld + id[0]];
+#pragma unroll
+    for (int c = 0; c < 3; c++) Ds[3 * r + c] = X[r * ld + id[c + 1]] - x0;
+  }
+  double idm[9];
+#pragma unroll
+  for (int k = 0; k < 9; k++) idm[k] = dminv[(int64_t)k * nT + t];
+  double F[9];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int c = 0; c < 3; c++) F[3 * r + c] = Ds[3 * r] * idm[c] + Ds[3 * r + 1] * idm[3 + c] + Ds[3 * r + 2] * idm[6 + c];
+  double U[9], s[3], V[9];
+  svd3<double>(F, U, s, V);
+  // g_k and dg_k/ds_{src[k]} as in the forward projection
+  double sv[3] = {s[0], s[1], s[2]}, sgn[3] = {1.0, 1.0, 1.0};
+  int src[3] = {0, 1, 2};
+  if (s[0] * s[1] * s[2] < 0) {
+    sv[2] = -sv[2];
+    sgn[2] = -1.0;
+    // the forward's three compare-swaps on (hi, mid, lo), tracking where each value came from
+    int ih = 0, im = 1, il = 2;
+    double hi = sv[0], mid = sv[1], lo = sv[2], tt;
+    int ti;
+    if (mid < lo) { tt = lo; lo = mid; mid = tt; ti = il; il = im; im = ti; }
+    if (hi < lo) { tt = lo; lo = hi; hi = tt; ti = il; il = ih; ih = ti; }
+    if (hi < mid) { tt = mid; mid = hi; hi = tt; ti = im; im = ih; ih = ti; }
+    src[0] = ih; src[1] = im; src[2] = il;
+  }
+  double g[3], dg[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double v = sv[src[k]];
+    double r = v < sigHi ? v : sigHi;
+    g[k] = r < sigLo ? sigLo : r;
+    dg[k] = (v < sigHi && !(v < sigLo)) ? sgn[src[k]] : 0.0;  // d g_k / d s_{src[k]}
+  }
+  // Pbar
+  double h[4][3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    h[1][k] = idm[k];
+    h[2][k] = idm[3 + k];
+    h[3][k] = idm[6 + k];
+    h[0][k] = -(h[1][k] + h[2][k] + h[3][k]);
+  }
+  double wt = w[t];
+  double bb[4][3];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int r = 0; r < 3; r++) bb[c][r] = bbar[r * ld + id[c]];
+  double Pb[9];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int k = 0; k < 3; k++)
+      Pb[3 * r + k] = wt * (bb[0][r] * h[0][k] + bb[1][r] * h[1][k] + bb[2][r] * h[2][k] + bb[3][r] * h[3][k]);
+  // Pt = U^T Pbar V
+  double T1[9], Pt[9];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) T1[3 * i + k] = U[i] * Pb[k] + U[3 + i] * Pb[3 + k] + U[6 + i] * Pb[6 + k];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) Pt[3 * i + j] = T1[3 * i] * V[j] + T1[3 * i + 1] * V[3 + j] + T1[3 * i + 2] * V[6 + j];
+  double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+#pragma unroll
+  for (int k = 0; k < 3; k++) Ft[4 * src[k]] += dg[k] * Pt[4 * k];
+#pragma unroll
+  for (int i = 0; i < 3; i++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) {
+      if (i == j) continue;
+      double ds = s[j] - s[i], ss = s[i] + s[j];
+      double scale = fabs(s[i]) + fabs(s[j]) + 1.0;
+      double A = fabs(ds) > 1e-12 * scale ? (g[j] - g[i]) / ds : 0.5 * (fabs(dg[i]) + fabs(dg[j]));
+      double B = fabs(ss) > 1e-12 * scale ? (g[i] + g[j]) / ss : 0.0;
+      Ft[3 * i + j] = 0.5 * (A + B) * Pt[3 * i + j] + 0.5 * (A - B) * Pt[3 * j + i];
+    }
+  // Fbar = U Ft V^T
+  double T2[9], Fb[9];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int j = 0; j < 3; j++) T2[3 * r + j] = U[3 * r] * Ft[j] + U[3 * r + 1] * Ft[3 + j] + U[3 * r + 2] * Ft[6 + j];
+#pragma unroll
+  for (int r = 0; r < 3; r++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) Fb[3 * r + k] = T2[3 * r] * V[3 * k] + T2[3 * r + 1] * V[3 * k + 1] + T2[3 * r + 2] * V[3 * k + 2];
+#pragma unroll
+  for (int c = 0; c < 4; c++)
+#pragma unroll
+    for (int r = 0; r < 3; r++)
+      out[(int64_t)(c * 3 + r) * nT + t] = Fb[3 * r] * h[c][0] + Fb[3 * r + 1] * h[c][1] + Fb[3 * r + 2] * h[c][2];
+}
+
+// per point: ax = D u0 + (local step)^T bbar; spring-weight adjoints per slot;
+// bbar also flows to rhs0 and (after the first outer iteration) the collision rhs
+__global__ void k_adj_gather(int nSim, int ld, int nT, const double* __restrict__ D, const double* __restrict__ u0,
+                             const int* __restrict__ vtOff, const int* __restrict__ vtList,
+                             const double* __restrict__ tetAdj, const int* __restrict__ clothOfPoint,
+                             const int* __restrict__ cOff, const int* __restrict__ cNbr,
+                             const double* __restrict__ cRest, const double* __restrict__ cStrain,
+                             const double* __restrict__ cLboIn, const double* __restrict__ Aadj,
+                             const double* __restrict__ X, const double* __restrict__ bbar, double* __restrict__ ax,
+                             double* __restrict__ sbar, double* __restrict__ rhs0bar, double* __restrict__ colRbar) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nSim) return;
+  double di = D[i];
+  double bi[3] = {bbar[i], bbar[ld + i], bbar[2 * ld + i]};
+  double r[3] = {di * u0[i], di * u0[ld + i], di * u0[2 * ld + i]};
+  for (int q = vtOff[i]; q < vtOff[i + 1]; q++) {
+    int tc = vtList[q];
+    int t = tc >> 2, corner = tc & 3;
+#pragma unroll
+    for (int k = 0; k < 3; k++) r[k] += tetAdj[(int64_t)(corner * 3 + k) * nT + t];
+  }
+  int c = clothOfPoint[i];
+  if (c >= 0) {
+    int o = cOff[c], e = cOff[c + 1];
+    double xi[3] = {X[i], X[ld + i], X[2 * ld + i]};
+    for (int s = o + 1; s < e; s++) {
+      int j = cNbr[s];
+      double d[3], db[3];
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        d[k] = xi[k] - X[k * ld + j];
+        db[k] = bi[k] - bbar[k * ld + j];
+      }
+      double len = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+      double n[3] = {d[0] / len, d[1] / len, d[2] / len};
+      double coef = cStrain[s] * cRest[s] / len;
+      double pr = n[0] * db[0] + n[1] * db[1] + n[2] * db[2];
+#pragma unroll
+      for (int k = 0; k < 3; k++) r[k] += coef * (db[k] - n[k] * pr);
+      sbar[s] += cRest[s] * (n[0] * bi[0] + n[1] * bi[1] + n[2] * bi[2]);
+    }
+    for (int s = o; s < e; s++) {
+      int j = cNbr[s];
+      double l = cLboIn[s];
+#pragma unroll
+      for (int k = 0; k < 3; k++) r[k] += l * Aadj[k * ld + j];
+    }
+  }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    ax[k * ld + i] = r[k];
+    rhs0bar[k * ld + i] += bi[k];
+    if (colRbar) colRbar[k * ld + i] += bi[k];
+  }
+}
+
 }  // namespace
+
+void launch_adj_local(const DeviceMesh& m, const AdjMesh& a, double sigLo, double sigHi, const double* x,
+                      const double* bbar, const double* u0, double* aA, double* tetAdj, double* ax, double* sbar,
+                      double* rhs0bar, double* colRbar, hipStream_t s) {
+  if (m.nCloth > 0)
+    hipLaunchKernelGGL(k_adj_cloth_a, dim3(grid_for(m.nCloth)), dim3(kBlock), 0, s, m.nCloth, m.ld, m.cCenter, m.cOff,
+                       m.cNbr, m.cLbo, m.cH0, a.cBendOut, x, bbar, aA);
+  if (m.nTets > 0)
+    hipLaunchKernelGGL(k_adj_tet, dim3(grid_for(m.nTets)), dim3(kBlock), 0, s, m.nTets, m.ld, sigLo, sigHi, m.tetIdx,
+                       m.tetDminv, m.tetW, x, bbar, tetAdj);
+  hipLaunchKernelGGL(k_adj_gather, dim3(grid_for(m.nSim)), dim3(kBlock), 0, s, m.nSim, m.ld, m.nTets, m.sysDiag, u0,
+                     m.vtOff, m.vtList, tetAdj, m.clothOfPoint, m.cOff, m.cNbr, m.cRest, m.cStrain, a.cLboIn, aA, x,
+                     bbar, ax, sbar, rhs0bar, colRbar);
+}
 
 void launch_local(const DeviceMesh& m, int precision, double sigLo, double sigHi, int withCollision, hipStream_t s) {
   if (precision == 1) local_impl<float>(m, (float)sigLo, (float)sigHi, withCollision, s);

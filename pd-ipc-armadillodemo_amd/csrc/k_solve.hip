@@ -231,6 +231,144 @@ void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precis
   }
 }
 
+// The recorded solve of the adjoint's forward pass: the FP64 recurrence with
+// every iterate x_1 .. x_iters written to its own buffer (x_0 = X is kept).
+template <int G>
+void solve_taped_impl(const DeviceMesh& m, double w, double rho, int iters, double* const* outs, hipStream_t s) {
+  const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
+  const FirstTermArgs ft{m.r1Off, m.r1Col, m.r1Val, m.sysDiag, m.sysRhs, w};
+  const FirstTermArgs none{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0};
+  double r = rho * rho;
+  double W = 2.0 / (2.0 - r);
+  hipLaunchKernelGGL((k_sweep64<G, 0>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, 1.0, m.sliceOff,
+                     m.sliceW, m.m2Col, m.m2Val, m.X, m.X, m.firstTerm, outs[1], nullptr, nullptr, nullptr, ft);
+  for (int it = 1; it < iters; it++) {
+    const double* pre = it == 1 ? m.X : outs[it - 1];
+    hipLaunchKernelGGL((k_sweep64<G, 1>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff,
+                       m.sliceW, m.m2Col, m.m2Val, outs[it], pre, m.firstTerm, outs[it + 1], nullptr, nullptr, nullptr,
+                       none);
+    W = 4.0 / (4.0 - r * W);
+  }
+  launch_copy(m.X, outs[iters], 3 * (int64_t)m.ld, s);
+}
+
+// Reverse sweep of the Chebyshev recurrence in the scaled adjoint u = D^{-1} xbar
+// (M^T = D M D^{-1} because R1 is symmetric, so M^T xbar = D M u):
+//   u_out = a M u_in + b u_oth,   Gbar_e += a u_in[row] . x_k[col],   uc[row] += a u_in[row]
+// Gbar accumulates D^{-1}-scaled adjoints of the M^2 entries, uc the scaled
+// adjoint of the first term c.
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_adj_sweep(int n, int nSlices, int ld, double a, double b,
+                                                      const int64_t* __restrict__ sliceOff,
+                                                      const int* __restrict__ sliceW, const int* __restrict__ col,
+                                                      const double* __restrict__ val, const double* __restrict__ uin,
+                                                      const double* __restrict__ uoth, const double* __restrict__ xk,
+                                                      double* __restrict__ uout, double* __restrict__ Gbar,
+                                                      double* __restrict__ uc) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int slice = gt / kWave, lane = gt % kWave;
+  if (slice >= nSlices) return;
+  constexpr int S = kWave / G;
+  int row = slice * S + lane / G;
+  int sub = lane % G;
+  int64_t off = sliceOff[slice] + lane;
+  int width = sliceW[slice];
+  double ui[3] = {0.0, 0.0, 0.0};
+  if (row < n) { ui[0] = uin[row]; ui[1] = uin[ld + row]; ui[2] = uin[2 * ld + row]; }
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll 4
+  for (int c = 0; c < width; c++) {
+    int64_t e = off + (int64_t)c * kWave;
+    int j = col[e];
+    double v = val[e];
+    double u0 = uin[j], u1 = uin[ld + j], u2 = uin[2 * ld + j];
+    acc[0] += v * u0;
+    acc[1] += v * u1;
+    acc[2] += v * u2;
+    Gbar[e] += a * (ui[0] * xk[j] + ui[1] * xk[ld + j] + ui[2] * xk[2 * ld + j]);
+  }
+  group_reduce<G>(acc);
+  if (sub != 0 || row >= n) return;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    int idx = k * ld + row;
+    uout[idx] = a * acc[k] + (b != 0.0 ? b * uoth[idx] : 0.0);
+    uc[idx] += a * ui[k];
+  }
+}
+
+template <int G>
+void adj_sweep_impl(const DeviceMesh& m, double a, double b, const double* uin, const double* uoth, const double* xk,
+                    double* uout, double* Gbar, double* uc, hipStream_t s) {
+  const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL((k_adj_sweep<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, a, b, m.sliceOff,
+                     m.sliceW, m.m2Col, m.m2Val, uin, uoth, xk, uout, Gbar, uc);
+}
+
+// Adjoint of the first term c_i = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
+// given its scaled adjoint uc = D^{-1} cbar:  bbar = w^2 D^{-1} R1 uc + (2 - w) w uc,
+// and the D and R1 parts of the derivative accumulated into Dbar and R1bar.
+__global__ void k_adj_first(int n, int ld, double w, const int* __restrict__ off, const int* __restrict__ col,
+                            const double* __restrict__ r1, const double* __restrict__ D, const double* __restrict__ b,
+                            const double* __restrict__ uc, double* __restrict__ bbar, double* __restrict__ Dbar,
+                            double* __restrict__ R1bar) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double ui[3] = {uc[i], uc[ld + i], uc[2 * ld + i]};
+  double s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
+  const double w2 = w * w;
+  for (int q = off[i]; q < off[i + 1]; q++) {
+    int j = col[q];
+    double a = r1[q], dj = D[j];
+    double bj[3] = {b[j] / dj, b[ld + j] / dj, b[2 * ld + j] / dj};
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      s1[k] += a * uc[k * ld + j];
+      s2[k] += a * bj[k];
+    }
+    R1bar[q] += w2 * (ui[0] * bj[0] + ui[1] * bj[1] + ui[2] * bj[2]);
+  }
+  double di = D[i];
+  double db = 0.0;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double bi = b[k * ld + i];
+    double c = (w2 * s2[k] + (2.0 - w) * w * bi) / di;
+    bbar[k * ld + i] = w2 * s1[k] / di + (2.0 - w) * w * ui[k];
+    db -= ui[k] * c + w2 * bi * s1[k] / (di * di);
+  }
+  Dbar[i] += db;
+}
+
+// Adjoint of the operator update: from Gbar (M^2-entry adjoints scaled by D_row^{-1})
+// to the diagonal D and the R1 values (through the 2-path products and R1_ij).
+__global__ void k_adj_operators(int64_t nE, double w, const int* __restrict__ rowFlag, const double* __restrict__ r1e,
+                                const int* __restrict__ r1Idx, const int64_t* __restrict__ pOff,
+                                const int* __restrict__ pK, const double* __restrict__ pA, const int* __restrict__ pIa,
+                                const int* __restrict__ pIb, const double* __restrict__ r1Val,
+                                const double* __restrict__ D, const double* __restrict__ Gbar,
+                                double* __restrict__ Dbar, double* __restrict__ R1bar) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nE) return;
+  double g = Gbar[e];
+  if (g == 0.0) return;
+  int row = rowFlag[e] & 0x7fffffff;
+  const double w2 = w * w;
+  double s = 0.0;
+  for (int64_t p = pOff[e]; p < pOff[e + 1]; p++) {
+    int k = pK[p];
+    double dk = D[k];
+    s += pA[p] / dk;
+    atomicAdd(Dbar + k, -g * w2 * pA[p] / (dk * dk));
+    double abar = g * w2 / dk;
+    atomicAdd(R1bar + pIa[p], abar * r1Val[pIb[p]]);
+    atomicAdd(R1bar + pIb[p], abar * r1Val[pIa[p]]);
+  }
+  atomicAdd(Dbar + row, -g * (w2 * s + 2.0 * (1.0 - w) * w * r1e[e]) / D[row]);
+  int q = r1Idx[e];
+  if (q >= 0) atomicAdd(R1bar + q, g * 2.0 * (1.0 - w) * w);
+}
+
 template <int G>
 void one_sweep_impl(const DeviceMesh& m, int precision, hipStream_t s) {
   const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
@@ -262,6 +400,39 @@ void launch_operators(const DeviceMesh& m, double omega, int precision, hipStrea
                      m.dChanged, m.dDirty);
   hipLaunchKernelGGL(k_operators, dim3((unsigned)((nE + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nE, omega,
                      m.m2RowFlag, m.dDirty, m.m2R1, m.p2Off, m.p2K, m.p2A, m.sysDiag, m.m2Val, m.m2ValF);
+}
+
+void launch_solve_taped(const DeviceMesh& m, double omega, double rho, int iters, double* const* outs, hipStream_t s) {
+  switch (m.sellLanes) {
+    case 1: solve_taped_impl<1>(m, omega, rho, iters, outs, s); break;
+    case 2: solve_taped_impl<2>(m, omega, rho, iters, outs, s); break;
+    case 8: solve_taped_impl<8>(m, omega, rho, iters, outs, s); break;
+    default: solve_taped_impl<4>(m, omega, rho, iters, outs, s); break;
+  }
+}
+
+void launch_adj_sweep(const DeviceMesh& m, double a, double b, const double* uin, const double* uoth, const double* xk,
+                      double* uout, double* Gbar, double* uc, hipStream_t s) {
+  switch (m.sellLanes) {
+    case 1: adj_sweep_impl<1>(m, a, b, uin, uoth, xk, uout, Gbar, uc, s); break;
+    case 2: adj_sweep_impl<2>(m, a, b, uin, uoth, xk, uout, Gbar, uc, s); break;
+    case 8: adj_sweep_impl<8>(m, a, b, uin, uoth, xk, uout, Gbar, uc, s); break;
+    default: adj_sweep_impl<4>(m, a, b, uin, uoth, xk, uout, Gbar, uc, s); break;
+  }
+}
+
+void launch_adj_first(const DeviceMesh& m, double omega, const double* b, const double* uc, double* bbar, double* Dbar,
+                      double* R1bar, hipStream_t s) {
+  hipLaunchKernelGGL(k_adj_first, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, omega,
+                     m.r1Off, m.r1Col, m.r1Val, m.sysDiag, b, uc, bbar, Dbar, R1bar);
+}
+
+void launch_adj_operators(const DeviceMesh& m, const AdjMesh& a, double omega, const double* Gbar, double* Dbar,
+                          double* R1bar, hipStream_t s) {
+  int64_t nE = m.sellEntries;
+  hipLaunchKernelGGL(k_adj_operators, dim3((unsigned)((nE + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nE, omega,
+                     m.m2RowFlag, m.m2R1, a.sellR1Idx, m.p2Off, m.p2K, m.p2A, a.pIa, a.pIb, m.r1Val, m.sysDiag, Gbar,
+                     Dbar, R1bar);
 }
 
 void launch_solve(const DeviceMesh& m, double omega, double rho, int iters, int precision, hipStream_t s) {

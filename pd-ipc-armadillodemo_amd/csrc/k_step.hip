@@ -158,6 +158,108 @@ __global__ void k_copy(double* __restrict__ d, const double* __restrict__ s, int
   if (i < n) d[i] = s[i];
 }
 
+// ---- adjoint helpers (simulated rows of [3][ld] vectors)
+__global__ void k_axpby(int n, int ld, double a, const double* __restrict__ x, double b, const double* __restrict__ y,
+                        double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    double v = a * x[c * ld + i];
+    if (y) v += b * y[c * ld + i];
+    out[c * ld + i] = v;
+  }
+}
+void launch_axpby(const DeviceMesh& m, double a, const double* x, double b, const double* y, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_axpby, dim3(grid_for(m.nSim)), dim3(kBlock), 0, s, m.nSim, m.ld, a, x, b, y, out);
+}
+
+__global__ void k_div_rows(int n, int ld, const double* __restrict__ x, const double* __restrict__ D,
+                           double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = D[i];
+#pragma unroll
+  for (int c = 0; c < 3; c++) out[c * ld + i] = x[c * ld + i] / d;
+}
+void launch_div_rows(const DeviceMesh& m, const double* x, const double* D, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_div_rows, dim3(grid_for(m.nSim)), dim3(kBlock), 0, s, m.nSim, m.ld, x, D, out);
+}
+
+__global__ void k_mul_rows(int n, int ld, const double* __restrict__ x, const double* __restrict__ D,
+                           double* __restrict__ out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  double d = D[i];
+#pragma unroll
+  for (int c = 0; c < 3; c++) out[c * ld + i] = x[c * ld + i] * d;
+}
+void launch_mul_rows(const DeviceMesh& m, const double* x, const double* D, double* out, hipStream_t s) {
+  hipLaunchKernelGGL(k_mul_rows, dim3(grid_for(m.nSim)), dim3(kBlock), 0, s, m.nSim, m.ld, x, D, out);
+}
+
+// scale * sum_i a_i . (x1_i - x0_i): the adjoint of the TOI (deterministic two-pass sum)
+__global__ void k_dot_dir(int n, int ld, const double* __restrict__ a, const double* __restrict__ x1,
+                          const double* __restrict__ x0, double* __restrict__ partials) {
+  __shared__ double red[kBlock / kWave];
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  double acc = 0.0;
+  if (i < n) {
+#pragma unroll
+    for (int c = 0; c < 3; c++) acc += a[c * ld + i] * (x1[c * ld + i] - x0[c * ld + i]);
+  }
+  for (int off = kWave / 2; off > 0; off >>= 1) acc += __shfl_down(acc, off, kWave);
+  int lane = threadIdx.x % kWave, wv = threadIdx.x / kWave;
+  if (lane == 0) red[wv] = acc;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double t = 0.0;
+    for (int w = 0; w < kBlock / kWave; w++) t += red[w];
+    partials[blockIdx.x] = t;
+  }
+}
+__global__ void k_scale_scalar(double* p, double scale) { *p *= scale; }
+void launch_dot_dir(const DeviceMesh& m, const double* a, const double* x1, const double* x0, double scale,
+                    double* partials, double* out, hipStream_t s) {
+  int g = grid_for(m.nSim);
+  hipLaunchKernelGGL(k_dot_dir, dim3(g), dim3(kBlock), 0, s, m.nSim, m.ld, a, x1, x0, partials);
+  hipLaunchKernelGGL(k_sum_partials, dim3(1), dim3(kBlock), 0, s, g, partials, out);
+  hipLaunchKernelGGL(k_scale_scalar, dim3(1), dim3(1), 0, s, out, scale);
+}
+
+// adjoint of the prediction s = x + dt (v + dt f / m), rhs0 = (m / dt^2) s, and of
+// x0 = x:  xbar = axn + sbar, vbar = dt sbar with sbar = as + (m / dt^2) rhs0bar
+// (fixed points keep x, so sbar = as and vbar = 0)
+__global__ void k_predict_adj(int n, int ld, double dt, const int* __restrict__ fixFlag,
+                              const double* __restrict__ inertialDiag, const double* __restrict__ as,
+                              const double* __restrict__ rhs0bar, const double* __restrict__ axn,
+                              double* __restrict__ xbar, double* __restrict__ vbar) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  bool fx = fixFlag[i] != 0;
+  double w = inertialDiag[i];
+#pragma unroll
+  for (int c = 0; c < 3; c++) {
+    int idx = c * ld + i;
+    double sb = fx ? as[idx] : as[idx] + w * rhs0bar[idx];
+    xbar[idx] = axn[idx] + sb;
+    vbar[idx] = fx ? 0.0 : dt * sb;
+  }
+}
+void launch_predict_adj(const DeviceMesh& m, double dt, const double* as, const double* rhs0bar, const double* axn,
+                        double* xbar, double* vbar, hipStream_t s) {
+  hipLaunchKernelGGL(k_predict_adj, dim3(grid_for(m.nSim)), dim3(kBlock), 0, s, m.nSim, m.ld, dt, m.fixFlag,
+                     m.inertialDiag, as, rhs0bar, axn, xbar, vbar);
+}
+
+__global__ void k_add(double* __restrict__ d, const double* __restrict__ s, int64_t n) {
+  int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) d[i] += s[i];
+}
+void launch_add_rows(double* dst, const double* src, int64_t n, hipStream_t s) {
+  if (n > 0) hipLaunchKernelGGL(k_add, dim3(grid_for(n)), dim3(kBlock), 0, s, dst, src, n);
+}
+
 void launch_zero(double* p, int64_t n, hipStream_t s) {
   if (n > 0) hipLaunchKernelGGL(k_zero, dim3(grid_for(n)), dim3(kBlock), 0, s, p, n);
 }

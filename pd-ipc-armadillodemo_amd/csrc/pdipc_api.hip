@@ -13,13 +13,11 @@
 #include <string>
 #include <vector>
 
-#include "../../include/pdipc.h"
-#include "device.hpp"
-#include "host_setup.hpp"
+#include "handle.hpp"
 
 using namespace pdipc;
 
-namespace {
+namespace pdipc {
 
 thread_local std::string g_err;
 
@@ -28,73 +26,9 @@ int fail(int code, const std::string& msg) {
   return code;
 }
 
-#define HIPCHK(expr)                                                                          \
-  do {                                                                                        \
-    hipError_t e_ = (expr);                                                                   \
-    if (e_ != hipSuccess) return fail(PDIPC_E_HIP, std::string(#expr) + ": " + hipGetErrorString(e_)); \
-  } while (0)
+}  // namespace pdipc
 
-}  // namespace
-
-struct pdipc_handle {
-  pdipc_params prm;
-  std::vector<ModelInput> sims, statics;
-  bool hostReady = false, deviceReady = false;
-  HostSetup hs;
-  SellLayout sell;
-  DeviceMesh m;
-  hipStream_t stream = nullptr;
-  // local/global loop captured once per (collision flag, precision) and replayed
-  hipGraphExec_t lgGraph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
-  std::vector<void*> allocs;
-  // host mirrors
-  void* scalHost = nullptr;  // pinned: residual, toiBits, counters[8]
-  void* scalDev = nullptr;
-  std::vector<double> stage;
-  pdipc_counters cnt;
-  int lastRecs = 0, lastPairs = 0;
-
-  template <class T>
-  int alloc(T** p, size_t n) {
-    size_t bytes = std::max<size_t>(n, 1) * sizeof(T);
-    hipError_t e = hipMalloc((void**)p, bytes);
-    if (e != hipSuccess) return fail(PDIPC_E_HIP, std::string("hipMalloc: ") + hipGetErrorString(e));
-    allocs.push_back(*p);
-    return PDIPC_OK;
-  }
-  template <class T>
-  int upload(T** p, const std::vector<T>& v) {
-    int rc = alloc(p, v.size());
-    if (rc) return rc;
-    if (!v.empty()) HIPCHK(hipMemcpy(*p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice));
-    return PDIPC_OK;
-  }
-  void release(void* p) {
-    if (!p) return;
-    auto it = std::find(allocs.begin(), allocs.end(), p);
-    if (it != allocs.end()) allocs.erase(it);
-    (void)hipFree(p);
-  }
-  ~pdipc_handle() {
-    for (void* p : allocs) (void)hipFree(p);
-    if (scalHost) (void)hipHostFree(scalHost);
-    drop_graphs();
-    if (stream) (void)hipStreamDestroy(stream);
-  }
-  void drop_graphs() {
-    for (auto& row : lgGraph)
-      for (auto& g : row)
-        if (g) {
-          (void)hipGraphExecDestroy(g);
-          g = nullptr;
-        }
-  }
-  double* hResidual() { return (double*)scalHost; }
-  unsigned long long* hToi() { return (unsigned long long*)((char*)scalHost + 8); }
-  int* hCounters() { return (int*)((char*)scalHost + 16); }
-};
-
-namespace {
+namespace pdipc {
 
 // AoS xyz (npoints) -> SoA [3][ld] device
 int put_soa(pdipc_handle* h, double* dst, const double* aos, int npts) {
@@ -223,6 +157,11 @@ int upload_all(pdipc_handle* h) {
   AL(recs, m.recCap);
   m.scanBlockCap = (std::max(P, m.pairCap) + 1023) / 1024 + 2;
   AL(scanBlock, m.scanBlockCap);
+  m.actCap = 4096;
+  AL(actRec, m.actCap); AL(actW, m.actCap); AL(actT, 12 * (size_t)m.actCap);
+  AL(entKey, 4 * (size_t)m.actCap); AL(entA, 4 * (size_t)m.actCap);
+  AL(ptCount, hs.nSim + 1); AL(ptOff, hs.nSim + 1); AL(ptCursor, hs.nSim + 1);
+  AL(ptScan, (hs.nSim + 1 + 1023) / 1024 + 2);
   m.nPartials = (std::max(1, hs.nSim) + 255) / 256;
   AL(partials, m.nPartials);
   // scalars: residual (8), toiBits (8), counters (8 ints)
@@ -303,6 +242,31 @@ int broad_phase(pdipc_handle* h, const double* x, const double* dx) {
   return PDIPC_OK;
 }
 
+// collision terms at m.X into colDiag / colRhs (simulated rows), growing the
+// active-record list when a pass overflows it
+int collision(pdipc_handle* h) {
+  DeviceMesh& m = h->m;
+  const pdipc_params& p = h->prm;
+  hipStream_t s = h->stream;
+  int rc;
+  for (int attempt = 0; attempt < 2; attempt++) {
+    HIPCHK(hipMemsetAsync(m.counters + 2, 0, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(m.counters + 5, 0, sizeof(int), s));
+    launch_collision(m, h->lastRecs, p.dhat, p.kappa, p.dt, s);
+    if ((rc = check_launch(h))) return rc;
+    if ((rc = read_scalars(h))) return rc;
+    if (!h->hCounters()[5]) return PDIPC_OK;
+    int A = h->hCounters()[2];
+    int cap = A + A / 2 + 4096;
+    for (void* q : {(void*)m.actRec, (void*)m.actW, (void*)m.actT, (void*)m.entKey, (void*)m.entA}) h->release(q);
+    m.actCap = cap;
+    if ((rc = h->alloc(&m.actRec, cap)) || (rc = h->alloc(&m.actW, cap)) || (rc = h->alloc(&m.actT, 12 * (size_t)cap)) ||
+        (rc = h->alloc(&m.entKey, 4 * (size_t)cap)) || (rc = h->alloc(&m.entA, 4 * (size_t)cap)))
+      return rc;
+  }
+  return fail(PDIPC_E_CAPACITY, "active collision list did not fit after growing");
+}
+
 void enqueue_local_global(pdipc_handle* h, int withCol) {
   DeviceMesh& m = h->m;
   const pdipc_params& p = h->prm;
@@ -363,13 +327,8 @@ int advance(pdipc_handle* h) {
     launch_toi_init(m, s);
     launch_ccd(m, h->lastRecs, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap, p.touch_policy, s);
     launch_clamp_residual(m, p.toi_scale, s);
-    launch_zero(m.colDiag, m.nTotal, s);
-    launch_zero(m.colRhs, (int64_t)V3, s);
-    HIPCHK(hipMemsetAsync(m.counters + 2, 0, sizeof(int), s));
-    launch_collision(m, h->lastRecs, p.dhat, p.kappa, p.dt, s);
+    if ((rc = collision(h))) return rc;  // also reads the scalars back
     activeCol = true;
-    if ((rc = check_launch(h))) return rc;
-    if ((rc = read_scalars(h))) return rc;
     double res = *h->hResidual();
     double tmin;
     unsigned long long b = *h->hToi();
@@ -407,7 +366,7 @@ ModelInput make_input(int kind, const double* v, int nv, const int* el, int nel,
   return m;
 }
 
-}  // namespace
+}  // namespace pdipc
 
 extern "C" {
 
@@ -422,7 +381,7 @@ void pdipc_default_params(pdipc_params* o) {
   o->precision = PDIPC_FP64; o->sell_lanes = 4; o->use_graphs = 0;
 }
 
-const char* pdipc_last_error(void) { return g_err.c_str(); }
+const char* pdipc_last_error(void) { return pdipc::g_err.c_str(); }
 
 int pdipc_create(const pdipc_params* params, pdipc_handle** out) {
   if (!out) return fail(PDIPC_E_ARG, "null output pointer");
@@ -465,6 +424,43 @@ int pdipc_add_static(pdipc_handle* h, const double* verts, int nverts, const int
   if (!h || !verts || !tris || nverts <= 0 || ntris <= 0) return fail(PDIPC_E_ARG, "bad static arguments");
   if (h->hostReady) return fail(PDIPC_E_ARG, "models must be added before setup");
   h->statics.push_back(make_input(MODEL_STATIC, verts, nverts, tris, ntris, 3, 0, 0, 0, 1e5, patch_dim, nullptr, 0));
+  return PDIPC_OK;
+}
+
+int pdipc_set_cloth_stiffness(pdipc_handle* h, int model, const double* k, int ntris) {
+  if (!h || !k) return fail(PDIPC_E_ARG, "null argument");
+  if (model < 0 || model >= (int)h->sims.size() || h->sims[model].kind != MODEL_CLOTH)
+    return fail(PDIPC_E_ARG, "model is not a simulated cloth");
+  if (ntris * 3 != (int)h->sims[model].elems.size()) return fail(PDIPC_E_ARG, "one stiffness per cloth triangle");
+  for (int t = 0; t < ntris; t++)
+    if (!(k[t] > 0) || !std::isfinite(k[t])) return fail(PDIPC_E_ARG, "stiffness must be positive and finite");
+  std::vector<double> old = h->sims[model].triK;
+  h->sims[model].triK.assign(k, k + ntris);
+  if (!h->hostReady) return PDIPC_OK;
+  // recompute the values; the patterns must not change (they size the device arrays)
+  StepConstants c;
+  c.dt = h->prm.dt;
+  c.gravity = h->prm.gravity;
+  c.dhat = h->prm.dhat;
+  HostSetup ns;
+  std::string err = build_host_setup(h->sims, h->statics, c, ns);
+  if (!err.empty() || ns.r1Col != h->hs.r1Col || ns.m2Col != h->hs.m2Col || ns.p2K != h->hs.p2K ||
+      ns.cNbr != h->hs.cNbr) {
+    h->sims[model].triK = old;
+    return fail(PDIPC_E_ARG, err.empty() ? "stiffness change alters the operator pattern" : err);
+  }
+  for (int i = ns.nSim; i < ns.nTotal; i++) ns.colMass[i] = h->prm.static_mass;
+  h->hs = std::move(ns);
+  if (!h->deviceReady) return PDIPC_OK;
+  build_sell(h->hs, h->sell.lanes, h->sell);
+  DeviceMesh& m = h->m;
+  const HostSetup& hs = h->hs;
+  HIPCHK(hipMemcpy(m.elasticDiag, hs.elasticDiag.data(), hs.elasticDiag.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m.r1Val, hs.r1Val.data(), hs.r1Val.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m.cStrain, hs.cStrain.data(), hs.cStrain.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m.m2R1, h->sell.r1.data(), h->sell.r1.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemcpy(m.p2A, h->sell.pA.data(), h->sell.pA.size() * sizeof(double), hipMemcpyHostToDevice));
+  HIPCHK(hipMemset(m.dStale, 1, std::max(1, hs.nSim)));  // every M^2 row is recomputed next
   return PDIPC_OK;
 }
 
@@ -660,15 +656,10 @@ int pdipc_debug_collision(pdipc_handle* h, const double* x, double* diag, double
   if (rc) return rc;
   DeviceMesh& m = h->m;
   if ((rc = put_soa(h, m.X, x, m.nTotal))) return rc;
-  launch_zero(m.colDiag, m.nTotal, h->stream);
-  launch_zero(m.colRhs, 3 * (int64_t)m.ld, h->stream);
-  HIPCHK(hipMemsetAsync(m.counters + 2, 0, sizeof(int), h->stream));
-  launch_collision(m, h->lastRecs, h->prm.dhat, h->prm.kappa, h->prm.dt, h->stream);
-  if ((rc = check_launch(h))) return rc;
+  if ((rc = collision(h))) return rc;
+  if (active) *active = h->hCounters()[2];
   HIPCHK(hipMemcpyAsync(diag, m.colDiag, m.nSim * sizeof(double), hipMemcpyDeviceToHost, h->stream));
   if ((rc = get_soa(h, m.colRhs, rhs, m.nSim))) return rc;
-  if ((rc = read_scalars(h))) return rc;
-  if (active) *active = h->hCounters()[2];
   return PDIPC_OK;
 }
 

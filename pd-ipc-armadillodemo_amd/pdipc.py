@@ -78,6 +78,12 @@ SIGNATURES = {
     "pdipc_debug_solve": (ctypes.c_int, [_H, _D, _D, _D, _D]),
     "pdipc_debug_lbvh": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_ulonglong), _I, _I, _I, _I]),
     "pdipc_time_kernel": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _D, _D]),
+    "pdipc_set_cloth_stiffness": (ctypes.c_int, [_H, ctypes.c_int, _D, ctypes.c_int]),
+    "pdipc_rollout": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int]),
+    "pdipc_backward": (ctypes.c_int, [_H, _D, _D, _D, _D, _D, ctypes.c_int]),
+    "pdipc_adjoint_stats": (ctypes.c_int, [_H, _D]),
+    "pdipc_debug_solve_vjp": (ctypes.c_int, [_H, _D, _D, _D, _D, _D, _D]),
+    "pdipc_debug_local_vjp": (ctypes.c_int, [_H, _D, _D, _D]),
 }
 
 KERNEL_IDS = {"sweep_fp64": 0, "sweep_fp32": 1, "local": 2, "ccd": 3, "operators": 4}
@@ -149,8 +155,10 @@ class Simulator:
     @classmethod
     def from_scene(cls, scene, init=True, **extra):
         s = cls(params_from_scene(scene, **extra))
-        for c in scene["cloths"]:
+        for mi, c in enumerate(scene["cloths"]):
             s.add_cloth(c["v"], c["f"], c["s0"], c["s1"], c["s2"], c["density"], c["patch_dim"], c["fixed"])
+            if c.get("tri_k") is not None:
+                s.set_cloth_stiffness(mi, c["tri_k"])
         for t in scene["tets"]:
             s.add_tet(t["v"], t["t"], t["s0"], t["s1"], t["s2"], t["density"], t["patch_dim"], t["fixed"])
         for st in scene["statics"]:
@@ -306,6 +314,51 @@ class Simulator:
         us, b = ctypes.c_double(), ctypes.c_double()
         _check(lib().pdipc_time_kernel(self.h, KERNEL_IDS[name], int(reps), ctypes.byref(us), ctypes.byref(b)))
         return us.value, b.value
+
+    # --- per-triangle stiffness and the reverse-mode derivative
+    def set_cloth_stiffness(self, model, k):
+        k = np.ascontiguousarray(k, np.float64)
+        self._keep.append(k)
+        _check(lib().pdipc_set_cloth_stiffness(self.h, int(model), _dp(k), len(k)))
+
+    def rollout(self, nsteps, checkpoint_every=10):
+        _check(lib().pdipc_rollout(self.h, int(nsteps), int(checkpoint_every)))
+
+    def backward(self, dx=None, dv=None, n_tris=None):
+        """dL/dx, dL/dv of the rolled-out final state -> (dL/dx0, dL/dv0, dL/dk)
+        for the initial state and every cloth triangle's stiffness."""
+        keep = [np.ascontiguousarray(np.asarray(a, np.float64).reshape(-1, 3)[: self.sim]) for a in (dx, dv)
+                if a is not None]
+        sx = _dp(keep[0]) if dx is not None else None
+        sv = _dp(keep[-1]) if dv is not None else None
+        dx0 = np.zeros((self.sim, 3))
+        dv0 = np.zeros((self.sim, 3))
+        nk = int(n_tris) if n_tris is not None else 0
+        dk = np.zeros(max(nk, 1))
+        _check(lib().pdipc_backward(self.h, sx, sv, _dp(dx0), _dp(dv0), _dp(dk) if nk else None, nk))
+        return dx0, dv0, dk[:nk]
+
+    def debug_solve_vjp(self, diag, rhs, x0, xbar_out):
+        d = np.ascontiguousarray(diag, np.float64)
+        r = np.ascontiguousarray(np.asarray(rhs, np.float64).reshape(-1))
+        a = np.ascontiguousarray(np.asarray(x0, np.float64).reshape(-1))
+        g = np.ascontiguousarray(np.asarray(xbar_out, np.float64).reshape(-1))
+        xb = np.zeros((self.sim, 3))
+        bb = np.zeros((self.sim, 3))
+        _check(lib().pdipc_debug_solve_vjp(self.h, _dp(d), _dp(r), _dp(a), _dp(g), _dp(xb), _dp(bb)))
+        return xb, bb
+
+    def debug_local_vjp(self, x, bbar):
+        a = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
+        g = np.ascontiguousarray(np.asarray(bbar, np.float64).reshape(-1))
+        xb = np.zeros((self.sim, 3))
+        _check(lib().pdipc_debug_local_vjp(self.h, _dp(a), _dp(g), _dp(xb)))
+        return xb
+
+    def adjoint_stats(self):
+        out = np.zeros(4)
+        _check(lib().pdipc_adjoint_stats(self.h, _dp(out)))
+        return dict(outer_iters=int(out[0]), tape_bytes=float(out[1]), segments=int(out[2]), steps=int(out[3]))
 
     def debug_lbvh(self):
         n = ctypes.c_int()
