@@ -187,6 +187,11 @@ int pdipc_adjoint_stats(pdipc_handle* h, double* out4);
 int pdipc_debug_solve_vjp(pdipc_handle* h, const double* diag, const double* rhs, const double* x0,
                           const double* xbar_out, double* xbar0, double* bbar);
 int pdipc_debug_local_vjp(pdipc_handle* h, const double* x, const double* bbar, double* xbar);
+/* a vector recorded by the last backward pass (its last segment): step within
+ * the segment, outer iteration, which = 0 xold, 1 x at the local/global start,
+ * 2 system diagonal (nsim), 3 rhs of pass l, 4 Chebyshev iterate k of pass l,
+ * 5 the scalars (toi, clamped, argmin record, collision terms active) */
+int pdipc_debug_tape(pdipc_handle* h, int step, int outer, int which, int l, int k, double* out);
 
 #ifdef __cplusplus
 }

@@ -503,6 +503,38 @@ int pdipc_debug_local_vjp(pdipc_handle* h, const double* x, const double* bbar, 
   return get_soa(h, A->asol, xbar, m.nSim);
 }
 
+int pdipc_debug_tape(pdipc_handle* h, int step, int outer, int which, int l, int k, double* out) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  AdjointState* A = h->adj;
+  if (!A || step < 0 || step >= (int)A->seg.size()) return fail(PDIPC_E_ARG, "no such recorded step");
+  StepTape& st = A->seg[step];
+  if (outer < 0 || outer >= (int)st.outers.size()) return fail(PDIPC_E_ARG, "no such outer iteration");
+  OuterTape& ot = st.outers[outer];
+  if (which == 5) {  // scalars: toi, clamped, argmin, collision terms active
+    out[0] = ot.toi;
+    out[1] = ot.clamped;
+    out[2] = ot.argmin;
+    out[3] = ot.withCol;
+    return PDIPC_OK;
+  }
+  const double* src = nullptr;
+  switch (which) {
+    case 0: src = ot.xold; break;
+    case 1: src = ot.xstart; break;
+    case 2: src = ot.D; break;
+    case 3: src = (l >= 0 && l < (int)ot.b.size()) ? ot.b[l] : nullptr; break;
+    default:
+      src = (l >= 0 && l < (int)ot.it.size() && k >= 0 && k < (int)ot.it[l].size()) ? ot.it[l][k] : nullptr;
+  }
+  if (!src) return fail(PDIPC_E_ARG, "no such recorded vector");
+  if (which == 2) {
+    HIPCHK(hipMemcpy(out, src, h->m.nSim * sizeof(double), hipMemcpyDeviceToHost));
+    return PDIPC_OK;
+  }
+  return get_soa(h, src, out, h->m.nSim);
+}
+
 int pdipc_adjoint_stats(pdipc_handle* h, double* out4) {
   if (!h || !out4) return fail(PDIPC_E_ARG, "null argument");
   for (int i = 0; i < 4; i++) out4[i] = h->adj ? h->adj->stats[i] : 0.0;

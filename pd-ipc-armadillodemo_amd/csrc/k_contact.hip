@@ -561,8 +561,10 @@ __global__ void k_adj_ccd(int ld, int nSim, const int4* __restrict__ recs, int a
   for (int q = 0; q < 4; q++)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
+      // X1 holds the simulated rows only: static points end where they start
+      const double* src1 = id[q] < nSim ? X1 : X0;
       x[q].v[k] = D24::seed(X0[k * ld + id[q]], 3 * q + k);
-      D24 y = D24::seed(X1[k * ld + id[q]], 12 + 3 * q + k);
+      D24 y = D24::seed(src1[k * ld + id[q]], 12 + 3 * q + k);
       dx[q].v[k] = y - x[q].v[k];
     }
   D24 toc = accd(type, x, dx, iters, step, gapf, touchPolicy);

@@ -249,7 +249,8 @@ void solve_taped_impl(const DeviceMesh& m, double w, double rho, int iters, doub
                        none);
     W = 4.0 / (4.0 - r * W);
   }
-  launch_copy(m.X, outs[iters], 3 * (int64_t)m.ld, s);
+  // the sweeps write the simulated rows only: copy those back (the static rows of X stay)
+  launch_axpby(m, 1.0, outs[iters], 0.0, nullptr, m.X, s);
 }
 
 // Reverse sweep of the Chebyshev recurrence in the scaled adjoint u = D^{-1} xbar

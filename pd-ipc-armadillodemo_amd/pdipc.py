@@ -84,6 +84,7 @@ SIGNATURES = {
     "pdipc_adjoint_stats": (ctypes.c_int, [_H, _D]),
     "pdipc_debug_solve_vjp": (ctypes.c_int, [_H, _D, _D, _D, _D, _D, _D]),
     "pdipc_debug_local_vjp": (ctypes.c_int, [_H, _D, _D, _D]),
+    "pdipc_debug_tape": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, ctypes.c_int, _D]),
 }
 
 KERNEL_IDS = {"sweep_fp64": 0, "sweep_fp32": 1, "local": 2, "ccd": 3, "operators": 4}
@@ -354,6 +355,11 @@ class Simulator:
         xb = np.zeros((self.sim, 3))
         _check(lib().pdipc_debug_local_vjp(self.h, _dp(a), _dp(g), _dp(xb)))
         return xb
+
+    def debug_tape(self, step, outer, which, l=0, k=0):
+        out = np.zeros(self.sim if which == 2 else (4 if which == 5 else 3 * self.sim))
+        _check(lib().pdipc_debug_tape(self.h, step, outer, which, l, k, _dp(out)))
+        return out if which in (2, 5) else out.reshape(-1, 3)
 
     def adjoint_stats(self):
         out = np.zeros(4)
