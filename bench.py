@@ -29,6 +29,10 @@ import scenes  # noqa: E402
 
 METRIC = "PD-IPC steps/s, 256² cloth: FP32-split vs FP64 at 1e-5 err; adjoint/fwd ratio"
 HBM_PEAK_GBS = 8000.0
+# Bytes one FP32 sweep launch moves past L2 at the default workload, from two
+# rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
+# a plain 1 GiB copy gives): profiles/k_sweep_config1_hbm_traffic.md.
+SWEEP32_TRAFFIC_BYTES = {(256, 6): 38.69e6}
 
 
 def log(msg):
@@ -65,6 +69,69 @@ def cpu_baseline(scene, seconds_budget=20.0):
                        "single thread")
 
 
+def adjoint_bench(graphs, fd_dirs, log_every=8):
+    """The differentiable run: 128 x 128 cloth over the config-0 obstacle, 50 FP64
+    steps, loss |x_T - target|^2, gradient w.r.t. per-triangle stiffness and
+    initial velocity by the library's adjoint with checkpoints every 10 steps;
+    timed forward (rollout) and backward, then central differences (eps 1e-6)
+    on `fd_dirs` random directions."""
+    steps, every = 50, 10
+    sc, v0, target = scenes.config3(0, n_cloth=128, steps=steps)
+    sim = pdipc.Simulator.from_scene(sc, use_graphs=graphs)
+    x0 = sim.x()[: sim.sim].copy()
+    k0 = sc["cloths"][0]["tri_k"].copy()
+    ntri = len(k0)
+
+    def forward(k, v):
+        sim.set_cloth_stiffness(0, k)
+        sim.set_x(x0)
+        sim.set_v(v)
+        t0 = time.perf_counter()
+        sim.rollout(steps, every)
+        t = time.perf_counter() - t0
+        x = sim.x()[: sim.sim]
+        return float(np.sum((x - target) ** 2)), x, t
+
+    forward(k0, v0)  # warm: allocations, graphs
+    _, xT, _ = forward(k0, v0)
+    sim.backward(dx=2.0 * (xT - target), n_tris=ntri)
+    fw = []
+    bw = []
+    for _ in range(2):
+        loss, xT, tf = forward(k0, v0)
+        t0 = time.perf_counter()
+        _, dv0, dk = sim.backward(dx=2.0 * (xT - target), n_tris=ntri)
+        bw.append(time.perf_counter() - t0)
+        fw.append(tf)
+    stats = sim.adjoint_stats()
+    tf, tb = min(fw), min(bw)
+    log(f"adjoint: forward {tf:.3f} s, backward {tb:.3f} s (ratio {tb / tf:.2f}); {stats}")
+    rng = np.random.default_rng(12345)
+    errs = []
+    eps = 1e-6
+    for q in range(fd_dirs):
+        dkq = rng.normal(size=ntri) * k0
+        dvq = rng.normal(size=v0.shape) * 0.05
+        lp, _, _ = forward(k0 + eps * dkq, v0 + eps * dvq)
+        lm, _, _ = forward(k0 - eps * dkq, v0 - eps * dvq)
+        fd = (lp - lm) / (2 * eps)
+        ad = float(dk @ dkq + np.sum(dv0 * dvq))
+        errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
+        if q % log_every == 0:
+            log(f"  direction {q}: fd {fd:.6e} adjoint {ad:.6e} rel {errs[-1]:.2e}")
+    errs = np.array(errs)
+    return {
+        "workload": "cloth 128x128 (32,258 triangles) over icosphere subdiv 4 (config-0 obstacle), 50 FP64 steps, "
+                    "checkpoints every 10, loss |x_T - target|^2, d/d(per-triangle stiffness, v0)",
+        "forward_s": tf, "backward_s": tb, "adjoint_fwd_ratio": tb / tf,
+        "recorded_outer_iters": stats["outer_iters"], "peak_tape_bytes": stats["tape_bytes"],
+        "fd_directions": int(fd_dirs), "fd_eps": eps,
+        "fd_rel_err_median": float(np.median(errs)) if fd_dirs else None,
+        "fd_rel_err_max": float(errs.max()) if fd_dirs else None,
+        "fd_dirs_within_1e-3": int(np.sum(errs < 1e-3)) if fd_dirs else None,
+    }
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -78,11 +145,29 @@ def main():
     ap.add_argument("--graphs", type=int, default=1, help="replay the local/global loop from a HIP graph")
     ap.add_argument("--keep-static-pairs", action="store_true",
                     help="keep static-static patch pairs (the reference's full candidate set)")
+    ap.add_argument("--adjoint-fd", type=int, default=32, help="finite-difference directions of the adjoint check")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain 1 GiB device copy (3 times): the reference for the HBM byte counters")
+    ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--only-adjoint", action="store_true", help="run only the differentiable-run measurement")
     args = ap.parse_args()
+    if args.copy_probe:
+        import torch
+        a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read_per_launch": a.numel() * 4,
+                          "copy_probe_bytes_written_per_launch": b.numel() * 4}))
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs the PD-IPC step on one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)
         sys.exit(2)
+    if args.only_adjoint:
+        print(json.dumps(adjoint_bench(args.graphs, args.adjoint_fd)), flush=True)
+        return
 
     scene = scenes.cloth_over_sphere(args.n, args.subdiv, ground=True, seed=0, patch_dim=args.n,
                                      sphere_patch_dim=args.n)
@@ -135,6 +220,10 @@ def main():
     ach = bytes32 / (us32 * 1e-6) / 1e9
     log(f"sweep fp64 {us64:.2f} us ({bytes64 / (us64 * 1e-6) / 1e9:.0f} GB/s), fp32 {us32:.2f} us ({ach:.0f} GB/s)")
 
+    adj = None
+    if not args.no_adjoint:
+        adj = adjoint_bench(args.graphs, args.adjoint_fd)
+
     cpu = None
     if not args.no_cpu:
         cpu = cpu_baseline(scene, args.cpu_seconds)
@@ -169,6 +258,7 @@ def main():
                                    "skipped (inert: they only write rows the step never reads)",
             "graphs": bool(args.graphs),
             "parallelism": "single GPU",
+            "adjoint": adj,
         },
         "roofline": {
             "kernel": "k_sweep32 (FP32 Chebyshev sweep, SELL M^2)",
@@ -177,7 +267,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": SWEEP32_TRAFFIC_BYTES.get((args.n, args.subdiv)),
             "avg_us": us32,
             "bytes_per_launch": bytes32,
             "fp64_sweep": {"avg_us": us64, "achieved": bytes64 / (us64 * 1e-6) / 1e9},
