@@ -19,6 +19,7 @@
 // iterate (15 x 10 x 3N doubles per outer iteration), then walks it backwards.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <cmath>
 #include <cstring>
@@ -63,12 +64,31 @@ struct Pool {
   }
 };
 
+// The vectors one outer iteration records, laid out in one contiguous stage:
+//   [0] Xold, [1] X when the local/global loop starts, [2, 2+LG) the rhs of
+//   each pass, then the Chebyshev iterates 1..K of each pass, then D (nsim).
+// The recorded forward and the reverse pass both work on a fixed stage (so
+// their loops can be captured once in HIP graphs); one device copy per outer
+// iteration moves the stage to or from the tape.
+struct StageLayout {
+  size_t V3 = 0, nSim = 0;
+  int LG = 0, K = 0;
+  size_t slots() const { return 2 + (size_t)LG + (size_t)LG * K; }
+  size_t bytes() const { return slots() * V3 * sizeof(double) + ((nSim * sizeof(double) + 255) & ~(size_t)255); }
+  double* xold(double* b) const { return b; }
+  double* xstart(double* b) const { return b + V3; }
+  double* rhs(double* b, int l) const { return b + (2 + (size_t)l) * V3; }
+  double* it(double* b, int l, int k) const {  // k-th iterate of pass l; k = 0 is the pass's input
+    if (k == 0) return l == 0 ? xstart(b) : it(b, l - 1, K);
+    return b + (2 + (size_t)LG + (size_t)l * K + (k - 1)) * V3;
+  }
+  double* D(double* b) const { return b + slots() * V3; }
+};
+
 struct OuterTape {
-  double* xold = nullptr;    // Xold of this outer iteration
-  double* xstart = nullptr;  // X when the local/global loop starts
-  double* D = nullptr;       // system diagonal
-  std::vector<double*> b;    // rhs of each local/global pass
-  std::vector<std::vector<double*>> it;  // it[l][k] = k-th Chebyshev iterate of pass l (k = 0: its input)
+  double* blob = nullptr;  // this outer iteration's stage, on the tape
+  int* act = nullptr;      // records active in the collision pass that ends it
+  int nAct = 0;
   double toi = 1.0;
   int clamped = 0;
   int argmin = -1;
@@ -101,11 +121,23 @@ struct AdjointState {
   double* recToi = nullptr;
   int recToiCap = 0;
   double *partials = nullptr, *gscale = nullptr;
+  StageLayout lay;
+  double* stage = nullptr;
+  std::vector<double*> uS;  // reverse Chebyshev iterates u_0 .. u_{K+1} (u_{K+1} stays zero)
+  double* colRacc = nullptr;
+  // local/global loops of the recorded forward and of the reverse pass,
+  // captured once per collision flag
+  hipGraphExec_t fwdGraph[2] = {nullptr, nullptr};
+  hipGraphExec_t revGraph[2] = {nullptr, nullptr};
   double stats[4] = {0, 0, 0, 0};  // recorded outer iterations, tape bytes, segments, steps
 };
 
 void adjoint_free(AdjointState* a) {
   if (!a) return;
+  for (auto& g : a->fwdGraph)
+    if (g) (void)hipGraphExecDestroy(g);
+  for (auto& g : a->revGraph)
+    if (g) (void)hipGraphExecDestroy(g);
   for (void* p : a->allocs) (void)hipFree(p);
   for (double* p : a->ckX) (void)hipFree(p);
   for (double* p : a->ckV) (void)hipFree(p);
@@ -142,6 +174,52 @@ int ensure_state(pdipc_handle* h) {
       (rc = dupload(A, &A->am.pIb, h->sell.pIb)) || (rc = dupload(A, &A->am.cSlotR1, hs.cSlotR1)) ||
       (rc = dupload(A, &A->am.cBendOut, hs.cBendOut)) || (rc = dupload(A, &A->am.cLboIn, hs.cLboIn)))
     return rc;
+  // 2-paths grouped by middle point, and by the spring R1 entries they use
+  {
+    const SellLayout& L = h->sell;
+    const int64_t nE = (int64_t)L.pOff.size() - 1;
+    const int n = hs.nSim;
+    std::vector<int> midOff(n + 1, 0);
+    for (int64_t e = 0; e < nE; e++)
+      for (int64_t q = L.pOff[e]; q < L.pOff[e + 1]; q++) midOff[L.pK[q] + 1]++;
+    for (int k = 0; k < n; k++) midOff[k + 1] += midOff[k];
+    std::vector<int64_t> midEnt(midOff[n]);
+    std::vector<double> midA(midOff[n]);
+    std::vector<int> fill(midOff.begin(), midOff.end() - 1);
+    for (int64_t e = 0; e < nE; e++)
+      for (int64_t q = L.pOff[e]; q < L.pOff[e + 1]; q++) {
+        int at = fill[L.pK[q]]++;
+        midEnt[at] = e;
+        midA[at] = L.pA[q];
+      }
+    std::vector<int> qt(hs.r1Col.size(), -1), springQ;
+    for (int q : hs.cSlotR1)
+      if (q >= 0 && qt[q] < 0) { qt[q] = 0; springQ.push_back(q); }
+    std::sort(springQ.begin(), springQ.end());
+    for (size_t t = 0; t < springQ.size(); t++) qt[springQ[t]] = (int)t;
+    std::vector<int> sOff(springQ.size() + 1, 0);
+    for (int64_t e = 0; e < nE; e++)
+      for (int64_t q = L.pOff[e]; q < L.pOff[e + 1]; q++) {
+        if (qt[L.pIa[q]] >= 0) sOff[qt[L.pIa[q]] + 1]++;
+        if (qt[L.pIb[q]] >= 0) sOff[qt[L.pIb[q]] + 1]++;
+      }
+    for (size_t t = 0; t < springQ.size(); t++) sOff[t + 1] += sOff[t];
+    std::vector<int64_t> sEnt(sOff.back());
+    std::vector<int> sMid(sOff.back()), sOther(sOff.back());
+    std::vector<int> sf(sOff.begin(), sOff.end() - 1);
+    for (int64_t e = 0; e < nE; e++)
+      for (int64_t q = L.pOff[e]; q < L.pOff[e + 1]; q++) {
+        int ia = L.pIa[q], ib = L.pIb[q];
+        if (qt[ia] >= 0) { int at = sf[qt[ia]]++; sEnt[at] = e; sMid[at] = L.pK[q]; sOther[at] = ib; }
+        if (qt[ib] >= 0) { int at = sf[qt[ib]]++; sEnt[at] = e; sMid[at] = L.pK[q]; sOther[at] = ia; }
+      }
+    A->am.nSpringQ = (int)springQ.size();
+    if ((rc = dupload(A, &A->am.midOff, midOff)) || (rc = dupload(A, &A->am.midEnt, midEnt)) ||
+        (rc = dupload(A, &A->am.midA, midA)) || (rc = dupload(A, &A->am.springQ, springQ)) ||
+        (rc = dupload(A, &A->am.springOff, sOff)) || (rc = dupload(A, &A->am.springEnt, sEnt)) ||
+        (rc = dupload(A, &A->am.springMid, sMid)) || (rc = dupload(A, &A->am.springOther, sOther)))
+      return rc;
+  }
   for (double** p : {&A->ax, &A->axo, &A->asol, &A->uA, &A->uB, &A->uC, &A->uc, &A->bbar, &A->aA, &A->rhs0bar,
                      &A->axn, &A->colR[0], &A->colR[1], &A->xbarA, &A->vbarA, &A->xbarB, &A->vbarB, &A->endX,
                      &A->endV})
@@ -154,7 +232,61 @@ int ensure_state(pdipc_handle* h) {
   if ((rc = dalloc(A, &A->sbar, hs.cNbr.size()))) return rc;
   if ((rc = dalloc(A, &A->partials, (size_t)m.nPartials + 1))) return rc;
   if ((rc = dalloc(A, &A->gscale, 1))) return rc;
+  if ((rc = dalloc(A, &A->colRacc, V3))) return rc;
+  A->lay.V3 = V3;
+  A->lay.nSim = (size_t)m.nSim;
+  A->lay.LG = h->prm.lg_iters;
+  A->lay.K = h->prm.cheb_iters;
+  if ((rc = dalloc(A, &A->stage, A->lay.bytes() / sizeof(double)))) return rc;
+  A->uS.assign(h->prm.cheb_iters + 2, nullptr);
+  for (auto& u : A->uS)
+    if ((rc = dalloc(A, &u, V3))) return rc;
   return PDIPC_OK;
+}
+
+// replays `g` (capturing `body` on first use) or, without graphs, runs `body` directly
+template <class F>
+int run_captured(pdipc_handle* h, hipGraphExec_t& g, F&& body) {
+  if (!h->prm.use_graphs) {
+    body();
+    return check_launch(h);
+  }
+  if (!g) {
+    hipGraph_t graph = nullptr;
+    HIPCHK(hipStreamBeginCapture(h->stream, hipStreamCaptureModeThreadLocal));
+    body();
+    HIPCHK(hipStreamEndCapture(h->stream, &graph));
+    hipError_t e = hipGraphInstantiate(&g, graph, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(graph);
+    if (e != hipSuccess) {
+      g = nullptr;
+      return fail(PDIPC_E_HIP, std::string("hipGraphInstantiate: ") + hipGetErrorString(e));
+    }
+  }
+  HIPCHK(hipGraphLaunch(g, h->stream));
+  return check_launch(h);
+}
+
+// the recorded local/global loop of one outer iteration, into the stage
+void enqueue_taped_lg(pdipc_handle* h, int withCol) {
+  AdjointState* A = h->adj;
+  DeviceMesh& m = h->m;
+  const pdipc_params& p = h->prm;
+  hipStream_t s = h->stream;
+  const StageLayout& L = A->lay;
+  double* b = A->stage;
+  launch_copy(L.xold(b), m.Xold, (int64_t)L.V3, s);
+  launch_copy(L.xstart(b), m.X, (int64_t)L.V3, s);
+  launch_diag(m, withCol, s);
+  launch_copy(L.D(b), m.sysDiag, m.nSim, s);
+  launch_operators(m, p.omega, PDIPC_FP64, s);
+  std::vector<double*> outs(p.cheb_iters + 1);
+  for (int l = 0; l < p.lg_iters; l++) {
+    launch_local(m, PDIPC_FP64, p.sigma_lo, p.sigma_hi, withCol, s);
+    launch_copy(L.rhs(b, l), m.sysRhs, (int64_t)L.V3, s);
+    for (int k = 0; k <= p.cheb_iters; k++) outs[k] = L.it(b, l, k);
+    launch_solve_taped(m, p.omega, p.rho, p.cheb_iters, outs.data(), s);
+  }
 }
 
 int slot(AdjointState* A, size_t n, double** out) { return A->tape.alloc(n * sizeof(double), (void**)out); }
@@ -181,24 +313,9 @@ int taped_step(pdipc_handle* h, StepTape& st) {
   for (; outer < p.max_outer; outer++) {
     OuterTape ot;
     ot.withCol = activeCol ? 1 : 0;
-    if ((rc = slot(A, V3, &ot.xold)) || (rc = slot(A, V3, &ot.xstart)) || (rc = slot(A, (size_t)m.nSim, &ot.D)))
-      return rc;
-    launch_copy(ot.xold, m.Xold, (int64_t)V3, s);
-    launch_copy(ot.xstart, m.X, (int64_t)V3, s);
-    launch_diag(m, ot.withCol, s);
-    launch_copy(ot.D, m.sysDiag, m.nSim, s);
-    launch_operators(m, p.omega, PDIPC_FP64, s);
-    ot.b.assign(p.lg_iters, nullptr);
-    ot.it.assign(p.lg_iters, std::vector<double*>(p.cheb_iters + 1, nullptr));
-    for (int l = 0; l < p.lg_iters; l++) {
-      launch_local(m, PDIPC_FP64, p.sigma_lo, p.sigma_hi, ot.withCol, s);
-      if ((rc = slot(A, V3, &ot.b[l]))) return rc;
-      launch_copy(ot.b[l], m.sysRhs, (int64_t)V3, s);
-      ot.it[l][0] = l == 0 ? ot.xstart : ot.it[l - 1][p.cheb_iters];
-      for (int k = 1; k <= p.cheb_iters; k++)
-        if ((rc = slot(A, V3, &ot.it[l][k]))) return rc;
-      launch_solve_taped(m, p.omega, p.rho, p.cheb_iters, ot.it[l].data(), s);
-    }
+    if ((rc = run_captured(h, A->fwdGraph[ot.withCol], [&]() { enqueue_taped_lg(h, ot.withCol); }))) return rc;
+    if ((rc = A->tape.alloc(A->lay.bytes(), (void**)&ot.blob))) return rc;
+    HIPCHK(hipMemcpyAsync(ot.blob, A->stage, A->lay.bytes(), hipMemcpyDeviceToDevice, s));
     launch_direction(m, s);
     if (outer == 0) {
       if ((rc = broad_phase(h, m.Xold, m.Dir))) return rc;
@@ -222,6 +339,10 @@ int taped_step(pdipc_handle* h, StepTape& st) {
     launch_clamp_residual(m, p.toi_scale, s);
     if ((rc = collision(h))) return rc;  // reads the scalars back
     activeCol = true;
+    ot.nAct = h->hCounters()[2];
+    if ((rc = A->tape.alloc((size_t)std::max(1, ot.nAct) * sizeof(int), (void**)&ot.act))) return rc;
+    if (ot.nAct > 0)
+      HIPCHK(hipMemcpyAsync(ot.act, m.actRec, (size_t)ot.nAct * sizeof(int), hipMemcpyDeviceToDevice, s));
     double res = *h->hResidual();
     double tmin;
     unsigned long long bits = *h->hToi();
@@ -272,61 +393,59 @@ int reverse_step(pdipc_handle* h, StepTape& st, const double* xbarIn, const doub
   launch_zero(A->colD[cur], m.nSim, s);
   launch_zero(A->colR[cur], (int64_t)V3, s);
   const int O = (int)st.outers.size();
+  const StageLayout& L = A->lay;
+  double* sg = A->stage;
   for (int o = O - 1; o >= 0; o--) {
     OuterTape& ot = st.outers[o];
-    const double* xsol = ot.it[LG - 1][K];
+    HIPCHK(hipMemcpyAsync(sg, ot.blob, L.bytes(), hipMemcpyDeviceToDevice, s));
+    const double* xsol = L.it(sg, LG - 1, K);
+    const double* xold = L.xold(sg);
     // collision terms computed at the end of this outer iteration (read by the next)
     if (o < O - 1)
-      launch_adj_collision(m, st.recs, st.nRecs, st.outers[o + 1].xold, p.dhat, p.kappa, p.dt, A->colD[cur],
-                           A->colR[cur], A->ax, s);
+      launch_adj_collision(m, st.recs, ot.act, ot.nAct, L.xold(st.outers[o + 1].blob), p.dhat, p.kappa, p.dt,
+                           A->colD[cur], A->colR[cur], A->ax, s);
     // x = xold + toi (xsol - xold)
     launch_axpby(m, ot.toi, A->ax, 0.0, nullptr, A->asol, s);
     launch_axpby(m, 1.0 - ot.toi, A->ax, 0.0, nullptr, A->axo, s);
     if (ot.clamped && ot.argmin >= 0) {
-      launch_dot_dir(m, A->ax, xsol, ot.xold, p.toi_scale, A->partials, A->gscale, s);
-      launch_adj_ccd(m, st.recs, ot.argmin, ot.xold, xsol, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap,
+      launch_dot_dir(m, A->ax, xsol, xold, p.toi_scale, A->partials, A->gscale, s);
+      launch_adj_ccd(m, st.recs, ot.argmin, xold, xsol, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap,
                      p.touch_policy, A->gscale, A->axo, A->asol, s);
     }
     int nxt = 1 - cur;
     launch_zero(A->Gbar, m.sellEntries, s);
     launch_zero(A->Dbar, m.nSim, s);
-    if (o > 0) {
-      launch_zero(A->colD[nxt], m.nSim, s);
-      launch_zero(A->colR[nxt], (int64_t)V3, s);
-    }
+    if (ot.withCol) launch_zero(A->colRacc, (int64_t)V3, s);
     // the operators of this outer iteration, from its recorded diagonal
-    launch_copy(m.sysDiag, ot.D, m.nSim, s);
+    launch_copy(m.sysDiag, L.D(sg), m.nSim, s);
     HIPCHK(hipMemsetAsync(m.dChanged, 1, m.nSim, s));
     HIPCHK(hipMemsetAsync(m.dStale, 1, m.nSim, s));  // the next plain step recomputes every row
     launch_operators(m, p.omega, PDIPC_FP64, s);
-    for (int l = LG - 1; l >= 0; l--) {
-      const std::vector<double*>& it = ot.it[l];
-      launch_div_rows(m, A->asol, m.sysDiag, A->uB, s);  // u_K
-      launch_zero(A->uA, (int64_t)V3, s);                // u_{K+1}
-      launch_zero(A->uc, (int64_t)V3, s);
-      for (int k = K - 1; k >= 0; k--) {
-        double b = k + 1 <= K - 1 ? 1.0 - W[k + 1] : 0.0;
-        launch_adj_sweep(m, W[k], b, A->uB, A->uA, it[k], A->uC, A->Gbar, A->uc, s);
-        double* t = A->uA;
-        A->uA = A->uB;
-        A->uB = A->uC;
-        A->uC = t;
+    int rc = run_captured(h, A->revGraph[ot.withCol], [&]() {
+      std::vector<const double*> x(K);
+      for (int l = LG - 1; l >= 0; l--) {
+        launch_div_rows(m, A->asol, m.sysDiag, A->uS[K], s);  // u_K
+        launch_zero(A->uc, (int64_t)V3, s);
+        for (int k = 0; k < K; k++) x[k] = L.it(sg, l, k);
+        launch_adj_pass(m, W.data(), K, A->uS.data(), x.data(), A->Gbar, A->uc, s);
+        launch_adj_first(m, p.omega, L.rhs(sg, l), A->uc, A->bbar, A->Dbar, A->R1bar, s);
+        launch_adj_local(m, A->am, p.sigma_lo, p.sigma_hi, L.it(sg, l, 0), A->bbar, A->uS[0], A->aA, A->tetAdj,
+                         A->asol, A->sbar, A->rhs0bar, ot.withCol ? A->colRacc : nullptr, s);
       }
-      launch_adj_first(m, p.omega, ot.b[l], A->uc, A->bbar, A->Dbar, A->R1bar, s);
-      launch_adj_local(m, A->am, p.sigma_lo, p.sigma_hi, it[0], A->bbar, A->uB, A->aA, A->tetAdj, A->asol, A->sbar,
-                       A->rhs0bar, ot.withCol ? A->colR[nxt] : nullptr, s);
-    }
-    launch_adj_operators(m, A->am, p.omega, A->Gbar, A->Dbar, A->R1bar, s);
+    });
+    if (rc) return rc;
+    launch_adj_ops_gather(m, A->am, p.omega, A->Gbar, A->Dbar, A->R1bar, s);
     launch_add_rows(A->DbarEl, A->Dbar, m.nSim, s);
     if (o > 0) {
       launch_copy(A->colD[nxt], A->Dbar, m.nSim, s);
+      launch_copy(A->colR[nxt], A->colRacc, (int64_t)V3, s);
       launch_axpby(m, 1.0, A->axo, 1.0, A->asol, A->ax, s);  // both read x_clamp of outer o-1
       cur = nxt;
     } else {
       launch_axpby(m, 1.0, A->axn, 1.0, A->axo, A->uC, s);  // xold of the first outer iteration is x_n
       launch_predict_adj(m, p.dt, A->asol, A->rhs0bar, A->uC, xbarOut, vbarOut, s);
     }
-    if (int rc = check_launch(h)) return rc;
+    if ((rc = check_launch(h))) return rc;
   }
   return PDIPC_OK;
 }
@@ -342,6 +461,7 @@ int pdipc_rollout(pdipc_handle* h, int nsteps, int every) {
   int rc = require_device(h);
   if (rc) return rc;
   if (nsteps < 1 || every < 1) return fail(PDIPC_E_ARG, "nsteps and checkpoint_every must be positive");
+  if (h->prm.cheb_iters > kMaxCheb) return fail(PDIPC_E_ARG, "the adjoint supports at most 32 Chebyshev sweeps");
   if (h->prm.precision != PDIPC_FP64) return fail(PDIPC_E_ARG, "the adjoint runs on FP64 steps");
   if ((rc = ensure_state(h))) return rc;
   AdjointState* A = h->adj;
@@ -465,21 +585,14 @@ int pdipc_debug_solve_vjp(pdipc_handle* h, const double* diag, const double* rhs
   launch_operators(m, p.omega, PDIPC_FP64, s);
   const std::vector<double> W = cheb_weights(p);
   const int K = p.cheb_iters;
-  launch_div_rows(m, A->asol, m.sysDiag, A->uB, s);
-  launch_zero(A->uA, (int64_t)V3, s);
+  launch_div_rows(m, A->asol, m.sysDiag, A->uS[K], s);
   launch_zero(A->uc, (int64_t)V3, s);
   launch_zero(A->Gbar, m.sellEntries, s);
-  for (int k = K - 1; k >= 0; k--) {
-    double b = k + 1 <= K - 1 ? 1.0 - W[k + 1] : 0.0;
-    launch_adj_sweep(m, W[k], b, A->uB, A->uA, A->axo, A->uC, A->Gbar, A->uc, s);
-    double* t = A->uA;
-    A->uA = A->uB;
-    A->uB = A->uC;
-    A->uC = t;
-  }
+  std::vector<const double*> x(K, A->axo);
+  launch_adj_pass(m, W.data(), K, A->uS.data(), x.data(), A->Gbar, A->uc, s);
   launch_zero(A->Dbar, m.nSim, s);
   launch_adj_first(m, p.omega, A->bbar, A->uc, A->ax, A->Dbar, A->R1bar, s);
-  launch_mul_rows(m, A->uB, m.sysDiag, A->asol, s);
+  launch_mul_rows(m, A->uS[0], m.sysDiag, A->asol, s);
   if ((rc = check_launch(h))) return rc;
   if ((rc = get_soa(h, A->asol, xbar0, m.nSim))) return rc;
   return get_soa(h, A->ax, bbar, m.nSim);
@@ -518,14 +631,14 @@ int pdipc_debug_tape(pdipc_handle* h, int step, int outer, int which, int l, int
     out[3] = ot.withCol;
     return PDIPC_OK;
   }
+  const StageLayout& L = A->lay;
   const double* src = nullptr;
   switch (which) {
-    case 0: src = ot.xold; break;
-    case 1: src = ot.xstart; break;
-    case 2: src = ot.D; break;
-    case 3: src = (l >= 0 && l < (int)ot.b.size()) ? ot.b[l] : nullptr; break;
-    default:
-      src = (l >= 0 && l < (int)ot.it.size() && k >= 0 && k < (int)ot.it[l].size()) ? ot.it[l][k] : nullptr;
+    case 0: src = L.xold(ot.blob); break;
+    case 1: src = L.xstart(ot.blob); break;
+    case 2: src = L.D(ot.blob); break;
+    case 3: src = (l >= 0 && l < L.LG) ? L.rhs(ot.blob, l) : nullptr; break;
+    default: src = (l >= 0 && l < L.LG && k >= 0 && k <= L.K) ? L.it(ot.blob, l, k) : nullptr;
   }
   if (!src) return fail(PDIPC_E_ARG, "no such recorded vector");
   if (which == 2) {

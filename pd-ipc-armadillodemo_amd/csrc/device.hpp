@@ -116,14 +116,27 @@ struct AdjMesh {
   int *pIa = nullptr, *pIb = nullptr;  // per 2-path (SELL order): R1 indices of its factors
   int* cSlotR1 = nullptr;    // per cloth slot: R1 index of (center, nbr) or -1
   double *cBendOut = nullptr, *cLboIn = nullptr;  // per cloth slot
+  // the 2-paths grouped by their middle point k: (SELL entry, A_p)
+  int* midOff = nullptr;
+  int64_t* midEnt = nullptr;
+  double* midA = nullptr;
+  // per spring R1 entry q: the 2-paths using q as a factor (SELL entry, middle point, other factor)
+  int nSpringQ = 0;
+  int *springQ = nullptr, *springOff = nullptr, *springMid = nullptr, *springOther = nullptr;
+  int64_t* springEnt = nullptr;
 };
 void launch_solve_taped(const DeviceMesh& m, double omega, double rho, int iters, double* const* outs, hipStream_t s);
-void launch_adj_sweep(const DeviceMesh& m, double a, double b, const double* uin, const double* uoth, const double* xk,
-                      double* uout, double* Gbar, double* uc, hipStream_t s);
+// the whole reverse of one K-sweep solve: u[k] for k = K-1 .. 0 from u[K] (given)
+// and u[K+1] (zero), then one SELL pass adding sum_k W_k u_{k+1} x_k^T to Gbar
+constexpr int kMaxCheb = 32;
+void launch_adj_pass(const DeviceMesh& m, const double* W, int K, double* const* u, const double* const* x,
+                     double* Gbar, double* uc, hipStream_t s);
 void launch_adj_first(const DeviceMesh& m, double omega, const double* b, const double* uc, double* bbar, double* Dbar,
                       double* R1bar, hipStream_t s);
-void launch_adj_operators(const DeviceMesh& m, const AdjMesh& a, double omega, const double* Gbar, double* Dbar,
-                          double* R1bar, hipStream_t s);
+// adjoint of the operator update (M^2 from D and R1) as three deterministic
+// gathers over the lists in AdjMesh
+void launch_adj_ops_gather(const DeviceMesh& m, const AdjMesh& a, double omega, const double* Gbar, double* Dbar,
+                           double* R1bar, hipStream_t s);
 // cloth curvature + tet projection adjoints, then the per-point gather:
 //   ax = D u0 + J_local(x)^T bbar;  sbar += d rhs / d spring weight;  rhs0bar += bbar;
 //   colRbar += bbar (when the collision rhs entered the local step)
@@ -135,7 +148,8 @@ void launch_adj_local(const DeviceMesh& m, const AdjMesh& a, double sigLo, doubl
 void launch_ccd_records(const DeviceMesh& m, const int4* recs, int nRecs, const double* x0, const double* x1,
                         double filter, int iters, double step, double gap, int touchPolicy, double* recToi,
                         int* argmin, hipStream_t s);
-void launch_adj_collision(const DeviceMesh& m, const int4* recs, int nRecs, const double* x, double dhat,
+// over the active records `act` (indices into recs) of that collision pass
+void launch_adj_collision(const DeviceMesh& m, const int4* recs, const int* act, int nAct, const double* x, double dhat,
                           double kappa, double dt, const double* colDbar, const double* colRbar, double* ax,
                           hipStream_t s);
 void launch_adj_ccd(const DeviceMesh& m, const int4* recs, int argmin, const double* x0, const double* x1,

@@ -440,14 +440,14 @@ __global__ void k_col_gather(int nSim, int ld, const int* __restrict__ ptOff, un
 // collision diagonal and rhs (rows of simulated points), accumulate into ax the
 // gradient of  sum_q colDbar_q w + colRbar_q . (w target_q)  with respect to the
 // simulated points' positions (forward-mode over the 12 coordinates).
-__global__ void __launch_bounds__(64) k_adj_collision(int nRecs, int ld, int nSim, const int4* __restrict__ recs,
-                                                      const double* __restrict__ X, const double* __restrict__ cm,
-                                                      double dHat, double kappa, double dt,
-                                                      const double* __restrict__ colDbar,
+__global__ void __launch_bounds__(64) k_adj_collision(int nAct, int ld, int nSim, const int4* __restrict__ recs,
+                                                      const int* __restrict__ act, const double* __restrict__ X,
+                                                      const double* __restrict__ cm, double dHat, double kappa,
+                                                      double dt, const double* __restrict__ colDbar,
                                                       const double* __restrict__ colRbar, double* __restrict__ ax) {
-  int t = blockIdx.x * blockDim.x + threadIdx.x;
-  if (t >= nRecs) return;
-  int4 r = recs[t];
+  int a = blockIdx.x * blockDim.x + threadIdx.x;
+  if (a >= nAct) return;
+  int4 r = recs[act[a]];
   int type = r.x < 0 ? 1 : 0;
   int id[4];
   if (!record_ids(r, id)) return;
@@ -541,41 +541,38 @@ __global__ void k_toi_argmin(int nRecs, const double* __restrict__ recToi, const
   if (tmin < 1.0 && recToi[t] == tmin) atomicMin(argmin, t);
 }
 
-// d toi / d (x0, x1) of one record, scaled by g, added to ax0 / ax1 (simulated points)
-__global__ void k_adj_ccd(int ld, int nSim, const int4* __restrict__ recs, int argmin,
-                          const double* __restrict__ X0, const double* __restrict__ X1, double filter, int iters,
-                          double step, double gapf, int touchPolicy, const double* __restrict__ gscale,
-                          double* __restrict__ ax0, double* __restrict__ ax1) {
-  if (blockIdx.x != 0 || threadIdx.x != 0) return;
-  int t = argmin;
-  if (t < 0) return;
+// d toi / d (x0, x1) of one record, scaled by g, added to ax0 / ax1 (simulated
+// points): thread c of 24 differentiates along input coordinate c
+__global__ void k_adj_ccd(int ld, int nSim, const int4* __restrict__ recs, int argmin, const double* __restrict__ X0,
+                          const double* __restrict__ X1, double filter, int iters, double step, double gapf,
+                          int touchPolicy, const double* __restrict__ gscale, double* __restrict__ ax0,
+                          double* __restrict__ ax1) {
+  int c = threadIdx.x;
+  if (blockIdx.x != 0 || c >= 24 || argmin < 0) return;
   double g = *gscale;
   if (g == 0.0) return;
-  int4 r = recs[t];
+  int4 r = recs[argmin];
   int type = r.x < 0 ? 1 : 0;
   int id[4];
   record_ids(r, id);
-  using D24 = Dual<24>;
-  P3T<D24> x[4], dx[4];
+  using D1 = Dual<1>;
+  P3T<D1> x[4], dx[4];
 #pragma unroll
   for (int q = 0; q < 4; q++)
 #pragma unroll
     for (int k = 0; k < 3; k++) {
       // X1 holds the simulated rows only: static points end where they start
       const double* src1 = id[q] < nSim ? X1 : X0;
-      x[q].v[k] = D24::seed(X0[k * ld + id[q]], 3 * q + k);
-      D24 y = D24::seed(src1[k * ld + id[q]], 12 + 3 * q + k);
+      x[q].v[k] = c == 3 * q + k ? D1::seed(X0[k * ld + id[q]], 0) : D1(X0[k * ld + id[q]]);
+      D1 y = c == 12 + 3 * q + k ? D1::seed(src1[k * ld + id[q]], 0) : D1(src1[k * ld + id[q]]);
       dx[q].v[k] = y - x[q].v[k];
     }
-  D24 toc = accd(type, x, dx, iters, step, gapf, touchPolicy);
-#pragma unroll
-  for (int q = 0; q < 4; q++)
-    if (id[q] < nSim)
-#pragma unroll
-      for (int k = 0; k < 3; k++) {
-        atomicAdd(ax0 + k * ld + id[q], g * toc.d[3 * q + k]);
-        atomicAdd(ax1 + k * ld + id[q], g * toc.d[12 + 3 * q + k]);
-      }
+  D1 toc = accd(type, x, dx, iters, step, gapf, touchPolicy);
+  int q = (c % 12) / 3, k = c % 3;
+  if (id[q] < nSim) {
+    double* dst = c < 12 ? ax0 : ax1;
+    dst[k * ld + id[q]] += g * toc.d[0];
+  }
 }
 
 }  // namespace
@@ -601,11 +598,11 @@ void launch_ccd_records(const DeviceMesh& m, const int4* recs, int nRecs, const 
                      argmin);
 }
 
-void launch_adj_collision(const DeviceMesh& m, const int4* recs, int nRecs, const double* x, double dhat,
+void launch_adj_collision(const DeviceMesh& m, const int4* recs, const int* act, int nAct, const double* x, double dhat,
                           double kappa, double dt, const double* colDbar, const double* colRbar, double* ax,
                           hipStream_t s) {
-  if (nRecs <= 0) return;
-  hipLaunchKernelGGL(k_adj_collision, dim3((nRecs + 63) / 64), dim3(64), 0, s, nRecs, m.ld, m.nSim, recs, x,
+  if (nAct <= 0) return;
+  hipLaunchKernelGGL(k_adj_collision, dim3((nAct + 63) / 64), dim3(64), 0, s, nAct, m.ld, m.nSim, recs, act, x,
                      m.colMass, dhat, kappa, dt, colDbar, colRbar, ax);
 }
 

@@ -253,19 +253,18 @@ void solve_taped_impl(const DeviceMesh& m, double w, double rho, int iters, doub
   launch_axpby(m, 1.0, outs[iters], 0.0, nullptr, m.X, s);
 }
 
-// Reverse sweep of the Chebyshev recurrence in the scaled adjoint u = D^{-1} xbar
-// (M^T = D M D^{-1} because R1 is symmetric, so M^T xbar = D M u):
-//   u_out = a M u_in + b u_oth,   Gbar_e += a u_in[row] . x_k[col],   uc[row] += a u_in[row]
-// Gbar accumulates D^{-1}-scaled adjoints of the M^2 entries, uc the scaled
-// adjoint of the first term c.
+// Reverse sweeps of the Chebyshev recurrence in the scaled adjoint u = D^{-1} xbar
+// (M^T = D M D^{-1} because R1 is symmetric, so M^T xbar = D M u): each is the
+// forward SELL SpMV on u,
+//   u_out = a M u_in + b u_oth,   uc[row] += a u_in[row]   (uc: D^{-1}-scaled first-term adjoint);
+// the M^2-entry adjoints are gathered once per solve by k_adj_gbar.
 template <int G>
-__global__ void __launch_bounds__(kBlock) k_adj_sweep(int n, int nSlices, int ld, double a, double b,
-                                                      const int64_t* __restrict__ sliceOff,
-                                                      const int* __restrict__ sliceW, const int* __restrict__ col,
-                                                      const double* __restrict__ val, const double* __restrict__ uin,
-                                                      const double* __restrict__ uoth, const double* __restrict__ xk,
-                                                      double* __restrict__ uout, double* __restrict__ Gbar,
-                                                      double* __restrict__ uc) {
+__global__ void __launch_bounds__(kBlock) k_adj_spmv(int n, int nSlices, int ld, double a, double b,
+                                                     const int64_t* __restrict__ sliceOff,
+                                                     const int* __restrict__ sliceW, const int* __restrict__ col,
+                                                     const double* __restrict__ val, const double* __restrict__ uin,
+                                                     const double* __restrict__ uoth, double* __restrict__ uout,
+                                                     double* __restrict__ uc) {
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int slice = gt / kWave, lane = gt % kWave;
   if (slice >= nSlices) return;
@@ -274,19 +273,15 @@ __global__ void __launch_bounds__(kBlock) k_adj_sweep(int n, int nSlices, int ld
   int sub = lane % G;
   int64_t off = sliceOff[slice] + lane;
   int width = sliceW[slice];
-  double ui[3] = {0.0, 0.0, 0.0};
-  if (row < n) { ui[0] = uin[row]; ui[1] = uin[ld + row]; ui[2] = uin[2 * ld + row]; }
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
     int64_t e = off + (int64_t)c * kWave;
     int j = col[e];
     double v = val[e];
-    double u0 = uin[j], u1 = uin[ld + j], u2 = uin[2 * ld + j];
-    acc[0] += v * u0;
-    acc[1] += v * u1;
-    acc[2] += v * u2;
-    Gbar[e] += a * (ui[0] * xk[j] + ui[1] * xk[ld + j] + ui[2] * xk[2 * ld + j]);
+    acc[0] += v * uin[j];
+    acc[1] += v * uin[ld + j];
+    acc[2] += v * uin[2 * ld + j];
   }
   group_reduce<G>(acc);
   if (sub != 0 || row >= n) return;
@@ -294,16 +289,99 @@ __global__ void __launch_bounds__(kBlock) k_adj_sweep(int n, int nSlices, int ld
   for (int k = 0; k < 3; k++) {
     int idx = k * ld + row;
     uout[idx] = a * acc[k] + (b != 0.0 ? b * uoth[idx] : 0.0);
-    uc[idx] += a * ui[k];
+    uc[idx] += a * uin[idx];
+  }
+}
+
+// Gbar_e += sum_k W_k u_{k+1}[row] . x_k[col] over the K sweeps of one pass
+// (u_{k+1} and x_k from the pass's reverse and recorded forward iterates)
+struct GbarArgs {
+  const double* u[kMaxCheb + 1];  // u_1 .. u_K
+  const double* x[kMaxCheb];      // x_0 .. x_{K-1}
+  double w[kMaxCheb];
+  int K;
+};
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_adj_gbar(int n, int nSlices, int ld, const int64_t* __restrict__ sliceOff,
+                                                     const int* __restrict__ sliceW, const int* __restrict__ col,
+                                                     GbarArgs ga, double* __restrict__ Gbar) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int slice = gt / kWave, lane = gt % kWave;
+  if (slice >= nSlices) return;
+  constexpr int S = kWave / G;
+  int row = slice * S + lane / G;
+  if (row >= n) return;  // padded rows (their entries' adjoints are never read)
+  int64_t off = sliceOff[slice] + lane;
+  int width = sliceW[slice];
+  for (int c = 0; c < width; c++) {
+    int64_t e = off + (int64_t)c * kWave;
+    int j = col[e];
+    double g = 0.0;
+    for (int k = 0; k < ga.K; k++) {
+      const double* u = ga.u[k + 1];
+      const double* x = ga.x[k];
+      g += ga.w[k] * (u[row] * x[j] + u[ld + row] * x[ld + j] + u[2 * ld + row] * x[2 * ld + j]);
+    }
+    Gbar[e] += g;
+  }
+}
+
+template <int G, int KK>
+__global__ void __launch_bounds__(kBlock) k_adj_gbar_k(int n, int nSlices, int ld, const int64_t* __restrict__ sliceOff,
+                                                       const int* __restrict__ sliceW, const int* __restrict__ col,
+                                                       GbarArgs ga, double* __restrict__ Gbar) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int slice = gt / kWave, lane = gt % kWave;
+  if (slice >= nSlices) return;
+  constexpr int S = kWave / G;
+  int row = slice * S + lane / G;
+  if (row >= n) return;
+  int64_t off = sliceOff[slice] + lane;
+  int width = sliceW[slice];
+  double ur[KK][3];
+#pragma unroll
+  for (int k = 0; k < KK; k++) {
+    const double* u = ga.u[k + 1];
+    double wk = ga.w[k];
+    ur[k][0] = wk * u[row];
+    ur[k][1] = wk * u[ld + row];
+    ur[k][2] = wk * u[2 * ld + row];
+  }
+  for (int c = 0; c < width; c++) {
+    int64_t e = off + (int64_t)c * kWave;
+    int j = col[e];
+    double g = 0.0;
+#pragma unroll
+    for (int k = 0; k < KK; k++) {
+      const double* x = ga.x[k];
+      g += ur[k][0] * x[j] + ur[k][1] * x[ld + j] + ur[k][2] * x[2 * ld + j];
+    }
+    Gbar[e] += g;
   }
 }
 
 template <int G>
-void adj_sweep_impl(const DeviceMesh& m, double a, double b, const double* uin, const double* uoth, const double* xk,
-                    double* uout, double* Gbar, double* uc, hipStream_t s) {
+void adj_pass_impl(const DeviceMesh& m, const double* W, int K, double* const* u, const double* const* x,
+                   double* Gbar, double* uc, hipStream_t s) {
   const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL((k_adj_sweep<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, a, b, m.sliceOff,
-                     m.sliceW, m.m2Col, m.m2Val, uin, uoth, xk, uout, Gbar, uc);
+  for (int k = K - 1; k >= 0; k--) {
+    double b = k + 1 <= K - 1 ? 1.0 - W[k + 1] : 0.0;
+    hipLaunchKernelGGL((k_adj_spmv<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W[k], b, m.sliceOff,
+                       m.sliceW, m.m2Col, m.m2Val, u[k + 1], u[k + 2], u[k], uc);
+  }
+  GbarArgs ga;
+  ga.K = K;
+  for (int k = 0; k <= K; k++) ga.u[k] = u[k];
+  for (int k = 0; k < K; k++) {
+    ga.x[k] = x[k];
+    ga.w[k] = W[k];
+  }
+  if (K == 15)
+    hipLaunchKernelGGL((k_adj_gbar_k<G, 15>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff,
+                       m.sliceW, m.m2Col, ga, Gbar);
+  else
+    hipLaunchKernelGGL((k_adj_gbar<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff,
+                       m.sliceW, m.m2Col, ga, Gbar);
 }
 
 // Adjoint of the first term c_i = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
@@ -341,33 +419,81 @@ __global__ void k_adj_first(int n, int ld, double w, const int* __restrict__ off
   Dbar[i] += db;
 }
 
-// Adjoint of the operator update: from Gbar (M^2-entry adjoints scaled by D_row^{-1})
-// to the diagonal D and the R1 values (through the 2-path products and R1_ij).
-__global__ void k_adj_operators(int64_t nE, double w, const int* __restrict__ rowFlag, const double* __restrict__ r1e,
-                                const int* __restrict__ r1Idx, const int64_t* __restrict__ pOff,
-                                const int* __restrict__ pK, const double* __restrict__ pA, const int* __restrict__ pIa,
-                                const int* __restrict__ pIb, const double* __restrict__ r1Val,
-                                const double* __restrict__ D, const double* __restrict__ Gbar,
-                                double* __restrict__ Dbar, double* __restrict__ R1bar) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nE) return;
-  double g = Gbar[e];
-  if (g == 0.0) return;
-  int row = rowFlag[e] & 0x7fffffff;
-  const double w2 = w * w;
-  double s = 0.0;
-  for (int64_t p = pOff[e]; p < pOff[e + 1]; p++) {
-    int k = pK[p];
-    double dk = D[k];
-    s += pA[p] / dk;
-    atomicAdd(Dbar + k, -g * w2 * pA[p] / (dk * dk));
-    double abar = g * w2 / dk;
-    atomicAdd(R1bar + pIa[p], abar * r1Val[pIb[p]]);
-    atomicAdd(R1bar + pIb[p], abar * r1Val[pIa[p]]);
+// Adjoint of the operator update as three gathers (no atomics, fixed order):
+//   rows: Dbar_i -= sum_e Gbar_e (w^2 s_e + 2 w (1-w) R1_e) / D_i over row i's
+//         entries (s_e = sum_paths A_p / D_k), and R1bar at each entry's own
+//         R1 position += 2 w (1-w) Gbar_e;
+//   mid:  Dbar_k -= (w^2 / D_k^2) sum_{paths through k} Gbar_e A_p;
+//   r1:   R1bar_q += sum_{paths using q} (w^2 / D_k) Gbar_e R1_other (spring entries only,
+//         the R1 values the stiffness gradient reads).
+// wave-wide sum in a fixed order (xor tree): deterministic
+__device__ inline double wave_sum(double v) {
+#pragma unroll
+  for (int o = kWave / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+// one wavefront per row; lane t takes the row's entries t, t + 64, ...
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_adj_ops_rows(int n, const int64_t* __restrict__ sliceOff,
+                                                         const int* __restrict__ sliceW, const double* __restrict__ r1e,
+                                                         const int* __restrict__ r1Idx,
+                                                         const int64_t* __restrict__ pOff, const int* __restrict__ pK,
+                                                         const double* __restrict__ pA, const double* __restrict__ D,
+                                                         const double* __restrict__ Gbar, double w,
+                                                         double* __restrict__ Dbar, double* __restrict__ R1bar) {
+  int row = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  int lane = threadIdx.x % kWave;
+  if (row >= n) return;  // whole wavefront
+  constexpr int S = kWave / G;
+  int slice = row / S, lr = row % S;
+  int64_t base = sliceOff[slice] + lr * G;
+  int cnt = sliceW[slice] * G;
+  const double w2 = w * w, c1 = 2.0 * (1.0 - w) * w;
+  double acc = 0.0;
+  for (int t = lane; t < cnt; t += kWave) {
+    int64_t e = base + (int64_t)(t / G) * kWave + (t % G);
+    double g = Gbar[e];
+    if (g == 0.0) continue;
+    double sp = 0.0;
+    for (int64_t p = pOff[e]; p < pOff[e + 1]; p++) sp += pA[p] / D[pK[p]];
+    acc += g * (w2 * sp + c1 * r1e[e]);
+    int q = r1Idx[e];
+    if (q >= 0) R1bar[q] += c1 * g;
   }
-  atomicAdd(Dbar + row, -g * (w2 * s + 2.0 * (1.0 - w) * w * r1e[e]) / D[row]);
-  int q = r1Idx[e];
-  if (q >= 0) atomicAdd(R1bar + q, g * 2.0 * (1.0 - w) * w);
+  acc = wave_sum(acc);
+  if (lane == 0) Dbar[row] -= acc / D[row];
+}
+
+// one wavefront per middle point k
+__global__ void __launch_bounds__(kBlock) k_adj_ops_mid(int n, const int* __restrict__ off,
+                                                        const int64_t* __restrict__ ent, const double* __restrict__ a,
+                                                        const double* __restrict__ D, const double* __restrict__ Gbar,
+                                                        double w, double* __restrict__ Dbar) {
+  int k = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  int lane = threadIdx.x % kWave;
+  if (k >= n) return;
+  double acc = 0.0;
+  for (int q = off[k] + lane; q < off[k + 1]; q += kWave) acc += Gbar[ent[q]] * a[q];
+  acc = wave_sum(acc);
+  double dk = D[k];
+  if (lane == 0) Dbar[k] -= w * w * acc / (dk * dk);
+}
+
+// one wavefront per spring R1 entry
+__global__ void __launch_bounds__(kBlock) k_adj_ops_r1(int nq, const int* __restrict__ qIdx,
+                                                       const int* __restrict__ off, const int64_t* __restrict__ ent,
+                                                       const int* __restrict__ mid, const int* __restrict__ other,
+                                                       const double* __restrict__ r1Val, const double* __restrict__ D,
+                                                       const double* __restrict__ Gbar, double w,
+                                                       double* __restrict__ R1bar) {
+  int t = (blockIdx.x * blockDim.x + threadIdx.x) / kWave;
+  int lane = threadIdx.x % kWave;
+  if (t >= nq) return;
+  double acc = 0.0;
+  for (int q = off[t] + lane; q < off[t + 1]; q += kWave) acc += Gbar[ent[q]] * r1Val[other[q]] / D[mid[q]];
+  acc = wave_sum(acc);
+  if (lane == 0) R1bar[qIdx[t]] += w * w * acc;
 }
 
 template <int G>
@@ -412,13 +538,13 @@ void launch_solve_taped(const DeviceMesh& m, double omega, double rho, int iters
   }
 }
 
-void launch_adj_sweep(const DeviceMesh& m, double a, double b, const double* uin, const double* uoth, const double* xk,
-                      double* uout, double* Gbar, double* uc, hipStream_t s) {
+void launch_adj_pass(const DeviceMesh& m, const double* W, int K, double* const* u, const double* const* x,
+                     double* Gbar, double* uc, hipStream_t s) {
   switch (m.sellLanes) {
-    case 1: adj_sweep_impl<1>(m, a, b, uin, uoth, xk, uout, Gbar, uc, s); break;
-    case 2: adj_sweep_impl<2>(m, a, b, uin, uoth, xk, uout, Gbar, uc, s); break;
-    case 8: adj_sweep_impl<8>(m, a, b, uin, uoth, xk, uout, Gbar, uc, s); break;
-    default: adj_sweep_impl<4>(m, a, b, uin, uoth, xk, uout, Gbar, uc, s); break;
+    case 1: adj_pass_impl<1>(m, W, K, u, x, Gbar, uc, s); break;
+    case 2: adj_pass_impl<2>(m, W, K, u, x, Gbar, uc, s); break;
+    case 8: adj_pass_impl<8>(m, W, K, u, x, Gbar, uc, s); break;
+    default: adj_pass_impl<4>(m, W, K, u, x, Gbar, uc, s); break;
   }
 }
 
@@ -428,12 +554,26 @@ void launch_adj_first(const DeviceMesh& m, double omega, const double* b, const 
                      m.r1Off, m.r1Col, m.r1Val, m.sysDiag, b, uc, bbar, Dbar, R1bar);
 }
 
-void launch_adj_operators(const DeviceMesh& m, const AdjMesh& a, double omega, const double* Gbar, double* Dbar,
-                          double* R1bar, hipStream_t s) {
-  int64_t nE = m.sellEntries;
-  hipLaunchKernelGGL(k_adj_operators, dim3((unsigned)((nE + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nE, omega,
-                     m.m2RowFlag, m.m2R1, a.sellR1Idx, m.p2Off, m.p2K, m.p2A, a.pIa, a.pIb, m.r1Val, m.sysDiag, Gbar,
-                     Dbar, R1bar);
+void launch_adj_ops_gather(const DeviceMesh& m, const AdjMesh& a, double omega, const double* Gbar, double* Dbar,
+                           double* R1bar, hipStream_t s) {
+  const int rowsPerBlock = kBlock / kWave;
+  const int gr = (m.nSim + rowsPerBlock - 1) / rowsPerBlock;
+#define ROWS(G)                                                                                                    \
+  hipLaunchKernelGGL(k_adj_ops_rows<G>, dim3(gr), dim3(kBlock), 0, s, m.nSim, m.sliceOff, m.sliceW, m.m2R1,        \
+                     a.sellR1Idx, m.p2Off, m.p2K, m.p2A, m.sysDiag, Gbar, omega, Dbar, R1bar)
+  switch (m.sellLanes) {
+    case 1: ROWS(1); break;
+    case 2: ROWS(2); break;
+    case 8: ROWS(8); break;
+    default: ROWS(4); break;
+  }
+#undef ROWS
+  hipLaunchKernelGGL(k_adj_ops_mid, dim3(gr), dim3(kBlock), 0, s, m.nSim, a.midOff, a.midEnt, a.midA, m.sysDiag, Gbar,
+                     omega, Dbar);
+  if (a.nSpringQ > 0)
+    hipLaunchKernelGGL(k_adj_ops_r1, dim3((a.nSpringQ + rowsPerBlock - 1) / rowsPerBlock), dim3(kBlock), 0, s,
+                       a.nSpringQ, a.springQ, a.springOff, a.springEnt, a.springMid, a.springOther, m.r1Val,
+                       m.sysDiag, Gbar, omega, R1bar);
 }
 
 void launch_solve(const DeviceMesh& m, double omega, double rho, int iters, int precision, hipStream_t s) {
