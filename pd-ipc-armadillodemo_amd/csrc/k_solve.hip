@@ -257,7 +257,7 @@ void solve_taped_impl(const DeviceMesh& m, double w, double rho, int iters, doub
 // (M^T = D M D^{-1} because R1 is symmetric, so M^T xbar = D M u): each is the
 // forward SELL SpMV on u,
 //   u_out = a M u_in + b u_oth,   uc[row] += a u_in[row]   (uc: D^{-1}-scaled first-term adjoint);
-// the M^2-entry adjoints are gathered once per solve by k_adj_gbar.
+// the M^2-entry adjoints are gathered once per solve by k_adj_gbar_e.
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_adj_spmv(int n, int nSlices, int ld, double a, double b,
                                                      const int64_t* __restrict__ sliceOff,
@@ -301,63 +301,22 @@ struct GbarArgs {
   double w[kMaxCheb];
   int K;
 };
-template <int G>
-__global__ void __launch_bounds__(kBlock) k_adj_gbar(int n, int nSlices, int ld, const int64_t* __restrict__ sliceOff,
-                                                     const int* __restrict__ sliceW, const int* __restrict__ col,
-                                                     GbarArgs ga, double* __restrict__ Gbar) {
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  int slice = gt / kWave, lane = gt % kWave;
-  if (slice >= nSlices) return;
-  constexpr int S = kWave / G;
-  int row = slice * S + lane / G;
-  if (row >= n) return;  // padded rows (their entries' adjoints are never read)
-  int64_t off = sliceOff[slice] + lane;
-  int width = sliceW[slice];
-  for (int c = 0; c < width; c++) {
-    int64_t e = off + (int64_t)c * kWave;
-    int j = col[e];
-    double g = 0.0;
-    for (int k = 0; k < ga.K; k++) {
-      const double* u = ga.u[k + 1];
-      const double* x = ga.x[k];
-      g += ga.w[k] * (u[row] * x[j] + u[ld + row] * x[ld + j] + u[2 * ld + row] * x[2 * ld + j]);
-    }
-    Gbar[e] += g;
-  }
-}
-
-template <int G, int KK>
-__global__ void __launch_bounds__(kBlock) k_adj_gbar_k(int n, int nSlices, int ld, const int64_t* __restrict__ sliceOff,
-                                                       const int* __restrict__ sliceW, const int* __restrict__ col,
-                                                       GbarArgs ga, double* __restrict__ Gbar) {
-  int gt = blockIdx.x * blockDim.x + threadIdx.x;
-  int slice = gt / kWave, lane = gt % kWave;
-  if (slice >= nSlices) return;
-  constexpr int S = kWave / G;
-  int row = slice * S + lane / G;
-  if (row >= n) return;
-  int64_t off = sliceOff[slice] + lane;
-  int width = sliceW[slice];
-  double ur[KK][3];
-#pragma unroll
-  for (int k = 0; k < KK; k++) {
+// one thread per SELL entry (the row comes from rowFlag): enough threads to
+// hide the latency of the 3K gathered loads each entry makes
+__global__ void __launch_bounds__(kBlock) k_adj_gbar_e(int64_t nE, int ld, const int* __restrict__ rowFlag,
+                                                       const int* __restrict__ col, GbarArgs ga,
+                                                       double* __restrict__ Gbar) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= nE) return;
+  int row = rowFlag[e] & 0x7fffffff;
+  int j = col[e];
+  double g = 0.0;
+  for (int k = 0; k < ga.K; k++) {
     const double* u = ga.u[k + 1];
-    double wk = ga.w[k];
-    ur[k][0] = wk * u[row];
-    ur[k][1] = wk * u[ld + row];
-    ur[k][2] = wk * u[2 * ld + row];
+    const double* x = ga.x[k];
+    g += ga.w[k] * (u[row] * x[j] + u[ld + row] * x[ld + j] + u[2 * ld + row] * x[2 * ld + j]);
   }
-  for (int c = 0; c < width; c++) {
-    int64_t e = off + (int64_t)c * kWave;
-    int j = col[e];
-    double g = 0.0;
-#pragma unroll
-    for (int k = 0; k < KK; k++) {
-      const double* x = ga.x[k];
-      g += ur[k][0] * x[j] + ur[k][1] * x[ld + j] + ur[k][2] * x[2 * ld + j];
-    }
-    Gbar[e] += g;
-  }
+  Gbar[e] += g;
 }
 
 template <int G>
@@ -376,12 +335,9 @@ void adj_pass_impl(const DeviceMesh& m, const double* W, int K, double* const* u
     ga.x[k] = x[k];
     ga.w[k] = W[k];
   }
-  if (K == 15)
-    hipLaunchKernelGGL((k_adj_gbar_k<G, 15>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff,
-                       m.sliceW, m.m2Col, ga, Gbar);
-  else
-    hipLaunchKernelGGL((k_adj_gbar<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff,
-                       m.sliceW, m.m2Col, ga, Gbar);
+  int64_t nE = m.sellEntries;
+  hipLaunchKernelGGL(k_adj_gbar_e, dim3((unsigned)((nE + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nE, m.ld,
+                     m.m2RowFlag, m.m2Col, ga, Gbar);
 }
 
 // Adjoint of the first term c_i = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
