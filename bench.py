@@ -32,7 +32,7 @@ HBM_PEAK_GBS = 8000.0
 # Bytes one FP32 sweep launch moves past L2 at the default workload, from two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
 # a plain 1 GiB copy gives): profiles/k_sweep_config1_hbm_traffic.md.
-SWEEP32_TRAFFIC_BYTES = {(256, 6): 38.69e6}
+SWEEP32_TRAFFIC_BYTES = {}
 
 
 def log(msg):
@@ -141,7 +141,7 @@ def main():
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-seconds", type=float, default=20.0)
-    ap.add_argument("--sell-lanes", type=int, default=4)
+    ap.add_argument("--sell-lanes", type=int, default=8)
     ap.add_argument("--graphs", type=int, default=1, help="replay the local/global loop from a HIP graph")
     ap.add_argument("--keep-static-pairs", action="store_true",
                     help="keep static-static patch pairs (the reference's full candidate set)")
@@ -267,7 +267,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
-            "traffic": SWEEP32_TRAFFIC_BYTES.get((args.n, args.subdiv)),
+            "traffic": SWEEP32_TRAFFIC_BYTES.get((args.n, args.subdiv, args.sell_lanes)),
             "avg_us": us32,
             "bytes_per_launch": bytes32,
             "fp64_sweep": {"avg_us": us64, "achieved": bytes64 / (us64 * 1e-6) / 1e9},

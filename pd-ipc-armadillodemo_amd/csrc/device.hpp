@@ -11,14 +11,22 @@ namespace pdipc {
 
 constexpr int kWave = 64;
 
+// one row (x, y, z, unused) of an FP64 work vector of the Chebyshev sweeps
+struct __align__(32) Row64 {
+  double x, y, z, w;
+};
+
 struct DeviceMesh {
   int nTotal = 0, nSim = 0, nCloth = 0, nTets = 0;
   int ld = 0;  // leading dimension of SoA vectors (>= nTotal, multiple of 64)
   // state and work vectors, SoA [3][ld] doubles
   double *X = nullptr, *Xold = nullptr, *X0 = nullptr, *Dir = nullptr, *Pre = nullptr, *Next = nullptr, *V = nullptr;
   double *initRhs = nullptr, *sysRhs = nullptr, *firstTerm = nullptr, *colRhs = nullptr, *clothA = nullptr, *Xrest = nullptr;
-  // FP32-split work vectors, SoA [3][ld] floats
-  float *dPre = nullptr, *dCur = nullptr, *dNext = nullptr, *r0 = nullptr;
+  // FP64 sweep work vectors, one Row64 per row (ld rows): three iterates and c
+  Row64 *xa0 = nullptr, *xa1 = nullptr, *xa2 = nullptr, *cA = nullptr;
+  // FP32-split work vectors, one float4 (x, y, z, 0) per row (ld rows): a
+  // sweep gathers one 16-byte vector per matrix entry
+  float4 *dPre = nullptr, *dCur = nullptr, *dNext = nullptr, *r0 = nullptr;
   // per point
   double *mass = nullptr, *invMass = nullptr, *gravY = nullptr, *fixK = nullptr, *colMass = nullptr;
   int* fixFlag = nullptr;
@@ -45,6 +53,7 @@ struct DeviceMesh {
   int64_t* sliceOff = nullptr;
   int* sliceW = nullptr;
   int *m2Col = nullptr, *m2RowFlag = nullptr;
+  int16_t* m2Col16 = nullptr;  // col - row per entry, or null when an offset exceeds 16 bits
   double *m2R1 = nullptr, *m2Val = nullptr;
   float* m2ValF = nullptr;
   int64_t* p2Off = nullptr;

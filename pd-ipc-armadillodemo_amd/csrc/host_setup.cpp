@@ -757,6 +757,22 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
         }
       }
     }
+  L.col16ok = true;
+  for (int s = 0; s < L.nSlices && L.col16ok; s++)
+    for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++) {
+      int r = s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes;
+      int d = L.col[e] - r;
+      if (d < -32768 || d > 32767) {
+        L.col16ok = false;
+        break;
+      }
+    }
+  if (L.col16ok) {
+    L.col16.resize(total);
+    for (int s = 0; s < L.nSlices; s++)
+      for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++)
+        L.col16[e] = (int16_t)(L.col[e] - (s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes));
+  }
   L.pOff.assign(total + 1, 0);
   for (int64_t e = 0; e < total; e++) {
     int64_t q = src[e];

@@ -74,20 +74,19 @@ struct FirstTermArgs {  // c = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
   double w;
 };
 
-// MODE 0: x1 = M^2 x0 + c, computing c on the fly (the first term is fused
-// into the first sweep: the G lanes of a row split its R1 entries) and
-// storing it for the later sweeps;  MODE 1: Chebyshev step;  MODE 2:
-// FP32-split start: c as in MODE 0, then r0 = M^2 x0 + c - x0, delta_0 = 0,
+// The first sweep of a solve, reading x0 = X (SoA) and computing c on the fly
+// (the first term is fused: the G lanes of a row split its R1 entries).
+// MODE 0 (FP64): c -> cA, x0 -> xa0, x1 = M^2 x0 + c -> xa1 (and out, SoA, when
+// given);  MODE 2 (FP32-split start): r0 = M^2 x0 + c - x0, delta_0 = 0,
 // delta_1 = r0.
 template <int G, int MODE>
-__global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, double W,
-                                                    const int64_t* __restrict__ sliceOff,
+__global__ void __launch_bounds__(kBlock) k_first64(int n, int nSlices, int ld, const int64_t* __restrict__ sliceOff,
                                                     const int* __restrict__ sliceW, const int* __restrict__ col,
-                                                    const double* __restrict__ val, const double* __restrict__ xcur,
-                                                    const double* __restrict__ xpre, double* __restrict__ cvec,
-                                                    double* __restrict__ xnext, float* __restrict__ dpreF,
-                                                    float* __restrict__ dnextF, float* __restrict__ r0F,
-                                                    FirstTermArgs ft) {
+                                                    const double* __restrict__ val, const double* __restrict__ X,
+                                                    Row64* __restrict__ cA, Row64* __restrict__ xa0,
+                                                    Row64* __restrict__ xa1, double* __restrict__ out,
+                                                    float4* __restrict__ dpreF, float4* __restrict__ dnextF,
+                                                    float4* __restrict__ r0F, FirstTermArgs ft) {
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int slice = gt / kWave, lane = gt % kWave;
   if (slice >= nSlices) return;  // whole wavefront
@@ -102,155 +101,246 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
     int64_t e = off + (int64_t)c * kWave;
     int j = col[e];
     double v = val[e];
-    acc[0] += v * xcur[j];
-    acc[1] += v * xcur[ld + j];
-    acc[2] += v * xcur[2 * ld + j];
+    acc[0] += v * X[j];
+    acc[1] += v * X[ld + j];
+    acc[2] += v * X[2 * ld + j];
   }
   group_reduce<G>(acc);
   double cr[3] = {0.0, 0.0, 0.0};
-  if (MODE != 1) {
-    if (row < n)
-      for (int q = ft.off[row] + sub; q < ft.off[row + 1]; q += G) {
-        int j = ft.col[q];
-        double f = ft.val[q] / ft.D[j];
+  if (row < n)
+    for (int q = ft.off[row] + sub; q < ft.off[row + 1]; q += G) {
+      int j = ft.col[q];
+      double f = ft.val[q] / ft.D[j];
 #pragma unroll
-        for (int k = 0; k < 3; k++) cr[k] += f * ft.b[k * ld + j];
-      }
-    group_reduce<G>(cr);
-  }
+      for (int k = 0; k < 3; k++) cr[k] += f * ft.b[k * ld + j];
+    }
+  group_reduce<G>(cr);
   if (sub != 0 || row >= n) return;
-  double di = MODE != 1 ? ft.D[row] : 1.0;
+  double di = ft.D[row];
+  double c[3], y[3], x0[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     int idx = k * ld + row;
-    double c;
-    if (MODE != 1) {
-      c = (ft.w * ft.w * cr[k] + (2.0 - ft.w) * ft.w * ft.b[idx]) / di;
-      cvec[idx] = c;
-    } else {
-      c = cvec[idx];
+    c[k] = (ft.w * ft.w * cr[k] + (2.0 - ft.w) * ft.w * ft.b[idx]) / di;
+    y[k] = acc[k] + c[k];
+    x0[k] = X[idx];
+  }
+  if (MODE == 0) {
+    cA[row] = Row64{c[0], c[1], c[2], 0.0};
+    xa0[row] = Row64{x0[0], x0[1], x0[2], 0.0};
+    xa1[row] = Row64{y[0], y[1], y[2], 0.0};
+    if (out) {
+      out[row] = y[0];
+      out[ld + row] = y[1];
+      out[2 * ld + row] = y[2];
     }
-    double y = acc[k] + c;
-    if (MODE == 0) {
-      xnext[idx] = y;
-    } else if (MODE == 1) {
-      double p = xpre[idx];
-      xnext[idx] = W * (y - p) + p;
-    } else {
-      float r = (float)(y - xcur[idx]);
-      r0F[idx] = r;
-      dpreF[idx] = 0.0f;
-      dnextF[idx] = r;
-    }
+  } else {
+    float4 rv = make_float4((float)(y[0] - x0[0]), (float)(y[1] - x0[1]), (float)(y[2] - x0[2]), 0.0f);
+    r0F[row] = rv;
+    dpreF[row] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    dnextF[row] = rv;
   }
 }
 
-// FP32 Chebyshev step on the correction; LAST also writes x = x0 + delta
-template <int G, int LAST>
-__global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, float W,
+// FP64 Chebyshev step x_{k+1} = W (M^2 x_k + c - x_{k-1}) + x_{k-1} on Row64
+// vectors (one 32-byte gather per matrix entry, U entries of a lane fetched
+// together); out (SoA), when given, receives x_{k+1} as well.
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, double W,
                                                     const int64_t* __restrict__ sliceOff,
                                                     const int* __restrict__ sliceW, const int* __restrict__ col,
-                                                    const float* __restrict__ val, const float* __restrict__ dcur,
-                                                    const float* __restrict__ dpre, const float* __restrict__ r0,
-                                                    float* __restrict__ dnext, double* __restrict__ X) {
+                                                    const double* __restrict__ val, const Row64* __restrict__ cur,
+                                                    const Row64* __restrict__ pre, const Row64* __restrict__ cA,
+                                                    Row64* __restrict__ next, double* __restrict__ out) {
+  constexpr int U = 4;
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int slice = gt / kWave, lane = gt % kWave;
   if (slice >= nSlices) return;
   constexpr int S = kWave / G;
-  int row = slice * S + lane / G;
+  int row = slice * S + lane / G;  // < ld: slices cover at most ld rows
   int64_t off = sliceOff[slice] + lane;
   int width = sliceW[slice];
-  float acc[3] = {0.f, 0.f, 0.f};
-#pragma unroll 4
-  for (int c = 0; c < width; c++) {
-    int64_t e = off + (int64_t)c * kWave;
-    int j = col[e];
-    float v = val[e];
-    acc[0] += v * dcur[j];
-    acc[1] += v * dcur[ld + j];
-    acc[2] += v * dcur[2 * ld + j];
+  double acc[3] = {0.0, 0.0, 0.0};
+  for (int c0 = 0; c0 < width; c0 += U) {
+    int j[U];
+    double v[U];
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      j[u] = row;
+      v[u] = 0.0;
+      if (c0 + u < width) {
+        int64_t e = off + (int64_t)(c0 + u) * kWave;
+        j[u] = col[e];
+        v[u] = val[e];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      Row64 x = cur[j[u]];
+      acc[0] += v[u] * x.x;
+      acc[1] += v[u] * x.y;
+      acc[2] += v[u] * x.z;
+    }
   }
   group_reduce<G>(acc);
   if ((lane % G) != 0 || row >= n) return;
-#pragma unroll
-  for (int k = 0; k < 3; k++) {
-    int idx = k * ld + row;
-    float p = dpre[idx];
-    float d = W * (acc[k] + r0[idx] - p) + p;
-    if (LAST) X[idx] = X[idx] + (double)d;
-    else dnext[idx] = d;
+  Row64 p = pre[row], c = cA[row];
+  double x0 = W * (acc[0] + c.x - p.x) + p.x;
+  double x1 = W * (acc[1] + c.y - p.y) + p.y;
+  double x2 = W * (acc[2] + c.z - p.z) + p.z;
+  next[row] = Row64{x0, x1, x2, 0.0};
+  if (out) {
+    out[row] = x0;
+    out[ld + row] = x1;
+    out[2 * ld + row] = x2;
   }
 }
 
-__global__ void k_apply_delta(int n, int ld, const float* __restrict__ d, double* __restrict__ X) {
+__global__ void k_row64_to_soa(int n, int ld, const Row64* __restrict__ a, double* __restrict__ X) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
+  Row64 v = a[i];
+  X[i] = v.x;
+  X[ld + i] = v.y;
+  X[2 * ld + i] = v.z;
+}
+
+// FP32 Chebyshev step on the correction; LAST also writes x = x0 + delta.
+// The correction vectors hold one float4 per row, so each matrix entry costs
+// one 16-byte gather; the entries of a lane are fetched U at a time (all
+// column / value loads first, then all gathers) to keep several memory
+// round trips in flight per wavefront.  C16: columns as 16-bit offsets from
+// the row.
+template <int G, int LAST, bool C16>
+__global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, float W,
+                                                    const int64_t* __restrict__ sliceOff,
+                                                    const int* __restrict__ sliceW, const int* __restrict__ col,
+                                                    const int16_t* __restrict__ col16, const float* __restrict__ val,
+                                                    const float4* __restrict__ dcur, const float4* __restrict__ dpre,
+                                                    const float4* __restrict__ r0, float4* __restrict__ dnext,
+                                                    double* __restrict__ X) {
+  constexpr int U = 4;
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int slice = gt / kWave, lane = gt % kWave;
+  if (slice >= nSlices) return;
+  constexpr int S = kWave / G;
+  int row = slice * S + lane / G;  // < ld: slices cover at most ld rows
+  int64_t off = sliceOff[slice] + lane;
+  int width = sliceW[slice];
+  float acc[3] = {0.f, 0.f, 0.f};
+  for (int c0 = 0; c0 < width; c0 += U) {
+    int j[U];
+    float v[U];
 #pragma unroll
-  for (int k = 0; k < 3; k++) X[k * ld + i] += (double)d[k * ld + i];
+    for (int u = 0; u < U; u++) {
+      j[u] = row;
+      v[u] = 0.0f;
+      if (c0 + u < width) {
+        int64_t e = off + (int64_t)(c0 + u) * kWave;
+        j[u] = C16 ? row + (int)col16[e] : col[e];
+        v[u] = val[e];
+      }
+    }
+#pragma unroll
+    for (int u = 0; u < U; u++) {
+      float4 x = dcur[j[u]];
+      acc[0] += v[u] * x.x;
+      acc[1] += v[u] * x.y;
+      acc[2] += v[u] * x.z;
+    }
+  }
+  group_reduce<G>(acc);
+  if ((lane % G) != 0 || row >= n) return;
+  float4 p = dpre[row], r = r0[row];
+  float d0 = W * (acc[0] + r.x - p.x) + p.x;
+  float d1 = W * (acc[1] + r.y - p.y) + p.y;
+  float d2 = W * (acc[2] + r.z - p.z) + p.z;
+  if (LAST) {
+    X[row] += (double)d0;
+    X[ld + row] += (double)d1;
+    X[2 * ld + row] += (double)d2;
+  } else {
+    dnext[row] = make_float4(d0, d1, d2, 0.0f);
+  }
+}
+
+__global__ void k_apply_delta(int n, int ld, const float4* __restrict__ d, double* __restrict__ X) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  float4 v = d[i];
+  X[i] += (double)v.x;
+  X[ld + i] += (double)v.y;
+  X[2 * ld + i] += (double)v.z;
+}
+
+template <int G, int LAST>
+void launch_sweep32(const DeviceMesh& m, int grid, float W, const float4* cur, const float4* pre, float4* next,
+                    hipStream_t s) {
+  if (m.m2Col16)
+    hipLaunchKernelGGL((k_sweep32<G, LAST, true>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W,
+                       m.sliceOff, m.sliceW, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
+  else
+    hipLaunchKernelGGL((k_sweep32<G, LAST, false>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W,
+                       m.sliceOff, m.sliceW, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
+}
+
+// The FP64 solve: x_0 = X, iterates rotate through xa0..xa2, the last one is
+// also written to X (SoA).  outs (when given: the recorded solve of the adjoint's
+// forward pass) receives every iterate x_1 .. x_iters in SoA as well; both
+// forms run the same kernels, so their iterates are bit-identical.
+template <int G>
+void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const* outs, hipStream_t s) {
+  const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
+  const FirstTermArgs ft{m.r1Off, m.r1Col, m.r1Val, m.sysDiag, m.sysRhs, w};
+  double r = rho * rho;
+  double W = 2.0 / (2.0 - r);
+  hipLaunchKernelGGL((k_first64<G, 0>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff,
+                     m.sliceW, m.m2Col, m.m2Val, m.X, m.cA, m.xa0, m.xa1, outs ? outs[1] : nullptr, nullptr,
+                     nullptr, nullptr, ft);
+  Row64 *pre = m.xa0, *cur = m.xa1, *next = m.xa2;
+  if (iters <= 1) {
+    hipLaunchKernelGGL(k_row64_to_soa, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, cur,
+                       m.X);
+    return;
+  }
+  for (int it = 1; it < iters; it++) {
+    double* out = outs ? outs[it + 1] : (it == iters - 1 ? m.X : nullptr);
+    hipLaunchKernelGGL((k_sweep64<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff,
+                       m.sliceW, m.m2Col, m.m2Val, cur, pre, m.cA, next, out);
+    W = 4.0 / (4.0 - r * W);
+    Row64* t = pre; pre = cur; cur = next; next = t;
+  }
+  // the sweeps write the simulated rows only: copy those back (the static rows of X stay)
+  if (outs) launch_axpby(m, 1.0, outs[iters], 0.0, nullptr, m.X, s);
 }
 
 template <int G>
 void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precision, hipStream_t s) {
-  const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
-  const FirstTermArgs ft{m.r1Off, m.r1Col, m.r1Val, m.sysDiag, m.sysRhs, w};
-  const FirstTermArgs none{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0};
-  double r = rho * rho;
-  double W = 2.0 / (2.0 - r);
   if (precision == 0) {
-    double *pre = m.Pre, *cur = m.X, *next = m.Next;
-    hipLaunchKernelGGL((k_sweep64<G, 0>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, 1.0, m.sliceOff,
-                       m.sliceW, m.m2Col, m.m2Val, cur, pre, m.firstTerm, next, nullptr, nullptr, nullptr, ft);
-    double* t = pre; pre = cur; cur = next; next = t;
-    for (int it = 1; it < iters; it++) {
-      hipLaunchKernelGGL((k_sweep64<G, 1>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff,
-                         m.sliceW, m.m2Col, m.m2Val, cur, pre, m.firstTerm, next, nullptr, nullptr, nullptr, none);
-      W = 4.0 / (4.0 - r * W);
-      t = pre; pre = cur; cur = next; next = t;
-    }
-    if (cur != m.X) launch_copy(m.X, cur, 3 * (int64_t)m.ld, s);
-  } else {
-    float *pre = m.dPre, *cur = m.dCur, *next = m.dNext;
-    // delta_0 = 0 -> pre, delta_1 = r0 -> cur
-    hipLaunchKernelGGL((k_sweep64<G, 2>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, 1.0, m.sliceOff,
-                       m.sliceW, m.m2Col, m.m2Val, m.X, nullptr, m.firstTerm, nullptr, pre, cur, m.r0, ft);
-    if (iters <= 1) {
-      hipLaunchKernelGGL(k_apply_delta, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, cur,
-                         m.X);
-      return;
-    }
-    for (int it = 1; it < iters; it++) {
-      if (it == iters - 1)
-        hipLaunchKernelGGL((k_sweep32<G, 1>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, (float)W,
-                           m.sliceOff, m.sliceW, m.m2Col, m.m2ValF, cur, pre, m.r0, next, m.X);
-      else
-        hipLaunchKernelGGL((k_sweep32<G, 0>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, (float)W,
-                           m.sliceOff, m.sliceW, m.m2Col, m.m2ValF, cur, pre, m.r0, next, m.X);
-      W = 4.0 / (4.0 - r * W);
-      float* t = pre; pre = cur; cur = next; next = t;
-    }
+    solve64<G>(m, w, rho, iters, nullptr, s);
+    return;
   }
-}
-
-// The recorded solve of the adjoint's forward pass: the FP64 recurrence with
-// every iterate x_1 .. x_iters written to its own buffer (x_0 = X is kept).
-template <int G>
-void solve_taped_impl(const DeviceMesh& m, double w, double rho, int iters, double* const* outs, hipStream_t s) {
   const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
   const FirstTermArgs ft{m.r1Off, m.r1Col, m.r1Val, m.sysDiag, m.sysRhs, w};
-  const FirstTermArgs none{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0};
   double r = rho * rho;
   double W = 2.0 / (2.0 - r);
-  hipLaunchKernelGGL((k_sweep64<G, 0>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, 1.0, m.sliceOff,
-                     m.sliceW, m.m2Col, m.m2Val, m.X, m.X, m.firstTerm, outs[1], nullptr, nullptr, nullptr, ft);
-  for (int it = 1; it < iters; it++) {
-    const double* pre = it == 1 ? m.X : outs[it - 1];
-    hipLaunchKernelGGL((k_sweep64<G, 1>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff,
-                       m.sliceW, m.m2Col, m.m2Val, outs[it], pre, m.firstTerm, outs[it + 1], nullptr, nullptr, nullptr,
-                       none);
-    W = 4.0 / (4.0 - r * W);
+  float4 *pre = m.dPre, *cur = m.dCur, *next = m.dNext;
+  // delta_0 = 0 -> pre, delta_1 = r0 -> cur
+  hipLaunchKernelGGL((k_first64<G, 2>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff, m.sliceW,
+                     m.m2Col, m.m2Val, m.X, nullptr, nullptr, nullptr, nullptr, pre, cur, m.r0, ft);
+  if (iters <= 1) {
+    hipLaunchKernelGGL(k_apply_delta, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, cur,
+                       m.X);
+    return;
   }
-  // the sweeps write the simulated rows only: copy those back (the static rows of X stay)
-  launch_axpby(m, 1.0, outs[iters], 0.0, nullptr, m.X, s);
+  for (int it = 1; it < iters; it++) {
+    if (it == iters - 1)
+      launch_sweep32<G, 1>(m, grid, (float)W, cur, pre, next, s);
+    else
+      launch_sweep32<G, 0>(m, grid, (float)W, cur, pre, next, s);
+    W = 4.0 / (4.0 - r * W);
+    float4* t = pre; pre = cur; cur = next; next = t;
+  }
 }
 
 // Reverse sweeps of the Chebyshev recurrence in the scaled adjoint u = D^{-1} xbar
@@ -457,12 +547,10 @@ void one_sweep_impl(const DeviceMesh& m, int precision, hipStream_t s) {
   const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
   const double W = 1.9;
   if (precision == 0)
-    hipLaunchKernelGGL((k_sweep64<G, 1>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff,
-                       m.sliceW, m.m2Col, m.m2Val, m.X, m.Pre, m.firstTerm, m.Next, nullptr, nullptr, nullptr,
-                       FirstTermArgs{nullptr, nullptr, nullptr, nullptr, nullptr, 0.0});
+    hipLaunchKernelGGL((k_sweep64<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff,
+                       m.sliceW, m.m2Col, m.m2Val, m.xa1, m.xa0, m.cA, m.xa2, nullptr);
   else
-    hipLaunchKernelGGL((k_sweep32<G, 0>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, (float)W,
-                       m.sliceOff, m.sliceW, m.m2Col, m.m2ValF, m.dCur, m.dPre, m.r0, m.dNext, m.X);
+    launch_sweep32<G, 0>(m, grid, (float)W, m.dCur, m.dPre, m.dNext, s);
 }
 
 }  // namespace
@@ -487,10 +575,10 @@ void launch_operators(const DeviceMesh& m, double omega, int precision, hipStrea
 
 void launch_solve_taped(const DeviceMesh& m, double omega, double rho, int iters, double* const* outs, hipStream_t s) {
   switch (m.sellLanes) {
-    case 1: solve_taped_impl<1>(m, omega, rho, iters, outs, s); break;
-    case 2: solve_taped_impl<2>(m, omega, rho, iters, outs, s); break;
-    case 8: solve_taped_impl<8>(m, omega, rho, iters, outs, s); break;
-    default: solve_taped_impl<4>(m, omega, rho, iters, outs, s); break;
+    case 1: solve64<1>(m, omega, rho, iters, outs, s); break;
+    case 2: solve64<2>(m, omega, rho, iters, outs, s); break;
+    case 8: solve64<8>(m, omega, rho, iters, outs, s); break;
+    default: solve64<4>(m, omega, rho, iters, outs, s); break;
   }
 }
 

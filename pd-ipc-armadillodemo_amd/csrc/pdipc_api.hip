@@ -76,7 +76,7 @@ int upload_all(pdipc_handle* h) {
   HostSetup& hs = h->hs;
   DeviceMesh& m = h->m;
   int lanes = h->prm.sell_lanes;
-  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) lanes = 4;
+  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) lanes = 8;
   build_sell(hs, lanes, h->sell);
   m.nTotal = hs.nTotal;
   m.nSim = hs.nSim;
@@ -89,11 +89,13 @@ int upload_all(pdipc_handle* h) {
 #define AL(field, n) if ((rc = h->alloc(&m.field, n))) return rc
   AL(X, V3); AL(Xold, V3); AL(X0, V3); AL(Dir, V3); AL(Pre, V3); AL(Next, V3); AL(V, V3);
   AL(initRhs, V3); AL(sysRhs, V3); AL(firstTerm, V3); AL(colRhs, V3); AL(clothA, V3); AL(Xrest, V3);
-  AL(dPre, V3); AL(dCur, V3); AL(dNext, V3); AL(r0, V3);
+  AL(dPre, m.ld); AL(dCur, m.ld); AL(dNext, m.ld); AL(r0, m.ld);
+  AL(xa0, m.ld); AL(xa1, m.ld); AL(xa2, m.ld); AL(cA, m.ld);
+  for (Row64* p : {m.xa0, m.xa1, m.xa2, m.cA}) HIPCHK(hipMemset(p, 0, m.ld * sizeof(Row64)));
   for (double* p : {m.X, m.Xold, m.X0, m.Dir, m.Pre, m.Next, m.V, m.initRhs, m.sysRhs, m.firstTerm, m.colRhs, m.clothA,
                     m.Xrest})
     HIPCHK(hipMemset(p, 0, V3 * sizeof(double)));
-  for (float* p : {m.dPre, m.dCur, m.dNext, m.r0}) HIPCHK(hipMemset(p, 0, V3 * sizeof(float)));
+  for (float4* p : {m.dPre, m.dCur, m.dNext, m.r0}) HIPCHK(hipMemset(p, 0, m.ld * sizeof(float4)));
   UP(mass, hs.mass); UP(invMass, hs.invMass); UP(gravY, hs.gravY); UP(fixK, hs.fixK); UP(fixFlag, hs.fixFlag);
   UP(colMass, hs.colMass); UP(elasticDiag, hs.elasticDiag);
   AL(inertialDiag, hs.nSim); AL(sysDiag, hs.nSim); AL(colDiag, hs.nTotal);
@@ -123,6 +125,7 @@ int upload_all(pdipc_handle* h) {
   m.nSlices = L.nSlices;
   m.sellEntries = L.sliceOff.back();
   UP(sliceOff, L.sliceOff); UP(sliceW, L.sliceW); UP(m2Col, L.col); UP(m2RowFlag, L.rowFlag); UP(m2R1, L.r1);
+  if (L.col16ok) UP(m2Col16, L.col16);
   UP(p2Off, L.pOff); UP(p2K, L.pK); UP(p2A, L.pA);
   AL(m2Val, m.sellEntries); AL(m2ValF, m.sellEntries);
   // collision
@@ -378,7 +381,7 @@ void pdipc_default_params(pdipc_params* o) {
   o->omega = 2.0 / 3.0; o->rho = 0.9992; o->sigma_lo = 0.999; o->sigma_hi = 1.001;
   o->ccd_filter = 5e-3; o->toi_scale = 0.98; o->ccd_iters = 10; o->ccd_step = 0.9; o->ccd_gap = 0.1;
   o->static_mass = 1e4; o->inflation = -1.0; o->skip_static_pairs = 0; o->touch_policy = 1;
-  o->precision = PDIPC_FP64; o->sell_lanes = 4; o->use_graphs = 0;
+  o->precision = PDIPC_FP64; o->sell_lanes = 8; o->use_graphs = 0;
 }
 
 const char* pdipc_last_error(void) { return pdipc::g_err.c_str(); }
@@ -726,7 +729,7 @@ int pdipc_time_kernel(pdipc_handle* h, int which, int reps, double* avg_us, doub
   double b = 0;
   switch (which) {
     case 0: b = nnz * 12.0 + D3 * 8.0 * 4.0; break;            // col+val FP64; x_k, x_{k-1}, c, x_{k+1}
-    case 1: b = nnz * 8.0 + D3 * 4.0 * 4.0; break;             // col+val FP32; d_k, d_{k-1}, r0, d_{k+1}
+    case 1: b = nnz * (m.m2Col16 ? 6.0 : 8.0) + D3 * 4.0 * 4.0; break;  // col (16/32-bit)+val FP32; d_k, d_{k-1}, r0, d_{k+1}
     case 2: {
       double slots = (double)h->hs.cNbr.size();
       b = slots * (4 + 8 + 8 + 8) + h->hs.nCloth * 3 * 8.0 * 2 + D3 * 8.0 * 4;  // ring data; a; x, rhs0, rhs
