@@ -16,6 +16,17 @@ struct __align__(32) Row64 {
   double x, y, z, w;
 };
 
+// LBVH node for the stackless traversal: box rounded outward to float (a
+// superset of the exact box, used only to prune), the left child (internal
+// nodes) or the patch id (leaves), and the next node in depth-first preorder
+// once this subtree is done (-1: traversal over).
+struct __align__(16) BvhNode {
+  float lo[3];
+  int child;
+  float hi[3];
+  int skip;
+};
+
 struct DeviceMesh {
   int nTotal = 0, nSim = 0, nCloth = 0, nTets = 0;
   int ld = 0;  // leading dimension of SoA vectors (>= nTotal, multiple of 64)
@@ -76,9 +87,17 @@ struct DeviceMesh {
   int *nodeLeft = nullptr, *nodeRight = nullptr, *nodeParent = nullptr;
   int2* nodeRange = nullptr;       // leaf range [first, last] of each internal node
   double* nodeTable = nullptr;     // sparse table of leaf-box unions, levels x [6][P]
-  double* nodeBox = nullptr;       // SoA [6][2P-1]
+  BvhNode* nodes = nullptr;        // 2P-1: internal nodes 0..P-2, leaf i (sorted position) at P-1+i
   int2* pairs = nullptr;
   int pairCap = 0;
+  // traversal output (lengths cnt*, offsets off*): pass 1 work items
+  // (patch, subtree root) and pairs in one segment per wavefront, pass 2
+  // pairs in one segment per work item (itemCap segments)
+  int2 *segI = nullptr, *segP = nullptr, *segS = nullptr;
+  int segCapI = 0, segCapP = 0;    // entries per pass-1 segment
+  int *cntI = nullptr, *offI = nullptr, *cntP = nullptr, *offP = nullptr, *cntS = nullptr, *offS = nullptr;
+  int2* items = nullptr;           // compacted work items
+  int itemCap = 0;
   int *recCount = nullptr, *recOff = nullptr;
   int4* recs = nullptr;            // EE records store ~a in .x
   int recCap = 0;
@@ -95,8 +114,9 @@ struct DeviceMesh {
   unsigned long long* toiBits = nullptr;
   double* partials = nullptr;      // reduction partials
   double* residual = nullptr;
-  // [0] pairs, [1] records, [2] active records, [3] traversal stack overflow,
-  // [4] TOI argmin (recorded steps), [5] active-list overflow
+  // [0] pass-2 pairs, [1] records, [2] active records, [3] pass-1 segment
+  // overflow (largest length needed), [4] TOI argmin (recorded steps),
+  // [5] active-list overflow, [6] work items, [7] pass-1 pairs
   int* counters = nullptr;
   int nPartials = 0;
 };
@@ -184,8 +204,15 @@ int launch_collision(DeviceMesh& m, int nRecs, double dhat, double kappa, double
 // pair and record totals (host sync) and whether the capacities sufficed.
 void launch_patch_boxes(const DeviceMesh& m, const double* x, const double* dx, double inflation, hipStream_t s);
 void launch_lbvh(DeviceMesh& m, hipStream_t s);
-// appends overlapping patch pairs, count in counters[0]
-void launch_pairs(const DeviceMesh& m, int skipStatic, hipStream_t s);
+// the two traversal passes: top (counts in counters[3], [6], [7]; the host
+// grows the pass-1 segments when counters[3] > 0 and repeats), then the
+// compaction of the work items and the subtree pass (its pair total in
+// counters[0]), then the compaction of both passes' pairs into m.pairs
+int traverse_top_waves(int nPatches);
+int traverse_sub_cap();  // pairs per work item (at most)
+void launch_traverse_top(const DeviceMesh& m, int skipStatic, hipStream_t s);
+void launch_traverse_sub(const DeviceMesh& m, int skipStatic, int nItems, hipStream_t s);
+void launch_pairs_compact(const DeviceMesh& m, int nItems, int nTopPairs, hipStream_t s);
 void launch_rec_count(const DeviceMesh& m, int nPairs, hipStream_t s);
 void launch_rec_fill(const DeviceMesh& m, int nPairs, int nRecs, hipStream_t s);
 void launch_exclusive_scan(int* data_in, int* data_out, int n, int* total_out, int* blockScratch, hipStream_t s);

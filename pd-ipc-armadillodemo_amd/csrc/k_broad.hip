@@ -6,11 +6,15 @@
 //      (lbvh/morton_code.cuh:9-40, bvh.cuh:197-221), a stable LSD radix sort
 //      (8-bit digits; in-block stable ranking with wavefront ballots), the
 //      (code << 32 | index) tie-break, Karras determine_range / find_split
-//      (bvh.cuh:69-163) and a bottom-up refit with arrival flags;
-//   3. one lane per patch traverses the tree and reports every patch q > p
-//      whose box overlaps (checkOverlap, PdIpcGpuFunc.cu:1615-1626, closed boxes):
-//      a count pass, an exclusive scan and a fill pass, so the pair list is
-//      ordered by p and deterministic;
+//      (bvh.cuh:69-163); node boxes from a sparse table of leaf-box unions
+//      (no inter-block synchronisation), stored rounded outward to float with
+//      preorder skip links;
+//   3. every patch, in Morton order, finds each patch q > p whose box overlaps
+//      (checkOverlap, PdIpcGpuFunc.cu:1615-1626, closed boxes, confirmed in
+//      FP64) in a two-pass stackless traversal (top of the tree, then one lane
+//      per overlapping small subtree); every wavefront appends to a segment
+//      of its own and the segments are compacted in wavefront order, so the
+//      pair list (and the record numbering after it) is reproducible;
 //   4. every overlapping patch pair expands to its point-triangle and
 //      edge-edge records in the reference's order (getCcdPotentialICPairsListKernel,
 //      PdIpcGpuFunc.cu:1838-1948), one lane per record.
@@ -26,7 +30,6 @@ constexpr int kScanItems = 4;                       // per thread
 constexpr int kScanTile = kBlock * kScanItems;      // 1024
 constexpr int kRadixItems = 8;
 constexpr int kRadixTile = kBlock * kRadixItems;    // 2048
-constexpr int kStack = 64;
 
 __device__ inline double dmin(double a, double b) { return a < b ? a : b; }
 __device__ inline double dmax(double a, double b) { return a > b ? a : b; }
@@ -393,27 +396,61 @@ __global__ void k_table_level(int P, int half, const double* __restrict__ prev, 
   }
 }
 
+// Node boxes rounded outward to float, child links and preorder skip links.
+// The skip link of node n is the right sibling of the nearest ancestor-or-self
+// that is a left child (-1 when there is none: n ends the root's preorder).
+// Internal nodes over at most kSubLeaves leaves carry kSmall in their child
+// link: the top-level traversal hands such subtrees to the second pass.
+constexpr int kSubLeaves = 32;
+constexpr int kSmall = 1 << 30;
+constexpr int kChildMask = kSmall - 1;
+
 __global__ void k_node_boxes(int P, const int2* __restrict__ range, const double* __restrict__ tab,
-                             double* __restrict__ nbox) {
+                             const int* __restrict__ left, const int* __restrict__ right,
+                             const int* __restrict__ parent, const int* __restrict__ sortedIdx,
+                             BvhNode* __restrict__ nodes) {
   int n = blockIdx.x * blockDim.x + threadIdx.x;
   const int NN = 2 * P - 1;
   if (n >= NN) return;
+  double lo[3], hi[3];
+  BvhNode nd;
   if (n >= P - 1) {
     int i = n - (P - 1);
 #pragma unroll
-    for (int k = 0; k < 6; k++) nbox[(int64_t)k * NN + n] = tab[(int64_t)k * P + i];
-    return;
+    for (int k = 0; k < 3; k++) {
+      lo[k] = tab[(int64_t)k * P + i];
+      hi[k] = tab[(int64_t)(3 + k) * P + i];
+    }
+    nd.child = sortedIdx[i];
+  } else {
+    int2 r = range[n];
+    int len = r.y - r.x + 1;
+    int j = 31 - __clz(len);
+    const double* t = tab + (int64_t)j * 6 * P;
+    int a = r.x, b = r.y - (1 << j) + 1;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      lo[k] = dmin(t[(int64_t)k * P + a], t[(int64_t)k * P + b]);
+      hi[k] = dmax(t[(int64_t)(3 + k) * P + a], t[(int64_t)(3 + k) * P + b]);
+    }
+    nd.child = left[n] | (len <= kSubLeaves ? kSmall : 0);
   }
-  int2 r = range[n];
-  int len = r.y - r.x + 1;
-  int j = 31 - __clz(len);
-  const double* t = tab + (int64_t)j * 6 * P;
-  int a = r.x, b = r.y - (1 << j) + 1;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    nbox[(int64_t)k * NN + n] = dmin(t[(int64_t)k * P + a], t[(int64_t)k * P + b]);
-    nbox[(int64_t)(3 + k) * NN + n] = dmax(t[(int64_t)(3 + k) * P + a], t[(int64_t)(3 + k) * P + b]);
+    nd.lo[k] = __double2float_rd(lo[k]);
+    nd.hi[k] = __double2float_ru(hi[k]);
   }
+  int skip = -1, cur = n;
+  while (cur != 0) {  // node 0 is the root (the only leaf when P == 1)
+    int pa = parent[cur];
+    if (left[pa] == cur) {
+      skip = right[pa];
+      break;
+    }
+    cur = pa;
+  }
+  nd.skip = skip;
+  nodes[n] = nd;
 }
 
 __device__ inline bool overlap(const double* a, const double* b) {
@@ -423,61 +460,161 @@ __device__ inline bool overlap(const double* a, const double* b) {
   return true;
 }
 
-// One lane per patch p walks the tree and appends every patch q > p whose box
-// overlaps; appends are aggregated per wavefront (one atomic per wave per hit
-// round).  Pair order is therefore not fixed; the pair set is.  Static patches
-// follow all simulated ones in patch order, so with skipStatic a static query
-// has nothing to report and exits at once.
-__global__ void __launch_bounds__(kBlock) k_traverse(int P, const double* __restrict__ pbox,
-                                                     const double* __restrict__ nbox, const int* __restrict__ left,
-                                                     const int* __restrict__ right, const int* __restrict__ sortedIdx,
-                                                     const uint8_t* __restrict__ patchStatic, int skipStatic,
-                                                     int* __restrict__ counter, int2* __restrict__ pairs, int cap,
-                                                     int* __restrict__ overflow) {
-  int p = blockIdx.x * blockDim.x + threadIdx.x;
+// Append to a wavefront-owned segment: lanes with `flag` get consecutive slots
+// after `count` (the wavefront's running count, the same in all its lanes that
+// are still looping), in lane order; nothing past the segment's capacity is
+// written, but the count keeps counting (the host sees the overflow, grows the
+// segments and repeats).  No atomics: a wavefront's output depends only on its
+// own inputs, so the order is reproducible.
+__device__ inline void seg_append(bool flag, int2 v, int& count, int2* __restrict__ seg, int cap) {
+  unsigned long long m = __ballot(flag);
+  if (!m) return;
   int lane = threadIdx.x % kWave;
   const unsigned long long lt = (lane == 0) ? 0ull : ((~0ull) >> (64 - lane));
-  bool alive = p < P && !(skipStatic && patchStatic[p]);
-  if (!alive) return;
-  const int NN = 2 * P - 1;
-  double q[6];
+  if (flag) {
+    int pos = count + (int)__popcll(m & lt);
+    if (pos < cap) seg[pos] = v;
+  }
+  count += (int)__popcll(m);
+}
+
+struct TraverseArgs {
+  int P;
+  const double* pbox;
+  const BvhNode* nodes;
+  const uint8_t* patchStatic;
+  int skipStatic;
+};
+
+__device__ inline void load_query(const TraverseArgs& a, int p, double q[6], float qlo[3], float qhi[3]) {
 #pragma unroll
-  for (int k = 0; k < 6; k++) q[k] = pbox[(int64_t)k * P + p];
-  int stack[kStack];
-  int sp = 0;
-  stack[sp++] = 0;  // root: node 0 (the single leaf when P == 1)
-  while (sp > 0) {
-    int node = stack[--sp];
-    double b[6];
+  for (int k = 0; k < 6; k++) q[k] = a.pbox[(int64_t)k * a.P + p];
 #pragma unroll
-    for (int k = 0; k < 6; k++) b[k] = nbox[(int64_t)k * NN + node];
-    bool hit = false;
+  for (int k = 0; k < 3; k++) {
+    qlo[k] = __double2float_rd(q[k]);
+    qhi[k] = __double2float_ru(q[3 + k]);
+  }
+}
+
+__device__ inline bool overlap_f(const float qlo[3], const float qhi[3], const BvhNode& nd) {
+  return !(qhi[0] < nd.lo[0] || nd.hi[0] < qlo[0] || qhi[1] < nd.lo[1] || nd.hi[1] < qlo[1] || qhi[2] < nd.lo[2] ||
+           nd.hi[2] < qlo[2]);
+}
+
+// a leaf reached with an overlapping float box: the exact FP64 test and the
+// emission rule -- every pair p < q once: by p, or, when static queries are
+// skipped, by the simulated side of a static/simulated pair
+__device__ inline bool leaf_hit(const TraverseArgs& a, int p, const double q[6], bool ov, int obj) {
+  if (!ov || !(obj > p || (a.skipStatic && obj < p && a.patchStatic[obj]))) return false;
+  double b[6];
+#pragma unroll
+  for (int k = 0; k < 6; k++) b[k] = a.pbox[(int64_t)k * a.P + obj];
+  return overlap(q, b);
+}
+
+// Two-pass stackless traversal (latency-bound walks are kept short and
+// numerous).  Pass 1: lane t takes the patch at sorted (Morton) position t,
+// walks the top of the tree and, instead of descending into a subtree of at
+// most kSubLeaves leaves whose float box overlaps, emits the work item
+// (patch, subtree root).  Pass 2: one lane per item walks that subtree, which
+// yields at most kSubLeaves pairs.  Internal nodes are pruned on their
+// outward-rounded float boxes; a leaf is confirmed on the exact FP64 boxes
+// (closed-box test), so the pair set is exactly that of the FP64 boxes.
+// Static patches follow all simulated ones in patch order, so with skipStatic
+// a static query has nothing to report.  Each wavefront writes segments of its
+// own; their lengths go to cntI / cntP (one entry per wavefront).
+__global__ void __launch_bounds__(kBlock) k_traverse_top(TraverseArgs a, const int* __restrict__ sortedIdx,
+                                                         int2* __restrict__ segI, int capI, int* __restrict__ cntI,
+                                                         int2* __restrict__ segP, int capP, int* __restrict__ cntP,
+                                                         int* __restrict__ overflow) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  int w = t / kWave;
+  int p = t < a.P ? sortedIdx[t] : 0;
+  bool alive = t < a.P && !(a.skipStatic && a.patchStatic[p]);
+  double q[6] = {0, 0, 0, 0, 0, 0};
+  float qlo[3] = {0, 0, 0}, qhi[3] = {0, 0, 0};
+  if (alive) load_query(a, p, q, qlo, qhi);
+  int nI = 0, nP = 0;
+  int2* sI = segI + (int64_t)w * capI;
+  int2* sP = segP + (int64_t)w * capP;
+  int node = alive ? 0 : -1;
+  while (node >= 0) {
+    BvhNode nd = a.nodes[node];
+    bool ov = overlap_f(qlo, qhi, nd);
+    bool hit = false, item = false;
     int obj = -1;
-    if (overlap(q, b)) {
-      if (node >= P - 1) {
-        obj = sortedIdx[node - (P - 1)];
-        // every pair p < q is emitted once: by p, or -- when static queries are
-        // skipped -- by the simulated side of a static/simulated pair
-        hit = obj > p || (skipStatic && obj < p && patchStatic[obj]);
-      } else if (sp + 2 <= kStack) {
-        stack[sp++] = right[node];  // right first: the left subtree is visited first
-        stack[sp++] = left[node];
-      } else {
-        atomicOr(overflow, 1);  // reported to the host as an error, never silently dropped
-      }
+    if (node >= a.P - 1) {
+      obj = nd.child;
+      hit = leaf_hit(a, p, q, ov, obj);
+      node = nd.skip;
+    } else if (ov && (nd.child & kSmall)) {
+      item = true;
+      obj = node;
+      node = nd.skip;
+    } else {
+      node = ov ? (nd.child & kChildMask) : nd.skip;
     }
-    unsigned long long m = __ballot(hit);
-    if (m) {
-      int leader = __ffsll((long long)m) - 1;
-      int base = 0;
-      if (lane == leader) base = atomicAdd(counter, (int)__popcll(m));
-      base = __shfl(base, leader, kWave);
-      if (hit) {
-        int pos = base + (int)__popcll(m & lt);
-        if (pos < cap) pairs[pos] = make_int2(min(p, obj), max(p, obj));
-      }
+    seg_append(hit, make_int2(min(p, obj), max(p, obj)), nP, sP, capP);
+    seg_append(item, make_int2(p, obj), nI, sI, capI);
+  }
+  // the running counts grow monotonically and the last lanes to leave the loop
+  // hold the totals: take the maximum over the wavefront
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    nI = max(nI, __shfl_xor(nI, o, kWave));
+    nP = max(nP, __shfl_xor(nP, o, kWave));
+  }
+  if ((threadIdx.x % kWave) == 0) {
+    cntI[w] = nI;
+    cntP[w] = nP;
+    if (nI > capI || nP > capP) atomicMax(overflow, max(nI, nP));
+  }
+}
+
+// one lane per item; item i's pairs (at most kSubLeaves) go to its own segment
+// seg[i * kSubLeaves ...], so the pair order follows the item order
+__global__ void __launch_bounds__(kBlock) k_traverse_sub(TraverseArgs a, int nItems, const int2* __restrict__ items,
+                                                         int2* __restrict__ seg, int* __restrict__ cnt) {
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= nItems) return;
+  int2 it = items[t];
+  int p = it.x;
+  double q[6];
+  float qlo[3], qhi[3];
+  load_query(a, p, q, qlo, qhi);
+  int n = 0;
+  int2* sg = seg + (int64_t)t * kSubLeaves;
+  int node = it.y;
+  const int stop = a.nodes[node].skip;
+  while (node != stop) {
+    BvhNode nd = a.nodes[node];
+    bool ov = overlap_f(qlo, qhi, nd);
+    if (node >= a.P - 1) {
+      int obj = nd.child;
+      if (leaf_hit(a, p, q, ov, obj) && n < kSubLeaves) sg[n++] = make_int2(min(p, obj), max(p, obj));
+      node = nd.skip;
+    } else {
+      node = ov ? (nd.child & kChildMask) : nd.skip;
     }
   }
+  cnt[t] = n;
+}
+
+// entry k of item i's segment goes to out[off[i] + k] when k < cnt[i]
+__global__ void k_compact_items(int nItems, const int2* __restrict__ seg, const int* __restrict__ cnt,
+                                const int* __restrict__ off, int2* __restrict__ out) {
+  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= (int64_t)nItems * kSubLeaves) return;
+  int i = (int)(e / kSubLeaves), k = (int)(e % kSubLeaves);
+  if (k < cnt[i]) out[off[i] + k] = seg[e];
+}
+
+// segment w's first min(cnt[w], cap) entries go to out[off[w] ...] (off: exclusive scan of cnt)
+__global__ void k_compact(int nw, int cap, const int2* __restrict__ seg, const int* __restrict__ cnt,
+                          const int* __restrict__ off, int2* __restrict__ out) {
+  int w = blockIdx.x;
+  if (w >= nw) return;
+  int n = min(cnt[w], cap), o = off[w];
+  for (int k = threadIdx.x; k < n; k += blockDim.x) out[o + k] = seg[(int64_t)w * cap + k];
 }
 
 __global__ void k_rec_count(int Q, const int2* __restrict__ pairs, const int* __restrict__ ppOff,
@@ -587,15 +724,43 @@ void launch_lbvh(DeviceMesh& m, hipStream_t s) {
     hipLaunchKernelGGL(k_table_level, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, 1 << (j - 1),
                        m.nodeTable + (int64_t)(j - 1) * 6 * P, m.nodeTable + (int64_t)j * 6 * P);
   hipLaunchKernelGGL(k_node_boxes, dim3((2 * P - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, m.nodeRange,
-                     m.nodeTable, m.nodeBox);
+                     m.nodeTable, m.nodeLeft, m.nodeRight, m.nodeParent, iin, m.nodes);
 }
 
-void launch_pairs(const DeviceMesh& m, int skipStatic, hipStream_t s) {
+int traverse_top_waves(int P) { return ((P + kBlock - 1) / kBlock) * (kBlock / kWave); }
+int traverse_sub_cap() { return kSubLeaves; }
+
+void launch_traverse_top(const DeviceMesh& m, int skipStatic, hipStream_t s) {
   int P = m.nPatches;
   if (P <= 0) return;
-  hipLaunchKernelGGL(k_traverse, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, m.patchBox, m.nodeBox,
-                     m.nodeLeft, m.nodeRight, m.idxA, m.patchStatic, skipStatic, m.counters + 0, m.pairs, m.pairCap,
-                     m.counters + 3);
+  const TraverseArgs a{P, m.patchBox, m.nodes, m.patchStatic, skipStatic};
+  int nw = traverse_top_waves(P);
+  hipLaunchKernelGGL(k_traverse_top, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, m.idxA, m.segI, m.segCapI,
+                     m.cntI, m.segP, m.segCapP, m.cntP, m.counters + 3);
+  launch_exclusive_scan(m.cntI, m.offI, nw, m.counters + 6, m.scanBlock, s);
+  launch_exclusive_scan(m.cntP, m.offP, nw, m.counters + 7, m.scanBlock, s);
+}
+
+void launch_traverse_sub(const DeviceMesh& m, int skipStatic, int nItems, hipStream_t s) {
+  if (nItems <= 0) return;
+  int P = m.nPatches;
+  int nw = traverse_top_waves(P);
+  hipLaunchKernelGGL(k_compact, dim3(nw), dim3(kBlock), 0, s, nw, m.segCapI, m.segI, m.cntI, m.offI, m.items);
+  const TraverseArgs a{P, m.patchBox, m.nodes, m.patchStatic, skipStatic};
+  hipLaunchKernelGGL(k_traverse_sub, dim3((nItems + kBlock - 1) / kBlock), dim3(kBlock), 0, s, a, nItems, m.items,
+                     m.segS, m.cntS);
+  launch_exclusive_scan(m.cntS, m.offS, nItems, m.counters + 0, m.scanBlock, s);
+}
+
+void launch_pairs_compact(const DeviceMesh& m, int nItems, int nTopPairs, hipStream_t s) {
+  int nw = traverse_top_waves(m.nPatches);
+  if (nTopPairs > 0)
+    hipLaunchKernelGGL(k_compact, dim3(nw), dim3(kBlock), 0, s, nw, m.segCapP, m.segP, m.cntP, m.offP, m.pairs);
+  if (nItems > 0) {
+    int64_t n = (int64_t)nItems * kSubLeaves;
+    hipLaunchKernelGGL(k_compact_items, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nItems,
+                       m.segS, m.cntS, m.offS, m.pairs + nTopPairs);
+  }
 }
 
 void launch_rec_count(const DeviceMesh& m, int nPairs, hipStream_t s) {

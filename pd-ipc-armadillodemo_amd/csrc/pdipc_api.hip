@@ -148,17 +148,27 @@ int upload_all(pdipc_handle* h) {
   int nbRadix = (P + 2047) / 2048;
   AL(radixHist, 256 * (size_t)nbRadix); AL(radixBlockSums, (256 * (size_t)nbRadix + 1023) / 1024 + 2);
   AL(nodeLeft, P); AL(nodeRight, P); AL(nodeParent, 2 * (size_t)P); AL(nodeRange, P);
-  AL(nodeBox, 6 * (2 * (size_t)P));
+  AL(nodes, 2 * (size_t)P);
   {
     int levels = 1;
     while ((1 << levels) <= P) levels++;
     AL(nodeTable, (size_t)levels * 6 * P);
   }
+  {
+    int nw = traverse_top_waves(P);
+    m.segCapI = 256;
+    m.segCapP = 256;
+    AL(segI, (size_t)nw * m.segCapI); AL(segP, (size_t)nw * m.segCapP);
+    AL(cntI, nw); AL(offI, nw); AL(cntP, nw); AL(offP, nw);
+    m.itemCap = std::max(1024, 2 * P);
+    AL(items, m.itemCap);
+    AL(segS, (size_t)m.itemCap * traverse_sub_cap()); AL(cntS, m.itemCap); AL(offS, m.itemCap);
+  }
   m.pairCap = std::max(1024, 8 * P);
   AL(pairs, m.pairCap); AL(recCount, m.pairCap); AL(recOff, m.pairCap);
   m.recCap = std::max(4096, 32 * P);
   AL(recs, m.recCap);
-  m.scanBlockCap = (std::max(P, m.pairCap) + 1023) / 1024 + 2;
+  m.scanBlockCap = (std::max(std::max(P, m.pairCap), m.itemCap) + 1023) / 1024 + 2;
   AL(scanBlock, m.scanBlockCap);
   m.actCap = 4096;
   AL(actRec, m.actCap); AL(actW, m.actCap); AL(actT, 12 * (size_t)m.actCap);
@@ -203,13 +213,47 @@ int broad_phase(pdipc_handle* h, const double* x, const double* dx) {
   HIPCHK(hipMemsetAsync(m.counters, 0, 8 * sizeof(int), s));
   launch_patch_boxes(m, x, dx, infl, s);
   launch_lbvh(m, s);
-  launch_pairs(m, h->prm.skip_static_pairs, s);
-  int rc = check_launch(h);
-  if (rc) return rc;
+  int rc;
+  int nItems = 0, nTop = 0;
+  for (int attempt = 0;; attempt++) {
+    launch_traverse_top(m, h->prm.skip_static_pairs, s);
+    if ((rc = check_launch(h))) return rc;
+    if ((rc = read_scalars(h))) return rc;
+    int need = h->hCounters()[3];
+    nItems = h->hCounters()[6];
+    nTop = h->hCounters()[7];
+    if (need <= 0) break;
+    if (attempt >= 2) return fail(PDIPC_E_CAPACITY, "traversal segments did not fit after regrowth");
+    int cap = need + need / 2 + 64;  // grow the pass-1 segments and search again
+    size_t nw = (size_t)traverse_top_waves(m.nPatches);
+    h->release(m.segI);
+    h->release(m.segP);
+    m.segI = m.segP = nullptr;
+    if ((rc = h->alloc(&m.segI, nw * cap)) || (rc = h->alloc(&m.segP, nw * cap))) return rc;
+    m.segCapI = m.segCapP = cap;
+    HIPCHK(hipMemsetAsync(m.counters + 3, 0, sizeof(int), s));
+  }
+  if (nItems > m.itemCap) {
+    int cap = nItems + nItems / 2 + 1024;
+    for (void* p : {(void*)m.items, (void*)m.segS, (void*)m.cntS, (void*)m.offS}) h->release(p);
+    if ((rc = h->alloc(&m.items, cap)) || (rc = h->alloc(&m.segS, (size_t)cap * traverse_sub_cap())) ||
+        (rc = h->alloc(&m.cntS, cap)) || (rc = h->alloc(&m.offS, cap)))
+      return rc;
+    m.itemCap = cap;
+    int sb = (cap + 1023) / 1024 + 2;
+    if (sb > m.scanBlockCap) {
+      h->release(m.scanBlock);
+      if ((rc = h->alloc(&m.scanBlock, sb))) return rc;
+      m.scanBlockCap = sb;
+    }
+  }
+  launch_traverse_sub(m, h->prm.skip_static_pairs, nItems, s);
+  if ((rc = check_launch(h))) return rc;
   if ((rc = read_scalars(h))) return rc;
-  int Q = h->hCounters()[0];
-  if (h->hCounters()[3]) return fail(PDIPC_E_CAPACITY, "LBVH traversal stack overflow");
-  if (Q > m.pairCap) {  // grow and search again
+  int64_t Q64 = (int64_t)nTop + (nItems > 0 ? h->hCounters()[0] : 0);
+  if (Q64 > INT32_MAX / 2) return fail(PDIPC_E_CAPACITY, "patch pair count overflows int32");
+  int Q = (int)Q64;
+  if (Q > m.pairCap) {
     int cap = Q + Q / 2 + 1024;
     h->release(m.pairs);
     if ((rc = h->alloc(&m.pairs, cap))) return rc;
@@ -224,9 +268,8 @@ int broad_phase(pdipc_handle* h, const double* x, const double* dx) {
       m.scanBlockCap = sb;
     }
     m.pairCap = cap;
-    HIPCHK(hipMemsetAsync(m.counters, 0, sizeof(int), s));
-    launch_pairs(m, h->prm.skip_static_pairs, s);
   }
+  launch_pairs_compact(m, nItems, nTop, s);
   launch_rec_count(m, Q, s);
   if ((rc = check_launch(h))) return rc;
   if ((rc = read_scalars(h))) return rc;
@@ -720,7 +763,10 @@ int pdipc_time_kernel(pdipc_handle* h, int which, int reps, double* avg_us, doub
       case 0: launch_one_sweep(m, 0, s); break;
       case 1: launch_one_sweep(m, 1, s); break;
       case 2: launch_local(m, p.precision, p.sigma_lo, p.sigma_hi, 0, s); break;
-      case 3: launch_ccd(m, h->lastRecs, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap, p.touch_policy, s); break;
+      case 3:
+        launch_toi_init(m, s);
+        launch_ccd(m, h->lastRecs, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap, p.touch_policy, s);
+        break;
       default: launch_operators(m, p.omega, p.precision, s); break;
     }
   };
