@@ -98,6 +98,7 @@ struct DeviceMesh {
   int *cntI = nullptr, *offI = nullptr, *cntP = nullptr, *offP = nullptr, *cntS = nullptr, *offS = nullptr;
   int2* items = nullptr;           // compacted work items
   int itemCap = 0;
+  // 2 * pairCap each: point-triangle counts / offsets per pair, then edge-edge ones
   int *recCount = nullptr, *recOff = nullptr;
   int4* recs = nullptr;            // EE records store ~a in .x
   int recCap = 0;
@@ -109,14 +110,16 @@ struct DeviceMesh {
   double *actW = nullptr, *actT = nullptr;  // weight, 4 targets x 3 per active record
   int *ptCount = nullptr, *ptOff = nullptr, *ptCursor = nullptr, *ptScan = nullptr;  // nSim + 1
   unsigned long long* entKey = nullptr;  // 4 * actCap: (record << 2 | corner)
-  int* entA = nullptr;
+  int *entA = nullptr, *entPt = nullptr;  // 4 * actCap: active-list index, point
+  unsigned long long* entSorted = nullptr;  // 4 * actCap: (active index << 2 | corner), key order per point
   // scalars
   unsigned long long* toiBits = nullptr;
   double* partials = nullptr;      // reduction partials
   double* residual = nullptr;
   // [0] pass-2 pairs, [1] records, [2] active records, [3] pass-1 segment
   // overflow (largest length needed), [4] TOI argmin (recorded steps),
-  // [5] active-list overflow, [6] work items, [7] pass-1 pairs
+  // [5] active-list overflow, [6] work items, [7] pass-1 pairs, [8] edge-edge
+  // records ([1]: point-triangle records)
   int* counters = nullptr;
   int nPartials = 0;
 };
@@ -214,7 +217,7 @@ void launch_traverse_top(const DeviceMesh& m, int skipStatic, hipStream_t s);
 void launch_traverse_sub(const DeviceMesh& m, int skipStatic, int nItems, hipStream_t s);
 void launch_pairs_compact(const DeviceMesh& m, int nItems, int nTopPairs, hipStream_t s);
 void launch_rec_count(const DeviceMesh& m, int nPairs, hipStream_t s);
-void launch_rec_fill(const DeviceMesh& m, int nPairs, int nRecs, hipStream_t s);
+void launch_rec_fill(const DeviceMesh& m, int nPairs, int nPT, int nRecs, hipStream_t s);
 void launch_exclusive_scan(int* data_in, int* data_out, int n, int* total_out, int* blockScratch, hipStream_t s);
 
 }  // namespace pdipc

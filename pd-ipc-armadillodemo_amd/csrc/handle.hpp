@@ -44,7 +44,7 @@ struct pdipc_handle {
   hipGraphExec_t lgGraph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   std::vector<void*> allocs;
   // host mirrors
-  void* scalHost = nullptr;  // pinned: residual, toiBits, counters[8]
+  void* scalHost = nullptr;  // pinned: residual, toiBits, counters[10]
   void* scalDev = nullptr;
   std::vector<double> stage;
   pdipc_counters cnt;

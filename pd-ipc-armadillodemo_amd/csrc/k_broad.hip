@@ -16,8 +16,10 @@
 //      of its own and the segments are compacted in wavefront order, so the
 //      pair list (and the record numbering after it) is reproducible;
 //   4. every overlapping patch pair expands to its point-triangle and
-//      edge-edge records in the reference's order (getCcdPotentialICPairsListKernel,
-//      PdIpcGpuFunc.cu:1838-1948), one lane per record.
+//      edge-edge records (getCcdPotentialICPairsListKernel,
+//      PdIpcGpuFunc.cu:1838-1948), one lane per record: all point-triangle
+//      records first (pair order, the reference's order within a pair), then
+//      all edge-edge records.
 // The pair set equals the reference heap-BVH's (both are "all overlapping
 // pairs"); the heap topology itself is not reproduced.
 #include "device.hpp"
@@ -617,14 +619,20 @@ __global__ void k_compact(int nw, int cap, const int2* __restrict__ seg, const i
   for (int k = threadIdx.x; k < n; k += blockDim.x) out[o + k] = seg[(int64_t)w * cap + k];
 }
 
+// Per pair: point-triangle records (points of A x faces of B, then points of
+// B x faces of A) into cntPT, edge-edge records (edges of A x edges of B)
+// into cntEE.  All point-triangle records precede all edge-edge ones, so the
+// wavefronts of the narrow-phase kernels see one record type.
 __global__ void k_rec_count(int Q, const int2* __restrict__ pairs, const int* __restrict__ ppOff,
-                            const int* __restrict__ pfOff, const int* __restrict__ peOff, int* __restrict__ count) {
+                            const int* __restrict__ pfOff, const int* __restrict__ peOff, int* __restrict__ cntPT,
+                            int* __restrict__ cntEE) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= Q) return;
   int2 pr = pairs[i];
   int pa = ppOff[pr.x + 1] - ppOff[pr.x], fa = pfOff[pr.x + 1] - pfOff[pr.x], ea = peOff[pr.x + 1] - peOff[pr.x];
   int pb = ppOff[pr.y + 1] - ppOff[pr.y], fb = pfOff[pr.y + 1] - pfOff[pr.y], eb = peOff[pr.y + 1] - peOff[pr.y];
-  count[i] = pa * fb + pb * fa + ea * eb;
+  cntPT[i] = pa * fb + pb * fa;
+  cntEE[i] = ea * eb;
 }
 
 __device__ inline int last_le(const int* __restrict__ off, int lo, int hi, int r) {
@@ -636,41 +644,57 @@ __device__ inline int last_le(const int* __restrict__ off, int lo, int hi, int r
   return lo;
 }
 
-__global__ void __launch_bounds__(kBlock) k_rec_fill(int Q, int K, const int2* __restrict__ pairs,
-                                                     const int* __restrict__ recOff, const int* __restrict__ ppOff,
-                                                     const int* __restrict__ ppList, const int* __restrict__ pfOff,
-                                                     const int* __restrict__ pfList, const int* __restrict__ peOff,
-                                                     const int* __restrict__ peList, const int3* __restrict__ faces,
-                                                     const int2* __restrict__ edges, int4* __restrict__ recs) {
+// One lane per record: records [0, nPT) are point-triangle ones, the pair
+// found in offPT; records [nPT, K) edge-edge ones, the pair found in offEE.
+// Within a pair the order is the reference's (getCcdPotentialICPairsListKernel,
+// PdIpcGpuFunc.cu:1838-1948) restricted to the record type.  A block whose
+// records lie in one segment and span at most kBlock pairs searches a copy
+// of its offsets in LDS.
+__global__ void __launch_bounds__(kBlock) k_rec_fill(int Q, int nPT, int K, const int2* __restrict__ pairs,
+                                                     const int* __restrict__ offPT, const int* __restrict__ offEE,
+                                                     const int* __restrict__ ppOff, const int* __restrict__ ppList,
+                                                     const int* __restrict__ pfOff, const int* __restrict__ pfList,
+                                                     const int* __restrict__ peOff, const int* __restrict__ peList,
+                                                     const int3* __restrict__ faces, const int2* __restrict__ edges,
+                                                     int4* __restrict__ recs) {
   __shared__ int sRange[2];
   __shared__ int sOff[kBlock + 1];
   int r0 = blockIdx.x * kBlock;
   int r1 = min(r0 + kBlock, K) - 1;
-  if (threadIdx.x == 0) sRange[0] = last_le(recOff, 0, Q - 1, r0);
-  if (threadIdx.x == 1) sRange[1] = last_le(recOff, 0, Q - 1, r1);
+  bool ee0 = r0 >= nPT, oneSeg = ee0 == (r1 >= nPT);
+  const int* off0 = ee0 ? offEE : offPT;
+  int base0 = ee0 ? nPT : 0;
+  if (oneSeg && threadIdx.x == 0) sRange[0] = last_le(off0, 0, Q - 1, r0 - base0);
+  if (oneSeg && threadIdx.x == 1) sRange[1] = last_le(off0, 0, Q - 1, r1 - base0);
   __syncthreads();
-  int lo = sRange[0], hi = sRange[1];
-  bool local = hi - lo + 1 <= kBlock;
-  if (local && threadIdx.x <= hi - lo) sOff[threadIdx.x] = recOff[lo + threadIdx.x];
+  int lo = oneSeg ? sRange[0] : 0, hi = oneSeg ? sRange[1] : 0;
+  bool local = oneSeg && hi - lo + 1 <= kBlock;
+  if (local && threadIdx.x <= hi - lo) sOff[threadIdx.x] = off0[lo + threadIdx.x];
   __syncthreads();
   int r = r0 + threadIdx.x;
   if (r >= K) return;
-  int pi = local ? lo + last_le(sOff, 0, hi - lo, r) : last_le(recOff, lo, hi, r);
+  bool ee = r >= nPT;
+  const int* off = ee ? offEE : offPT;
+  int rr = ee ? r - nPT : r;
+  int pi = local ? lo + last_le(sOff, 0, hi - lo, rr) : last_le(off, 0, Q - 1, rr);
   int2 pr = pairs[pi];
-  int id = r - recOff[pi];
+  int id = rr - off[pi];
   int A = pr.x, B = pr.y;
-  int pa = ppOff[A + 1] - ppOff[A], fa = pfOff[A + 1] - pfOff[A];
-  int pb = ppOff[B + 1] - ppOff[B], fb = pfOff[B + 1] - pfOff[B], eb = peOff[B + 1] - peOff[B];
-  int VaFb = pa * fb, VbFa = pb * fa;
   int4 out;
-  if (id < VaFb) {
-    int3 f = faces[pfList[pfOff[B] + id % fb]];
-    out = make_int4(ppList[ppOff[A] + id / fb], f.x, f.y, f.z);
-  } else if ((id -= VaFb) < VbFa) {
-    int3 f = faces[pfList[pfOff[A] + id % fa]];
-    out = make_int4(ppList[ppOff[B] + id / fa], f.x, f.y, f.z);
+  if (!ee) {
+    int pa = ppOff[A + 1] - ppOff[A], fa = pfOff[A + 1] - pfOff[A];
+    int fb = pfOff[B + 1] - pfOff[B];
+    int VaFb = pa * fb;
+    if (id < VaFb) {
+      int3 f = faces[pfList[pfOff[B] + id % fb]];
+      out = make_int4(ppList[ppOff[A] + id / fb], f.x, f.y, f.z);
+    } else {
+      id -= VaFb;
+      int3 f = faces[pfList[pfOff[A] + id % fa]];
+      out = make_int4(ppList[ppOff[B] + id / fa], f.x, f.y, f.z);
+    }
   } else {
-    id -= VbFa;
+    int eb = peOff[B + 1] - peOff[B];
     int2 e1 = edges[peList[peOff[A] + id / eb]];
     int2 e2 = edges[peList[peOff[B] + id % eb]];
     out = make_int4(~e1.x, e1.y, e2.x, e2.y);
@@ -766,14 +790,16 @@ void launch_pairs_compact(const DeviceMesh& m, int nItems, int nTopPairs, hipStr
 void launch_rec_count(const DeviceMesh& m, int nPairs, hipStream_t s) {
   if (nPairs <= 0) return;
   hipLaunchKernelGGL(k_rec_count, dim3((nPairs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nPairs, m.pairs, m.ppOff,
-                     m.pfOff, m.peOff, m.recCount);
+                     m.pfOff, m.peOff, m.recCount, m.recCount + m.pairCap);
   launch_exclusive_scan(m.recCount, m.recOff, nPairs, m.counters + 1, m.scanBlock, s);
+  launch_exclusive_scan(m.recCount + m.pairCap, m.recOff + m.pairCap, nPairs, m.counters + 8, m.scanBlock, s);
 }
 
-void launch_rec_fill(const DeviceMesh& m, int nPairs, int nRecs, hipStream_t s) {
+void launch_rec_fill(const DeviceMesh& m, int nPairs, int nPT, int nRecs, hipStream_t s) {
   if (nRecs <= 0 || nPairs <= 0) return;
-  hipLaunchKernelGGL(k_rec_fill, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nPairs, nRecs, m.pairs,
-                     m.recOff, m.ppOff, m.ppList, m.pfOff, m.pfList, m.peOff, m.peList, m.bFaces, m.bEdges, m.recs);
+  hipLaunchKernelGGL(k_rec_fill, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nPairs, nPT, nRecs, m.pairs,
+                     m.recOff, m.recOff + m.pairCap, m.ppOff, m.ppList, m.pfOff, m.pfList, m.peOff, m.peList,
+                     m.bFaces, m.bEdges, m.recs);
 }
 
 }  // namespace pdipc

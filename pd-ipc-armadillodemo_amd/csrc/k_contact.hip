@@ -335,7 +335,8 @@ __host__ __device__ inline bool record_ids(int4 r, int id[4]) {  // false for sh
 //      four targets to a compact list and counts its simulated points;
 //   2. exclusive scan of the per-point counts;
 //   3. k_col_fill: every (point, record, corner) entry goes to its point's range;
-//   4. k_col_gather: per point, the range is ordered by (record, corner) and summed.
+//   4. k_col_rank: each entry moves to its rank by (record, corner) in the range;
+//   5. k_col_gather: per point, the ordered range is summed.
 __global__ void __launch_bounds__(kBlock) k_col_terms(int nRecs, int ld, int nSim, const int4* __restrict__ recs,
                                                       const double* __restrict__ X, const double* __restrict__ cm,
                                                       double dHat, double kappa, double dt, int actCap,
@@ -386,7 +387,7 @@ __global__ void __launch_bounds__(kBlock) k_col_terms(int nRecs, int ld, int nSi
 
 __global__ void k_col_fill(int ld, int nSim, int actCap, const int4* __restrict__ recs, const int* __restrict__ counters,
                            const int* __restrict__ actRec, const int* __restrict__ ptOff, int* __restrict__ ptCursor,
-                           unsigned long long* __restrict__ entKey, int* __restrict__ entA) {
+                           unsigned long long* __restrict__ entKey, int* __restrict__ entA, int* __restrict__ entPt) {
   int A = min(counters[2], actCap);  // an overflowed pass is repeated by the host
   for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < A; a += gridDim.x * blockDim.x) {
     int t = actRec[a];
@@ -398,32 +399,36 @@ __global__ void k_col_fill(int ld, int nSim, int actCap, const int4* __restrict_
       int slot = ptOff[id[q]] + atomicAdd(ptCursor + id[q], 1);
       entKey[slot] = ((unsigned long long)t << 2) | (unsigned)q;
       entA[slot] = a;
+      entPt[slot] = id[q];
     }
   }
 }
 
-__global__ void k_col_gather(int nSim, int ld, const int* __restrict__ ptOff, unsigned long long* __restrict__ entKey,
-                             int* __restrict__ entA, const double* __restrict__ actW, const double* __restrict__ actT,
-                             double* __restrict__ diag, double* __restrict__ rhs) {
+// Each entry's rank among its point's entries (keys are distinct: a record
+// names four distinct points) places it in key order: entSorted = (a << 2 | corner).
+__global__ void k_col_rank(int nSim, int cap, const int* __restrict__ ptOff, const unsigned long long* __restrict__ entKey,
+                           const int* __restrict__ entA, const int* __restrict__ entPt,
+                           unsigned long long* __restrict__ entSorted) {
+  int E = min(ptOff[nSim], cap);
+  for (int e = blockIdx.x * blockDim.x + threadIdx.x; e < E; e += gridDim.x * blockDim.x) {
+    int i = entPt[e];
+    unsigned long long k = entKey[e];
+    int o = ptOff[i], end = ptOff[i + 1], rank = 0;
+    for (int q = o; q < end; q++) rank += entKey[q] < k ? 1 : 0;
+    entSorted[o + rank] = ((unsigned long long)entA[e] << 2) | (k & 3ull);
+  }
+}
+
+__global__ void k_col_gather(int nSim, int ld, const int* __restrict__ ptOff,
+                             const unsigned long long* __restrict__ entSorted, const double* __restrict__ actW,
+                             const double* __restrict__ actT, double* __restrict__ diag, double* __restrict__ rhs) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= nSim) return;
   int o = ptOff[i], e = ptOff[i + 1];
-  for (int p = o + 1; p < e; p++) {  // insertion sort: a point has a handful of entries
-    unsigned long long k = entKey[p];
-    int av = entA[p];
-    int q = p - 1;
-    while (q >= o && entKey[q] > k) {
-      entKey[q + 1] = entKey[q];
-      entA[q + 1] = entA[q];
-      q--;
-    }
-    entKey[q + 1] = k;
-    entA[q + 1] = av;
-  }
   double d = 0.0, r[3] = {0.0, 0.0, 0.0};
   for (int p = o; p < e; p++) {
-    int a = entA[p];
-    int q = (int)(entKey[p] & 3ull);
+    unsigned long long v = entSorted[p];
+    int a = (int)(v >> 2), q = (int)(v & 3ull);
     double w = actW[a];
     d += w;
 #pragma unroll
@@ -623,9 +628,11 @@ int launch_collision(DeviceMesh& m, int nRecs, double dhat, double kappa, double
                        m.counters);
   launch_exclusive_scan(m.ptCount, m.ptOff, m.nSim + 1, nullptr, m.ptScan, s);
   hipLaunchKernelGGL(k_col_fill, dim3(512), dim3(kBlock), 0, s, m.ld, m.nSim, m.actCap, m.recs, m.counters, m.actRec, m.ptOff,
-                     m.ptCursor, m.entKey, m.entA);
+                     m.ptCursor, m.entKey, m.entA, m.entPt);
+  hipLaunchKernelGGL(k_col_rank, dim3(512), dim3(kBlock), 0, s, m.nSim, 4 * m.actCap, m.ptOff, m.entKey, m.entA, m.entPt,
+                     m.entSorted);
   hipLaunchKernelGGL(k_col_gather, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, m.ptOff,
-                     m.entKey, m.entA, m.actW, m.actT, m.colDiag, m.colRhs);
+                     m.entSorted, m.actW, m.actT, m.colDiag, m.colRhs);
   return 0;
 }
 

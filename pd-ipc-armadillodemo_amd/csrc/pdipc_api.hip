@@ -165,19 +165,20 @@ int upload_all(pdipc_handle* h) {
     AL(segS, (size_t)m.itemCap * traverse_sub_cap()); AL(cntS, m.itemCap); AL(offS, m.itemCap);
   }
   m.pairCap = std::max(1024, 8 * P);
-  AL(pairs, m.pairCap); AL(recCount, m.pairCap); AL(recOff, m.pairCap);
+  AL(pairs, m.pairCap); AL(recCount, 2 * (size_t)m.pairCap); AL(recOff, 2 * (size_t)m.pairCap);
   m.recCap = std::max(4096, 32 * P);
   AL(recs, m.recCap);
   m.scanBlockCap = (std::max(std::max(P, m.pairCap), m.itemCap) + 1023) / 1024 + 2;
   AL(scanBlock, m.scanBlockCap);
   m.actCap = 4096;
   AL(actRec, m.actCap); AL(actW, m.actCap); AL(actT, 12 * (size_t)m.actCap);
-  AL(entKey, 4 * (size_t)m.actCap); AL(entA, 4 * (size_t)m.actCap);
+  AL(entKey, 4 * (size_t)m.actCap); AL(entA, 4 * (size_t)m.actCap); AL(entPt, 4 * (size_t)m.actCap);
+  AL(entSorted, 4 * (size_t)m.actCap);
   AL(ptCount, hs.nSim + 1); AL(ptOff, hs.nSim + 1); AL(ptCursor, hs.nSim + 1);
   AL(ptScan, (hs.nSim + 1 + 1023) / 1024 + 2);
   m.nPartials = (std::max(1, hs.nSim) + 255) / 256;
   AL(partials, m.nPartials);
-  // scalars: residual (8), toiBits (8), counters (8 ints)
+  // scalars: residual (8), toiBits (8), counters (10 ints)
   void* sd = nullptr;
   HIPCHK(hipMalloc(&sd, 64));
   h->allocs.push_back(sd);
@@ -210,7 +211,7 @@ int broad_phase(pdipc_handle* h, const double* x, const double* dx) {
   h->lastRecs = 0;
   if (m.nPatches <= 0) return PDIPC_OK;
   double infl = h->prm.inflation >= 0 ? h->prm.inflation : 0.5 * h->prm.dhat;
-  HIPCHK(hipMemsetAsync(m.counters, 0, 8 * sizeof(int), s));
+  HIPCHK(hipMemsetAsync(m.counters, 0, 10 * sizeof(int), s));
   launch_patch_boxes(m, x, dx, infl, s);
   launch_lbvh(m, s);
   int rc;
@@ -258,9 +259,9 @@ int broad_phase(pdipc_handle* h, const double* x, const double* dx) {
     h->release(m.pairs);
     if ((rc = h->alloc(&m.pairs, cap))) return rc;
     h->release(m.recCount);
-    if ((rc = h->alloc(&m.recCount, cap))) return rc;
+    if ((rc = h->alloc(&m.recCount, 2 * (size_t)cap))) return rc;
     h->release(m.recOff);
-    if ((rc = h->alloc(&m.recOff, cap))) return rc;
+    if ((rc = h->alloc(&m.recOff, 2 * (size_t)cap))) return rc;
     int sb = (cap + 1023) / 1024 + 2;
     if (sb > m.scanBlockCap) {
       h->release(m.scanBlock);
@@ -273,15 +274,18 @@ int broad_phase(pdipc_handle* h, const double* x, const double* dx) {
   launch_rec_count(m, Q, s);
   if ((rc = check_launch(h))) return rc;
   if ((rc = read_scalars(h))) return rc;
-  int K = Q > 0 ? h->hCounters()[1] : 0;
-  if (K < 0) return fail(PDIPC_E_CAPACITY, "candidate count overflows int32");
+  int64_t K64 = Q > 0 ? (int64_t)h->hCounters()[1] + h->hCounters()[8] : 0;
+  int nPT = Q > 0 ? h->hCounters()[1] : 0;
+  if (nPT < 0 || h->hCounters()[8] < 0 || K64 > INT32_MAX / 2)
+    return fail(PDIPC_E_CAPACITY, "candidate count overflows int32");
+  int K = (int)K64;
   if (K > m.recCap) {
     int cap = K + K / 2 + 4096;
     h->release(m.recs);
     if ((rc = h->alloc(&m.recs, cap))) return rc;
     m.recCap = cap;
   }
-  launch_rec_fill(m, Q, K, s);
+  launch_rec_fill(m, Q, nPT, K, s);
   if ((rc = check_launch(h))) return rc;
   h->lastPairs = Q;
   h->lastRecs = K;
@@ -304,10 +308,13 @@ int collision(pdipc_handle* h) {
     if (!h->hCounters()[5]) return PDIPC_OK;
     int A = h->hCounters()[2];
     int cap = A + A / 2 + 4096;
-    for (void* q : {(void*)m.actRec, (void*)m.actW, (void*)m.actT, (void*)m.entKey, (void*)m.entA}) h->release(q);
+    for (void* q : {(void*)m.actRec, (void*)m.actW, (void*)m.actT, (void*)m.entKey, (void*)m.entA, (void*)m.entPt,
+                    (void*)m.entSorted})
+      h->release(q);
     m.actCap = cap;
     if ((rc = h->alloc(&m.actRec, cap)) || (rc = h->alloc(&m.actW, cap)) || (rc = h->alloc(&m.actT, 12 * (size_t)cap)) ||
-        (rc = h->alloc(&m.entKey, 4 * (size_t)cap)) || (rc = h->alloc(&m.entA, 4 * (size_t)cap)))
+        (rc = h->alloc(&m.entKey, 4 * (size_t)cap)) || (rc = h->alloc(&m.entA, 4 * (size_t)cap)) ||
+        (rc = h->alloc(&m.entPt, 4 * (size_t)cap)) || (rc = h->alloc(&m.entSorted, 4 * (size_t)cap)))
       return rc;
   }
   return fail(PDIPC_E_CAPACITY, "active collision list did not fit after growing");
