@@ -15,7 +15,8 @@
 // 2-path lists once per outer iteration.
 //
 // FP32-split: the sweeps run in FP32 on the correction delta = x - x0, driven
-// by the FP64 residual of the fixed-point map r0 = M^2 x0 + c - x0:
+// by the FP64 residual of the fixed-point map r0 = M^2 x0 + c - x0 (computed
+// as M (M x0) + c - x0 over R1, two light passes instead of the M^2 pattern):
 //   delta_1 = r0,  delta_{k+1} = W_k (M^2 delta_k + r0 - delta_{k-1}) + delta_{k-1}
 // which is algebraically the same iterate; FP32 only rounds the (small)
 // correction, never the absolute coordinates.
@@ -74,19 +75,16 @@ struct FirstTermArgs {  // c = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
   double w;
 };
 
-// The first sweep of a solve, reading x0 = X (SoA) and computing c on the fly
-// (the first term is fused: the G lanes of a row split its R1 entries).
-// MODE 0 (FP64): c -> cA, x0 -> xa0, x1 = M^2 x0 + c -> xa1 (and out, SoA, when
-// given);  MODE 2 (FP32-split start): r0 = M^2 x0 + c - x0, delta_0 = 0,
-// delta_1 = r0.
-template <int G, int MODE>
+// The first FP64 sweep of a solve, reading x0 = X (SoA) and computing c on the
+// fly (the first term is fused: the G lanes of a row split its R1 entries):
+// c -> cA, x0 -> xa0, x1 = M^2 x0 + c -> xa1 (and out, SoA, when given).
+template <int G>
 __global__ void __launch_bounds__(kBlock) k_first64(int n, int nSlices, int ld, const int64_t* __restrict__ sliceOff,
                                                     const int* __restrict__ sliceW, const int* __restrict__ col,
                                                     const double* __restrict__ val, const double* __restrict__ X,
                                                     Row64* __restrict__ cA, Row64* __restrict__ xa0,
                                                     Row64* __restrict__ xa1, double* __restrict__ out,
-                                                    float4* __restrict__ dpreF, float4* __restrict__ dnextF,
-                                                    float4* __restrict__ r0F, FirstTermArgs ft) {
+                                                    FirstTermArgs ft) {
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int slice = gt / kWave, lane = gt % kWave;
   if (slice >= nSlices) return;  // whole wavefront
@@ -125,21 +123,81 @@ __global__ void __launch_bounds__(kBlock) k_first64(int n, int nSlices, int ld, 
     y[k] = acc[k] + c[k];
     x0[k] = X[idx];
   }
-  if (MODE == 0) {
-    cA[row] = Row64{c[0], c[1], c[2], 0.0};
-    xa0[row] = Row64{x0[0], x0[1], x0[2], 0.0};
-    xa1[row] = Row64{y[0], y[1], y[2], 0.0};
-    if (out) {
-      out[row] = y[0];
-      out[ld + row] = y[1];
-      out[2 * ld + row] = y[2];
-    }
-  } else {
-    float4 rv = make_float4((float)(y[0] - x0[0]), (float)(y[1] - x0[1]), (float)(y[2] - x0[2]), 0.0f);
-    r0F[row] = rv;
-    dpreF[row] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-    dnextF[row] = rv;
+  cA[row] = Row64{c[0], c[1], c[2], 0.0};
+  xa0[row] = Row64{x0[0], x0[1], x0[2], 0.0};
+  xa1[row] = Row64{y[0], y[1], y[2], 0.0};
+  if (out) {
+    out[row] = y[0];
+    out[ld + row] = y[1];
+    out[2 * ld + row] = y[2];
   }
+}
+
+// FP32-split start: the FP64 residual r0 = M^2 x + c - x of x = X, with M^2 x
+// applied as M (M x) (M = (1 - w) I + w D^{-1} R1: two passes over the ~7x
+// smaller R1 instead of one over the M^2 pattern; equal up to rounding), then
+// delta_0 = 0, delta_1 = r0.  With y = M x and u = D^{-1} b,
+//   M^2 x + c - x = (1 - w) y + (2 - w) w u - x  +  w D^{-1} R1 (y + w u),
+// so pass A gathers x only and writes v = y + w u and the pointwise part pt;
+// pass B adds w D^{-1} R1 v.  G lanes per R1 row.
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_res_a(int n, int ld, double w, const int* __restrict__ off,
+                                                  const int* __restrict__ col, const double* __restrict__ val,
+                                                  const double* __restrict__ D, const double* __restrict__ X,
+                                                  const double* __restrict__ b, Row64* __restrict__ v,
+                                                  Row64* __restrict__ pt) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int row = gt / G, sub = gt % G;
+  double sx[3] = {0.0, 0.0, 0.0};
+  if (row < n)
+    for (int q = off[row] + sub; q < off[row + 1]; q += G) {
+      int j = col[q];
+      double a = val[q];
+      sx[0] += a * X[j];
+      sx[1] += a * X[ld + j];
+      sx[2] += a * X[2 * ld + j];
+    }
+  group_reduce<G>(sx);
+  if (sub != 0 || row >= n) return;
+  double di = D[row];
+  double vv[3], pp[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    double x = X[k * ld + row];
+    double y = (1.0 - w) * x + w * sx[k] / di;
+    double u = b[k * ld + row] / di;
+    vv[k] = y + w * u;
+    pp[k] = (1.0 - w) * y + (2.0 - w) * w * u - x;
+  }
+  v[row] = Row64{vv[0], vv[1], vv[2], 0.0};
+  pt[row] = Row64{pp[0], pp[1], pp[2], 0.0};
+}
+
+template <int G>
+__global__ void __launch_bounds__(kBlock) k_res_b(int n, double w, const int* __restrict__ off,
+                                                  const int* __restrict__ col, const double* __restrict__ val,
+                                                  const double* __restrict__ D, const Row64* __restrict__ v,
+                                                  const Row64* __restrict__ pt, float4* __restrict__ dpreF,
+                                                  float4* __restrict__ dnextF, float4* __restrict__ r0F) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int row = gt / G, sub = gt % G;
+  double sv[3] = {0.0, 0.0, 0.0};
+  if (row < n)
+    for (int q = off[row] + sub; q < off[row + 1]; q += G) {
+      double a = val[q];
+      Row64 vj = v[col[q]];
+      sv[0] += a * vj.x;
+      sv[1] += a * vj.y;
+      sv[2] += a * vj.z;
+    }
+  group_reduce<G>(sv);
+  if (sub != 0 || row >= n) return;
+  double f = w / D[row];
+  Row64 p = pt[row];
+  float4 rv = make_float4((float)(p.x + f * sv[0]), (float)(p.y + f * sv[1]), (float)(p.z + f * sv[2]), 0.0f);
+  r0F[row] = rv;
+  dpreF[row] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+  dnextF[row] = rv;
 }
 
 // FP64 Chebyshev step x_{k+1} = W (M^2 x_k + c - x_{k-1}) + x_{k-1} on Row64
@@ -294,9 +352,8 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   const FirstTermArgs ft{m.r1Off, m.r1Col, m.r1Val, m.sysDiag, m.sysRhs, w};
   double r = rho * rho;
   double W = 2.0 / (2.0 - r);
-  hipLaunchKernelGGL((k_first64<G, 0>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff,
-                     m.sliceW, m.m2Col, m.m2Val, m.X, m.cA, m.xa0, m.xa1, outs ? outs[1] : nullptr, nullptr,
-                     nullptr, nullptr, ft);
+  hipLaunchKernelGGL((k_first64<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff, m.sliceW,
+                     m.m2Col, m.m2Val, m.X, m.cA, m.xa0, m.xa1, outs ? outs[1] : nullptr, ft);
   Row64 *pre = m.xa0, *cur = m.xa1, *next = m.xa2;
   if (iters <= 1) {
     hipLaunchKernelGGL(k_row64_to_soa, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, cur,
@@ -321,13 +378,16 @@ void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precis
     return;
   }
   const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
-  const FirstTermArgs ft{m.r1Off, m.r1Col, m.r1Val, m.sysDiag, m.sysRhs, w};
   double r = rho * rho;
   double W = 2.0 / (2.0 - r);
   float4 *pre = m.dPre, *cur = m.dCur, *next = m.dNext;
   // delta_0 = 0 -> pre, delta_1 = r0 -> cur
-  hipLaunchKernelGGL((k_first64<G, 2>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff, m.sliceW,
-                     m.m2Col, m.m2Val, m.X, nullptr, nullptr, nullptr, nullptr, pre, cur, m.r0, ft);
+  constexpr int GR = 4;  // lanes per R1 row
+  const int gridR = (m.nSim * GR + kBlock - 1) / kBlock;
+  hipLaunchKernelGGL((k_res_a<GR>), dim3(gridR), dim3(kBlock), 0, s, m.nSim, m.ld, w, m.r1Off, m.r1Col, m.r1Val,
+                     m.sysDiag, m.X, m.sysRhs, m.xa0, m.cA);
+  hipLaunchKernelGGL((k_res_b<GR>), dim3(gridR), dim3(kBlock), 0, s, m.nSim, w, m.r1Off, m.r1Col, m.r1Val, m.sysDiag,
+                     m.xa0, m.cA, pre, cur, m.r0);
   if (iters <= 1) {
     hipLaunchKernelGGL(k_apply_delta, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, cur,
                        m.X);
