@@ -137,6 +137,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--scene", choices=("cloth", "tet_block"), default="cloth",
+                    help="cloth: the n x n cloth over the sphere (default); tet_block: the 40^3-cell block "
+                         "(320,000 tets) under a 128x128 cloth, dt = 1/200, 15 local/global iterations")
     ap.add_argument("--n", type=int, default=256, help="cloth vertices per side")
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
@@ -169,8 +172,14 @@ def main():
         print(json.dumps(adjoint_bench(args.graphs, args.adjoint_fd)), flush=True)
         return
 
-    scene = scenes.cloth_over_sphere(args.n, args.subdiv, ground=True, seed=0, patch_dim=args.n,
-                                     sphere_patch_dim=args.n)
+    if args.scene == "tet_block":
+        scene = scenes.config2()
+        workload = ("tet block 40^3 cells (320,000 tets) + 128x128 cloth on the ground, dt=1/200, "
+                    "15 LG x 15 Chebyshev, dHat=1e-3, kappa=1e3")
+    else:
+        scene = scenes.cloth_over_sphere(args.n, args.subdiv, ground=True, seed=0, patch_dim=args.n,
+                                         sphere_patch_dim=args.n)
+        workload = None
     t0 = time.perf_counter()
     sims = {}
     for name, prec in (("fp64", pdipc.FP64), ("fp32_split", pdipc.FP32_SPLIT)):
@@ -245,8 +254,8 @@ def main():
         "dtype": "f32+f64",
         "data": "synthetic",
         "config": {
-            "workload": f"cloth {args.n}x{args.n} ({c['cloth_points']} vertices) over icosphere subdiv "
-                        f"{args.subdiv} + ground, dt=1/100, 10 LG x 15 Chebyshev, dHat=1e-3, kappa=1e3",
+            "workload": workload or f"cloth {args.n}x{args.n} ({c['cloth_points']} vertices) over icosphere subdiv "
+                                    f"{args.subdiv} + ground, dt=1/100, 10 LG x 15 Chebyshev, dHat=1e-3, kappa=1e3",
             "precision": "FP32-split (FP32 local step + sweeps on an FP64 residual; FP64 CCD/barrier)",
             "fp64_steps_per_s": steps / res["fp64"]["seconds"],
             "fp32_split_speedup_vs_fp64": res["fp64"]["seconds"] / t32,
@@ -267,7 +276,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
-            "traffic": SWEEP32_TRAFFIC_BYTES.get((args.n, args.subdiv, args.sell_lanes)),
+            "traffic": None if workload else SWEEP32_TRAFFIC_BYTES.get((args.n, args.subdiv, args.sell_lanes)),
             "avg_us": us32,
             "bytes_per_launch": bytes32,
             "fp64_sweep": {"avg_us": us64, "achieved": bytes64 / (us64 * 1e-6) / 1e9},
