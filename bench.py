@@ -48,25 +48,46 @@ def timed_steps(sim, steps):
     return time.perf_counter() - t0, outer
 
 
-def cpu_baseline(scene, seconds_budget=20.0):
-    """The FP64 oracle (a single-threaded C++ restatement of the reference step)
-    on this host, on a bounded sample: full-size scene, as many steps as fit
-    the budget after the first."""
+def _host_threads():
+    n = os.environ.get("OMP_NUM_THREADS", "")
+    if n.isdigit() and int(n) > 0:
+        return int(n)
+    return len(os.sched_getaffinity(0))
+
+
+def cpu_baseline(scene, x, v, seconds_budget=20.0):
+    """The FP64 oracle (the C++ restatement of the reference step; OpenMP over
+    the solve, operator update, TOI and barrier-activity loops) on this host, on
+    a bounded sample: the full-size scene from the same warmed-up state the GPU
+    timing starts from, as many steps as fit the budget, on all the host
+    threads this process may use (`value`) and on one thread."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker, timed here as the CPU baseline
 
-    t0 = time.perf_counter()
-    ref = oracle.Oracle(scene)
-    init_s = time.perf_counter() - t0
-    n, t = 0, 0.0
-    while t < seconds_budget and n < 50:
-        t1 = time.perf_counter()
-        ref.step(1)
-        t += time.perf_counter() - t1
-        n += 1
-    return dict(value=n / t, unit="steps/s", cores=1, kind="port",
-                sample=f"{n} FP64 oracle steps of the same 256x256 scene from rest ({t:.1f} s; setup {init_s:.1f} s), "
-                       "single thread")
+    def sample(threads, budget):
+        oracle.set_threads(threads)
+        t0 = time.perf_counter()
+        ref = oracle.Oracle(scene)
+        ref.set_x(x)
+        ref.set_v(v)
+        init_s = time.perf_counter() - t0
+        n, t = 0, 0.0
+        while t < budget and n < 50:
+            t1 = time.perf_counter()
+            ref.step(1)
+            t += time.perf_counter() - t1
+            n += 1
+        return n, t, init_s
+
+    threads = _host_threads()
+    n, t, init_s = sample(threads, 0.5 * seconds_budget)
+    n1, t1, _ = sample(1, seconds_budget)
+    oracle.set_threads(0)
+    return dict(value=n / t, unit="steps/s", cores=threads, kind="port",
+                single_thread_steps_per_s=n1 / t1,
+                sample=f"FP64 oracle steps of the same 256x256 scene from the GPU timing's warmed-up state: "
+                       f"{n} steps on {threads} threads ({t:.1f} s), {n1} steps on 1 thread ({t1:.1f} s); "
+                       f"setup {init_s:.1f} s each")
 
 
 def adjoint_bench(graphs, fd_dirs, log_every=8):
@@ -235,7 +256,7 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene, args.cpu_seconds)
+        cpu = cpu_baseline(scene, x_w, v_w, args.cpu_seconds)
         log(f"cpu baseline {cpu}")
 
     steps = args.steps

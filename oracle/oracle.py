@@ -30,6 +30,16 @@ def build():
     subprocess.run(["make", "-s", "-C", HERE], check=True)
 
 
+def set_threads(n):
+    """OpenMP threads of the oracle's parallel loops (n <= 0: all processors).
+    The results do not depend on the count."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads():
+    return lib().orc_get_threads()
+
+
 def lib():
     global _lib
     if _lib is None:
@@ -47,6 +57,8 @@ def lib():
         for name in ["orc_init"]:
             getattr(L, name).argtypes = [vp]
         L.orc_step.argtypes = [vp, ctypes.c_int]
+        L.orc_set_threads.argtypes = [ctypes.c_int]
+        L.orc_get_threads.restype = ctypes.c_int
         L.orc_sizes.argtypes = [vp, _I]
         for name in ["orc_get_x", "orc_set_x", "orc_get_v", "orc_set_v", "orc_counters", "orc_elastic_diag", "orc_mass"]:
             getattr(L, name).argtypes = [vp, _D]

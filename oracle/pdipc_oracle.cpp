@@ -35,6 +35,8 @@
 //      followed by Gram-Schmidt (the structure of McAdams et al. 2011, whose
 //      4-sweep approximate-Givens variant the reference uses, CudaSVD.cu:291-316).
 
+#include <omp.h>
+
 #include <algorithm>
 #include <array>
 #include <cfloat>
@@ -1283,8 +1285,10 @@ struct Sim {
   // updateR1/R2/R2RowOperatorsKernel (PdIpcGpuFunc.cu:899-974)
   void update_operators() {
     const R w = (R)prm.v[P_OMEGA];
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < simPoints; i++)
       for (size_t a = 0; a < r1Idx[i].size(); a++) r1Op[i][a] = r1Val[i][a] / sysDiag[i];
+#pragma omp parallel for schedule(dynamic, 256)
     for (int i = 0; i < simPoints; i++)
       for (size_t e = 0; e < r2Idx[i].size(); e++) {
         R val = 0;
@@ -1302,6 +1306,7 @@ struct Sim {
   void cheb_solve(std::vector<R>*& pre, std::vector<R>*& cur, std::vector<R>*& next) {
     const R w = (R)prm.v[P_OMEGA];
     const int n = simPoints;
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++)
       for (int k = 0; k < 3; k++) {
         R val = 0;
@@ -1316,6 +1321,7 @@ struct Sim {
       for (size_t e = 0; e < r2Idx[i].size(); e++) val += r2Row[i][e] * xin[3 * r2Idx[i][e] + k];
       return val;
     };
+#pragma omp parallel for schedule(static)
     for (int i = 0; i < n; i++)
       for (int k = 0; k < 3; k++) (*next)[3 * i + k] = spmv_row(*cur, i, k) + firstTerm[3 * i + k];
     std::swap(cur, pre);
@@ -1325,6 +1331,7 @@ struct Sim {
     R omega = (R)2.0 / ((R)2 - rho);
     int maxIter = (int)prm.v[P_CHEB_ITERS];
     for (int it = 1; it < maxIter; it++) {
+#pragma omp parallel for schedule(static)
       for (int i = 0; i < n; i++)
         for (int k = 0; k < 3; k++) {
           int idx = 3 * i + k;
@@ -1432,37 +1439,88 @@ struct Sim {
   }
 
   // getImpactOfTimeKernel + host min (PdIpcGpuFunc.cu:2610-2742)
+  // The minimum over the records: threads take contiguous ranges and keep the
+  // first record attaining their strict minimum; merging by (value, record)
+  // yields the serial loop's answer (the first record attaining the minimum).
   R compute_toi(const std::vector<R>& x0, const std::vector<R>& x1) {
     R toiMin = 1;
+    int64_t best = INT64_MAX;
     int tp = (int)prm.v[P_TOUCH_POLICY];
-    for (auto& c : cands) {
-      int ids[4] = {c[1], c[2], c[3], c[4]};
-      R x[4][3], dx[4][3];
-      for (int i = 0; i < 4; i++)
-        for (int k = 0; k < 3; k++) {
-          x[i][k] = x0[3 * ids[i] + k];
-          dx[i][k] = x1[3 * ids[i] + k] - x[i][k];
+    const int64_t nc = (int64_t)cands.size();
+#pragma omp parallel
+    {
+      R lmin = 1;
+      int64_t lidx = INT64_MAX;
+#pragma omp for schedule(static) nowait
+      for (int64_t ci = 0; ci < nc; ci++) {
+        const auto& c = cands[ci];
+        int ids[4] = {c[1], c[2], c[3], c[4]};
+        R x[4][3], dx[4][3];
+        for (int i = 0; i < 4; i++)
+          for (int k = 0; k < 3; k++) {
+            x[i][k] = x0[3 * ids[i] + k];
+            dx[i][k] = x1[3 * ids[i] + k] - x[i][k];
+          }
+        R toi = 1;
+        if (c[0] == 0) {
+          if (ids[0] == ids[1] || ids[0] == ids[2] || ids[0] == ids[3]) continue;
+          if (!pt_ccd_filter(x[0], x[1], x[2], x[3], dx[0], dx[1], dx[2], dx[3], (R)prm.v[P_CCD_FILTER])) continue;
+        } else {
+          if (ids[0] == ids[2] || ids[0] == ids[3] || ids[1] == ids[2] || ids[1] == ids[3]) continue;
+          if (!ee_ccd_filter(x[0], x[1], x[2], x[3], dx[0], dx[1], dx[2], dx[3], (R)prm.v[P_CCD_FILTER])) continue;
         }
-      R toi = 1;
-      if (c[0] == 0) {
-        if (ids[0] == ids[1] || ids[0] == ids[2] || ids[0] == ids[3]) continue;
-        if (!pt_ccd_filter(x[0], x[1], x[2], x[3], dx[0], dx[1], dx[2], dx[3], (R)prm.v[P_CCD_FILTER])) continue;
-      } else {
-        if (ids[0] == ids[2] || ids[0] == ids[3] || ids[1] == ids[2] || ids[1] == ids[3]) continue;
-        if (!ee_ccd_filter(x[0], x[1], x[2], x[3], dx[0], dx[1], dx[2], dx[3], (R)prm.v[P_CCD_FILTER])) continue;
+        toi = ccd_toi<R>(c[0], x, dx, (int)prm.v[P_CCD_ITERS], (R)prm.v[P_CCD_STEP], (R)prm.v[P_CCD_GAP], tp);
+        if (toi < lmin) {
+          lmin = toi;
+          lidx = ci;
+        }
       }
-      toi = ccd_toi<R>(c[0], x, dx, (int)prm.v[P_CCD_ITERS], (R)prm.v[P_CCD_STEP], (R)prm.v[P_CCD_GAP], tp);
-      if (toi < toiMin) toiMin = toi;
+#pragma omp critical(orc_toi_min)
+      {
+        if (lmin < toiMin || (!(toiMin < lmin) && lidx < best)) {
+          toiMin = lmin;
+          best = lidx;
+        }
+      }
     }
     if (toiMin < 1) toiMin *= (R)prm.v[P_TOI_SCALE];
     return toiMin;
   }
 
   // setCollisionConstraintsKernel (PdIpcGpuFunc.cu:2756-2892)
+  // The activity test (filter and distance) runs over the records in
+  // parallel; the accumulation then visits the active records in record order,
+  // so the sums are those of the serial loop.
   void collision_constraints(const std::vector<R>& x, R dt) {
     const R dHat = (R)prm.v[P_DHAT], dHat2 = dHat * dHat, kappa = (R)prm.v[P_KAPPA];
+    const int64_t nc = (int64_t)cands.size();
+    std::vector<uint8_t> act(nc, 0);
+#pragma omp parallel for schedule(static)
+    for (int64_t ci = 0; ci < nc; ci++) {
+      const auto& c = cands[ci];
+      int t0 = c[1], t1 = c[2], t2 = c[3], t3 = c[4];
+      const R* x0 = &x[3 * t0];
+      const R* x1 = &x[3 * t1];
+      const R* x2 = &x[3 * t2];
+      const R* x3 = &x[3 * t3];
+      R dir[3], d2;
+      if (c[0] == 0) {
+        if (t0 == t1 || t0 == t2 || t0 == t3) continue;
+        if (!pt_cd_filter(x0, x1, x2, x3, dHat)) continue;
+        R a, b, cc;
+        d2 = vertex_triangle_sqd(x0, x1, x2, x3, &a, &b, &cc, dir);
+      } else {
+        if (t0 == t2 || t0 == t3 || t1 == t2 || t1 == t3) continue;
+        if (!ee_cd_filter(x0, x1, x2, x3, dHat)) continue;
+        R r, sp;
+        d2 = edge_edge_sqd(x0, x1, x2, x3, &r, &sp, dir);
+      }
+      act[ci] = d2 < dHat2 ? 1 : 0;
+    }
     int active = 0;
-    for (auto& c : cands) {
+    for (int64_t ci = 0; ci < nc; ci++) {
+      if (!act[ci]) continue;
+      const auto& c = cands[ci];
       int t0 = c[1], t1 = c[2], t2 = c[3], t3 = c[4];
       const R* x0 = &x[3 * t0];
       const R* x1 = &x[3 * t1];
@@ -1580,6 +1638,11 @@ static auto with_sim(OracleHandle* h, F&& fn) {
 }
 
 extern "C" {
+// OpenMP threads of the parallel loops (solve, operator update, TOI, barrier
+// activity); n <= 0: all processors.  Results do not depend on the count.
+void orc_set_threads(int n) { omp_set_num_threads(n > 0 ? n : omp_get_num_procs()); }
+int orc_get_threads() { return omp_get_max_threads(); }
+
 
 void* orc_create(const double* params, int nparams, int useFloat) {
   auto* h = new OracleHandle();
