@@ -4,12 +4,40 @@
 #include <algorithm>
 #include <cfloat>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <numeric>
+#include <sched.h>
 #include <stdexcept>
+#include <thread>
 
 namespace pdipc {
 namespace {
+
+// Host threads for the row-parallel parts of the precompute: OMP_NUM_THREADS
+// when set, else the CPUs this process may run on (at most 64).
+int setup_threads() {
+  const char* e = std::getenv("OMP_NUM_THREADS");
+  int n = e ? std::atoi(e) : 0;
+  if (n <= 0) {
+    cpu_set_t set;
+    n = sched_getaffinity(0, sizeof(set), &set) == 0 ? CPU_COUNT(&set) : 1;
+  }
+  return std::max(1, std::min(n, 64));
+}
+
+// fn(begin, end, t) over `parts` contiguous ranges of [0, n), one thread each
+template <class F>
+void parallel_ranges(int64_t n, int parts, F fn) {
+  parts = (int)std::max<int64_t>(1, std::min<int64_t>(parts, n / 1024 + 1));
+  std::vector<std::thread> th;
+  for (int t = 0; t < parts; t++) {
+    int64_t b = n * t / parts, e = n * (t + 1) / parts;
+    if (t == parts - 1) fn(b, e, t);
+    else th.emplace_back(fn, b, e, t);
+  }
+  for (auto& x : th) x.join();
+}
 
 inline void sub3(const double* a, const double* b, double* r) { r[0] = a[0] - b[0]; r[1] = a[1] - b[1]; r[2] = a[2] - b[2]; }
 inline double dot3(const double* a, const double* b) { return a[0] * b[0] + a[1] * b[1] + a[2] * b[2]; }
@@ -606,18 +634,26 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
       for (int q = hs.r1Off[i]; q < hs.r1Off[i + 1]; q++)
         if (hs.r1Col[q] == hs.cNbr[sl]) { hs.cSlotR1[sl] = q; break; }
   }
-  // ---- R2 = R1 * R1 value pattern, then the 2-path lists (PdIpcSimulator.cpp:704-796)
+  // ---- R2 = R1 * R1 value pattern, then the 2-path lists (PdIpcSimulator.cpp:704-796).
+  // Rows are independent.  Two passes over contiguous row ranges on their own
+  // threads: count each row's entries and 2-paths, then (after one allocation
+  // of the exact sizes) write them at their offsets; the result is the serial one.
   {
-    int n = hs.nSim;
-    std::vector<double> acc(n, 0.0);
-    std::vector<char> seen(n, 0);
-    std::vector<int> touched;
-    hs.m2Off.assign(1, 0);
-    hs.p2Off.assign(1, 0);
-    for (int i = 0; i < n; i++) {
-      touched.clear();
+    const int n = hs.nSim;
+    const int nt = setup_threads();
+    std::vector<int> rowEnt(n, 0);
+    std::vector<int64_t> rowPath(n, 0);
+    // visit(i, emit): the row's entries in ascending column order; emit(j, r1q, paths)
+    // receives the column, the R1 index of (i, j) and the row's valid 2-paths (k, a, ia, ib)
+    struct Scratch {
+      std::vector<double> acc;
+      std::vector<char> seen;
+      std::vector<int> touched;
+    };
+    auto visit = [&](int i, Scratch& S, auto&& onEntry, auto&& onPath) {
+      S.touched.clear();
       // the diagonal slot of R1 holds an explicit zero; it only adds structure
-      auto touch = [&](int j) { if (!seen[j]) { seen[j] = 1; touched.push_back(j); acc[j] = 0.0; } };
+      auto touch = [&](int j) { if (!S.seen[j]) { S.seen[j] = 1; S.touched.push_back(j); S.acc[j] = 0.0; } };
       for (int q = hs.r1Off[i]; q < hs.r1Off[i + 1]; q++) touch(hs.r1Col[q]);
       touch(i);
       for (int a = hs.r1Off[i]; a < hs.r1Off[i + 1]; a++) {
@@ -627,15 +663,15 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
         for (int b = hs.r1Off[kk]; b < hs.r1Off[kk + 1]; b++) {
           int j = hs.r1Col[b];
           touch(j);
-          acc[j] += vik * hs.r1Val[b];
+          S.acc[j] += vik * hs.r1Val[b];
         }
       }
-      std::sort(touched.begin(), touched.end());
-      for (int j : touched) {
-        seen[j] = 0;
-        if (std::fabs(acc[j]) < IG) continue;
+      std::sort(S.touched.begin(), S.touched.end());
+      for (int j : S.touched) {
+        S.seen[j] = 0;
+        if (std::fabs(S.acc[j]) < IG) continue;
         // 2-paths: k in offdiag(R1 row i) and offdiag(R1 row j), ascending k
-        int64_t before = (int64_t)hs.p2K.size();
+        int np = 0;
         int a = hs.r1Off[i], ae = hs.r1Off[i + 1], b = hs.r1Off[j], be = hs.r1Off[j + 1];
         while (a < ae && b < be) {
           if (hs.r1Col[a] < hs.r1Col[b]) a++;
@@ -643,28 +679,66 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
           else {
             double ajk = hs.r1Val[a] * hs.r1Val[b];
             if (!(std::fabs(ajk) < IG)) {
-              hs.p2K.push_back(hs.r1Col[a]);
-              hs.p2A.push_back(ajk);
-              hs.p2Ia.push_back(a);
-              hs.p2Ib.push_back(b);
+              onPath(hs.r1Col[a], ajk, a, b);
+              np++;
             }
             a++; b++;
           }
         }
-        if ((int64_t)hs.p2K.size() == before) continue;  // no valid 2-path: entry dropped
-        hs.m2Col.push_back(j);
-        double r1v = 0.0;
+        if (np == 0) continue;  // no valid 2-path: entry dropped
         int r1q = -1;
         for (int q = hs.r1Off[i]; q < hs.r1Off[i + 1]; q++)
-          if (hs.r1Col[q] == j) { r1v = hs.r1Val[q]; r1q = q; break; }
-        hs.m2R1.push_back(r1v);
-        hs.m2R1Idx.push_back(r1q);
-        hs.p2Off.push_back((int64_t)hs.p2K.size());
+          if (hs.r1Col[q] == j) { r1q = q; break; }
+        onEntry(j, r1q, np);
       }
-      hs.m2Off.push_back((int)hs.m2Col.size());
+    };
+    parallel_ranges(n, nt, [&](int64_t r0, int64_t r1, int) {
+      Scratch S{std::vector<double>(n, 0.0), std::vector<char>(n, 0), {}};
+      for (int i = (int)r0; i < (int)r1; i++) {
+        int ne = 0;
+        int64_t np = 0;
+        visit(i, S, [&](int, int, int p) { ne++; np += p; }, [&](int, double, int, int) {});
+        rowEnt[i] = ne;
+        rowPath[i] = np;
+      }
+    });
+    hs.m2Off.assign(n + 1, 0);
+    std::vector<int64_t> pathOff(n + 1, 0);
+    int64_t nE = 0;
+    for (int i = 0; i < n; i++) {
+      nE += rowEnt[i];
+      if (nE > INT32_MAX) return "the M^2 pattern exceeds 2^31 entries";
+      hs.m2Off[i + 1] = (int)nE;
+      pathOff[i + 1] = pathOff[i] + rowPath[i];
     }
-    hs.nnzR2 = (int64_t)hs.m2Col.size();
-    hs.n2paths = (int64_t)hs.p2K.size();
+    int64_t nP = pathOff[n];
+    hs.m2Col.resize(nE); hs.m2R1.resize(nE); hs.m2R1Idx.resize(nE); hs.p2Off.resize(nE + 1);
+    hs.p2K.resize(nP); hs.p2A.resize(nP); hs.p2Ia.resize(nP); hs.p2Ib.resize(nP);
+    hs.p2Off[0] = 0;
+    parallel_ranges(n, nt, [&](int64_t r0, int64_t r1, int) {
+      Scratch S{std::vector<double>(n, 0.0), std::vector<char>(n, 0), {}};
+      for (int i = (int)r0; i < (int)r1; i++) {
+        int64_t e = hs.m2Off[i], q = pathOff[i];
+        visit(
+            i, S,
+            [&](int j, int r1q, int) {
+              hs.m2Col[e] = j;
+              hs.m2R1[e] = r1q >= 0 ? hs.r1Val[r1q] : 0.0;
+              hs.m2R1Idx[e] = r1q;
+              hs.p2Off[e + 1] = q;
+              e++;
+            },
+            [&](int k, double a, int ia, int ib) {
+              hs.p2K[q] = k;
+              hs.p2A[q] = a;
+              hs.p2Ia[q] = ia;
+              hs.p2Ib[q] = ib;
+              q++;
+            });
+      }
+    });
+    hs.nnzR2 = nE;
+    hs.n2paths = nP;
   }
   // ---- collision topology (PdIpcSimulator.cpp:1151-1247)
   {
@@ -739,24 +813,28 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
   L.r1.assign(total, 0.0);
   L.r1Idx.assign(total, -1);
   std::vector<int64_t> src(total, -1);  // CSR entry feeding each SELL slot
-  for (int s = 0; s < L.nSlices; s++)
-    for (int lr = 0; lr < L.rowsPerSlice; lr++) {
-      int r = s * L.rowsPerSlice + lr;
-      int rr = r < n ? r : n - 1;  // rows past the end pad with a valid index
-      for (int c = 0; c < L.sliceW[s] * lanes; c++) {
-        int64_t e = L.sliceOff[s] + (int64_t)(c / lanes) * 64 + lr * lanes + (c % lanes);
-        L.col[e] = rr;
-        L.rowFlag[e] = rr;
-        if (r < n && c < hs.m2Off[r + 1] - hs.m2Off[r]) {
-          int64_t q = hs.m2Off[r] + c;
-          L.col[e] = hs.m2Col[q];
-          L.rowFlag[e] = r | (hs.m2Col[q] == r ? (int)0x80000000 : 0);
-          L.r1[e] = hs.m2R1[q];
-          L.r1Idx[e] = hs.m2R1Idx[q];
-          src[e] = q;
+  const int nt = setup_threads();
+  parallel_ranges(L.nSlices, nt, [&](int64_t s0, int64_t s1, int) {
+    for (int s = (int)s0; s < (int)s1; s++)
+      for (int lr = 0; lr < L.rowsPerSlice; lr++) {
+        int r = s * L.rowsPerSlice + lr;
+        int rr = r < n ? r : n - 1;  // rows past the end pad with a valid index
+        for (int c = 0; c < L.sliceW[s] * lanes; c++) {
+          int64_t e = L.sliceOff[s] + (int64_t)(c / lanes) * 64 + lr * lanes + (c % lanes);
+          L.col[e] = rr;
+          L.rowFlag[e] = rr;
+          if (r < n && c < hs.m2Off[r + 1] - hs.m2Off[r]) {
+            int64_t q = hs.m2Off[r] + c;
+            L.col[e] = hs.m2Col[q];
+            L.rowFlag[e] = r | (hs.m2Col[q] == r ? (int)0x80000000 : 0);
+            L.r1[e] = hs.m2R1[q];
+            L.r1Idx[e] = hs.m2R1Idx[q];
+            src[e] = q;
+          }
         }
       }
-    }
+  });
+  // 16-bit row-relative columns when every offset fits
   L.col16ok = true;
   for (int s = 0; s < L.nSlices && L.col16ok; s++)
     for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++) {
@@ -769,9 +847,11 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
     }
   if (L.col16ok) {
     L.col16.resize(total);
-    for (int s = 0; s < L.nSlices; s++)
-      for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++)
-        L.col16[e] = (int16_t)(L.col[e] - (s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes));
+    parallel_ranges(L.nSlices, nt, [&](int64_t s0, int64_t s1, int) {
+      for (int s = (int)s0; s < (int)s1; s++)
+        for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++)
+          L.col16[e] = (int16_t)(L.col[e] - (s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes));
+    });
   }
   L.pOff.assign(total + 1, 0);
   for (int64_t e = 0; e < total; e++) {
@@ -782,14 +862,16 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
   L.pA.resize(L.pOff.back());
   L.pIa.resize(L.pOff.back());
   L.pIb.resize(L.pOff.back());
-  for (int64_t e = 0; e < total; e++) {
-    int64_t q = src[e];
-    if (q < 0) continue;
-    std::copy(hs.p2K.begin() + hs.p2Off[q], hs.p2K.begin() + hs.p2Off[q + 1], L.pK.begin() + L.pOff[e]);
-    std::copy(hs.p2A.begin() + hs.p2Off[q], hs.p2A.begin() + hs.p2Off[q + 1], L.pA.begin() + L.pOff[e]);
-    std::copy(hs.p2Ia.begin() + hs.p2Off[q], hs.p2Ia.begin() + hs.p2Off[q + 1], L.pIa.begin() + L.pOff[e]);
-    std::copy(hs.p2Ib.begin() + hs.p2Off[q], hs.p2Ib.begin() + hs.p2Off[q + 1], L.pIb.begin() + L.pOff[e]);
-  }
+  parallel_ranges(total, nt, [&](int64_t e0, int64_t e1, int) {
+    for (int64_t e = e0; e < e1; e++) {
+      int64_t q = src[e];
+      if (q < 0) continue;
+      std::copy(hs.p2K.begin() + hs.p2Off[q], hs.p2K.begin() + hs.p2Off[q + 1], L.pK.begin() + L.pOff[e]);
+      std::copy(hs.p2A.begin() + hs.p2Off[q], hs.p2A.begin() + hs.p2Off[q + 1], L.pA.begin() + L.pOff[e]);
+      std::copy(hs.p2Ia.begin() + hs.p2Off[q], hs.p2Ia.begin() + hs.p2Off[q + 1], L.pIa.begin() + L.pOff[e]);
+      std::copy(hs.p2Ib.begin() + hs.p2Off[q], hs.p2Ib.begin() + hs.p2Off[q + 1], L.pIb.begin() + L.pOff[e]);
+    }
+  });
 }
 
 }  // namespace pdipc
