@@ -48,6 +48,64 @@ def timed_steps(sim, steps):
     return time.perf_counter() - t0, outer
 
 
+def stacked_bench(args):
+    """The scale run: three stacked 1024^2 cloths (3,145,728 vertices) over the
+    sphere and the ground, FP32-split (FP64 CCD and barrier), `args.steps`
+    steps from rest; reports steps/s, the candidate records K and patch pairs
+    Q of the broad phases, and the FP64 triangle-triangle intersection count
+    before and after (must stay 0)."""
+    sc = scenes.config4()
+    sizes = scenes.scene_sizes(sc)
+    t0 = time.perf_counter()
+    sim = pdipc.Simulator.from_scene(sc, precision=pdipc.FP32_SPLIT, sell_lanes=args.sell_lanes,
+                                     use_graphs=args.graphs, skip_static_pairs=1)
+    setup_s = time.perf_counter() - t0
+    c0 = sim.counters()
+    log(f"setup {setup_s:.1f} s; {sizes}; nnz_r2 {c0['nnz_r2']}, 2-paths {c0['n_2paths']}")
+    n_start = sim.count_intersections()
+    log(f"intersections at start: {n_start}")
+    secs, outer, ks, qs = 0.0, 0, [], []
+    for k in range(args.steps):
+        t1 = time.perf_counter()
+        sim.step(1)
+        secs += time.perf_counter() - t1
+        c = sim.counters()
+        outer += c["outer_iters"]
+        ks.append(c["candidates"])
+        qs.append(c["patch_pairs"])
+        if k % 5 == 4 or k == args.steps - 1:
+            log(f"step {k + 1}: {secs / (k + 1) * 1000:.0f} ms/step, outer {c['outer_iters']}, "
+                f"K {c['candidates']}, Q {c['patch_pairs']}, active {c['active_pairs']}")
+    n_end = sim.count_intersections()
+    log(f"intersections at end: {n_end}")
+    return {
+        "metric": "PD-IPC steps/s, scale run (3 stacked 1024^2 cloths)",
+        "value": args.steps / secs,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": 0,
+        "ms_per_step": 1000.0 * secs / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32+f64",
+        "data": "synthetic",
+        "config": {
+            "workload": "3 stacked 1024x1024 cloths (2 m, 5 mm apart, seeded shear |s| <= 0.1) over icosphere "
+                        "subdiv 6 + ground, dt=1/100, 10 LG x 15 Chebyshev, dHat=1e-3, kappa=1e3, from rest",
+            "precision": "FP32-split (FP64 CCD/barrier)",
+            "sim_points": sizes["sim_points"], "triangles": sizes["surface_triangles"],
+            "nnz_r2": c0["nnz_r2"], "n_2paths": c0["n_2paths"], "setup_s": setup_s,
+            "outer_iters_per_step": outer / args.steps,
+            "candidates_K": {"first": ks[0], "last": ks[-1], "max": max(ks)},
+            "patch_pairs_Q": {"first": qs[0], "last": qs[-1], "max": max(qs)},
+            "intersections_fp64": {"start": n_start, "end": n_end},
+            "graphs": bool(args.graphs),
+        },
+    }
+
+
 def _host_threads():
     n = os.environ.get("OMP_NUM_THREADS", "")
     if n.isdigit() and int(n) > 0:
@@ -158,9 +216,10 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--scene", choices=("cloth", "tet_block"), default="cloth",
+    ap.add_argument("--scene", choices=("cloth", "tet_block", "stacked"), default="cloth",
                     help="cloth: the n x n cloth over the sphere (default); tet_block: the 40^3-cell block "
-                         "(320,000 tets) under a 128x128 cloth, dt = 1/200, 15 local/global iterations")
+                         "(320,000 tets) under a 128x128 cloth, dt = 1/200, 15 local/global iterations; "
+                         "stacked: the scale run, 3 stacked 1024^2 cloths, FP32-split only, --steps steps")
     ap.add_argument("--n", type=int, default=256, help="cloth vertices per side")
     ap.add_argument("--subdiv", type=int, default=6)
     ap.add_argument("--no-cpu", action="store_true")
@@ -189,6 +248,9 @@ def main():
         print(f"bench.py: this build runs the PD-IPC step on one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)
         sys.exit(2)
+    if args.scene == "stacked":
+        print(json.dumps(stacked_bench(args)), flush=True)
+        return
     if args.only_adjoint:
         print(json.dumps(adjoint_bench(args.graphs, args.adjoint_fd)), flush=True)
         return

@@ -145,6 +145,12 @@ int pdipc_debug_collision(pdipc_handle* h, const double* x, double* diag, double
 int pdipc_debug_local(pdipc_handle* h, const double* x, const double* init_rhs, double* rhs);
 /* updateSuperJacobiOperatorsHost + chebyshevJacobiR2SolverHost (PdIpcGpuFunc.cu:976-1346) */
 int pdipc_debug_solve(pdipc_handle* h, const double* diag, const double* rhs, const double* x0, double* xout);
+/* Validation, no counterpart in the reference: the number of FP64
+ * edge-triangle crossings (triangle-triangle intersections) at the current
+ * positions, over every edge/face pair whose patches' boxes overlap (and
+ * within each patch), pairs sharing a vertex excluded.  Runs a broad phase
+ * at the current positions (the candidate lists then describe it). */
+int pdipc_count_intersections(pdipc_handle* h, long long* count);
 /* LBVH of the last broad phase: sorted 64-bit keys (P), internal nodes
  * (left, right child as node ids: leaves are P-1+leaf) and parents (2P-1) */
 int pdipc_debug_lbvh(pdipc_handle* h, unsigned long long* keys, int* left, int* right, int* parent, int* npatches);

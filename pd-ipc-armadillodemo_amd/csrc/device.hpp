@@ -122,6 +122,7 @@ struct DeviceMesh {
   // records ([1]: point-triangle records)
   int* counters = nullptr;
   int nPartials = 0;
+  unsigned long long* checkCount = nullptr;  // the zero-intersection check's counter
 };
 
 // ---- kernel launch wrappers (all asynchronous on `s`) -------------------------
@@ -217,6 +218,9 @@ void launch_traverse_top(const DeviceMesh& m, int skipStatic, hipStream_t s);
 void launch_traverse_sub(const DeviceMesh& m, int skipStatic, int nItems, hipStream_t s);
 void launch_pairs_compact(const DeviceMesh& m, int nItems, int nTopPairs, hipStream_t s);
 void launch_rec_count(const DeviceMesh& m, int nPairs, hipStream_t s);
+// adds to *count the FP64 edge-triangle crossings at m.X over the last pair
+// list (both ways) and within each patch (validation)
+void launch_count_crossings(const DeviceMesh& m, int nPairs, unsigned long long* count, hipStream_t s);
 void launch_rec_fill(const DeviceMesh& m, int nPairs, int nPT, int nRecs, hipStream_t s);
 void launch_exclusive_scan(int* data_in, int* data_out, int n, int* total_out, int* blockScratch, hipStream_t s);
 

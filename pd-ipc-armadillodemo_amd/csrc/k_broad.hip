@@ -702,6 +702,78 @@ __global__ void __launch_bounds__(kBlock) k_rec_fill(int Q, int nPT, int K, cons
   recs[r] = out;
 }
 
+// ---------------- zero-intersection check (validation, no counterpart in the reference) ----------------
+// A triangle pair intersects (non-coplanar) iff an edge of one crosses the
+// other, so the check counts edge-triangle crossings, in FP64, over every
+// (edge, face) of overlapping patch pairs (both ways) and within each patch,
+// skipping pairs that share a vertex.  orient3 > 0: d above the plane (a, b, c).
+__device__ inline double orient3(const double* a, const double* b, const double* c, const double* d) {
+  double u[3] = {b[0] - a[0], b[1] - a[1], b[2] - a[2]};
+  double v[3] = {c[0] - a[0], c[1] - a[1], c[2] - a[2]};
+  double w[3] = {d[0] - a[0], d[1] - a[1], d[2] - a[2]};
+  return u[0] * (v[1] * w[2] - v[2] * w[1]) + u[1] * (v[2] * w[0] - v[0] * w[2]) + u[2] * (v[0] * w[1] - v[1] * w[0]);
+}
+
+// the segment (p, q) meets the triangle (a, b, c) in one point (coplanar
+// configurations do not count)
+__device__ inline bool edge_crosses(const double* p, const double* q, const double* a, const double* b,
+                                    const double* c) {
+  double s1 = orient3(a, b, c, p), s2 = orient3(a, b, c, q);
+  if ((s1 > 0 && s2 > 0) || (s1 < 0 && s2 < 0) || (s1 == 0 && s2 == 0)) return false;
+  double t1 = orient3(p, q, a, b), t2 = orient3(p, q, b, c), t3 = orient3(p, q, c, a);
+  bool pos = t1 >= 0 && t2 >= 0 && t3 >= 0, neg = t1 <= 0 && t2 <= 0 && t3 <= 0;
+  return (pos || neg) && !(t1 == 0 && t2 == 0 && t3 == 0);
+}
+
+// blocks [0, Q): patch pairs; blocks [Q, Q + P): single patches; one wavefront each
+__global__ void __launch_bounds__(kWave) k_count_crossings(int Q, int P, int ld, const int2* __restrict__ pairs,
+                                                           const int* __restrict__ pfOff, const int* __restrict__ pfList,
+                                                           const int* __restrict__ peOff, const int* __restrict__ peList,
+                                                           const int3* __restrict__ faces,
+                                                           const int2* __restrict__ edges, const double* __restrict__ X,
+                                                           unsigned long long* __restrict__ count) {
+  int b = blockIdx.x;
+  int A, B;
+  if (b < Q) {
+    int2 pr = pairs[b];
+    A = pr.x;
+    B = pr.y;
+  } else {
+    A = B = b - Q;
+    if (A >= P) return;
+  }
+  int ea = peOff[A + 1] - peOff[A], fa = pfOff[A + 1] - pfOff[A];
+  int eb = peOff[B + 1] - peOff[B], fb = pfOff[B + 1] - pfOff[B];
+  int n1 = ea * fb, n = A == B ? n1 : n1 + eb * fa;
+  unsigned long long hits = 0;
+  for (int c = threadIdx.x; c < n; c += kWave) {
+    int e, f;
+    if (c < n1) {
+      e = peList[peOff[A] + c / fb];
+      f = pfList[pfOff[B] + c % fb];
+    } else {
+      int d = c - n1;
+      e = peList[peOff[B] + d / fa];
+      f = pfList[pfOff[A] + d % fa];
+    }
+    int2 ed = edges[e];
+    int3 fc = faces[f];
+    if (ed.x == fc.x || ed.x == fc.y || ed.x == fc.z || ed.y == fc.x || ed.y == fc.y || ed.y == fc.z) continue;
+    double p[3], q[3], a[3], bb[3], cc[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      p[k] = X[k * ld + ed.x];
+      q[k] = X[k * ld + ed.y];
+      a[k] = X[k * ld + fc.x];
+      bb[k] = X[k * ld + fc.y];
+      cc[k] = X[k * ld + fc.z];
+    }
+    if (edge_crosses(p, q, a, bb, cc)) hits++;
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) hits += __shfl_xor(hits, o, kWave);
+  if (threadIdx.x == 0 && hits) atomicAdd(count, hits);
+}
+
 }  // namespace
 
 void launch_exclusive_scan(int* in, int* out, int n, int* total_out, int* blockScratch, hipStream_t s) {
@@ -785,6 +857,13 @@ void launch_pairs_compact(const DeviceMesh& m, int nItems, int nTopPairs, hipStr
     hipLaunchKernelGGL(k_compact_items, dim3((unsigned)((n + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nItems,
                        m.segS, m.cntS, m.offS, m.pairs + nTopPairs);
   }
+}
+
+void launch_count_crossings(const DeviceMesh& m, int nPairs, unsigned long long* count, hipStream_t s) {
+  int P = m.nPatches;
+  if (P <= 0) return;
+  hipLaunchKernelGGL(k_count_crossings, dim3((unsigned)(nPairs + P)), dim3(kWave), 0, s, nPairs, P, m.ld, m.pairs,
+                     m.pfOff, m.pfList, m.peOff, m.peList, m.bFaces, m.bEdges, m.X, count);
 }
 
 void launch_rec_count(const DeviceMesh& m, int nPairs, hipStream_t s) {

@@ -178,6 +178,7 @@ int upload_all(pdipc_handle* h) {
   AL(ptScan, (hs.nSim + 1 + 1023) / 1024 + 2);
   m.nPartials = (std::max(1, hs.nSim) + 255) / 256;
   AL(partials, m.nPartials);
+  AL(checkCount, 1);
   // scalars: residual (8), toiBits (8), counters (10 ints)
   void* sd = nullptr;
   HIPCHK(hipMalloc(&sd, 64));
@@ -740,6 +741,24 @@ int pdipc_debug_solve(pdipc_handle* h, const double* diag, const double* rhs, co
   launch_solve(m, h->prm.omega, h->prm.rho, h->prm.cheb_iters, h->prm.precision, h->stream);
   if ((rc = check_launch(h))) return rc;
   return get_soa(h, m.X, xout, m.nSim);
+}
+
+int pdipc_count_intersections(pdipc_handle* h, long long* count) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (!count) return fail(PDIPC_E_ARG, "count is null");
+  DeviceMesh& m = h->m;
+  hipStream_t s = h->stream;
+  launch_zero(m.Dir, 3 * (int64_t)m.ld, s);  // boxes of the current positions, no sweep
+  if ((rc = broad_phase(h, m.X, m.Dir))) return rc;
+  HIPCHK(hipMemsetAsync(m.checkCount, 0, sizeof(unsigned long long), s));
+  launch_count_crossings(m, h->lastPairs, m.checkCount, s);
+  if ((rc = check_launch(h))) return rc;
+  unsigned long long c = 0;
+  HIPCHK(hipMemcpyAsync(&c, m.checkCount, sizeof(c), hipMemcpyDeviceToHost, s));
+  HIPCHK(hipStreamSynchronize(s));
+  *count = (long long)c;
+  return PDIPC_OK;
 }
 
 int pdipc_debug_lbvh(pdipc_handle* h, unsigned long long* keys, int* left, int* right, int* parent, int* npatches) {

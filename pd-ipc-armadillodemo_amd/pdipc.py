@@ -77,6 +77,7 @@ SIGNATURES = {
     "pdipc_debug_local": (ctypes.c_int, [_H, _D, _D, _D]),
     "pdipc_debug_solve": (ctypes.c_int, [_H, _D, _D, _D, _D]),
     "pdipc_debug_lbvh": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_ulonglong), _I, _I, _I, _I]),
+    "pdipc_count_intersections": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_longlong)]),
     "pdipc_time_kernel": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _D, _D]),
     "pdipc_set_cloth_stiffness": (ctypes.c_int, [_H, ctypes.c_int, _D, ctypes.c_int]),
     "pdipc_rollout": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int]),
@@ -365,6 +366,13 @@ class Simulator:
         out = np.zeros(4)
         _check(lib().pdipc_adjoint_stats(self.h, _dp(out)))
         return dict(outer_iters=int(out[0]), tape_bytes=float(out[1]), segments=int(out[2]), steps=int(out[3]))
+
+    def count_intersections(self):
+        """FP64 edge-triangle crossings at the current positions (0: no two
+        triangles intersect)."""
+        n = ctypes.c_longlong()
+        _check(lib().pdipc_count_intersections(self.h, ctypes.byref(n)))
+        return n.value
 
     def debug_lbvh(self):
         n = ctypes.c_int()

@@ -241,6 +241,28 @@ def config3(seed=0, n_cloth=128, steps=50):
     return s, v0, target
 
 
+def config4(seed=0, n=1024, layers=3, spacing=5e-3, side=2.0):
+    """The scale test: `layers` stacked n x n cloths (3 x 1024^2 = 3,145,728
+    vertices, 6,279,174 triangles), `spacing` apart, each with a seeded
+    in-plane shear x += s z, |s| <= 0.1, over the subdivision-6 sphere and a
+    3 m ground plane.  The side is 2 m so the grid spacing (1.96 mm) is that of
+    the 256^2 cloth on 0.5 m (an edge shorter than dHat = 1 mm would put
+    barrier pairs inside each sheet); the out-of-plane jitter is 1 mm, so
+    adjacent layers start at least 3 mm apart."""
+    s = make_scene(dict(kappa=1e3))
+    rng = np.random.Generator(np.random.PCG64(7_000_003 + seed))
+    for k in range(layers):
+        v, f = cloth_grid(n, side=side, height=0.2 + k * spacing, jitter=1e-3, seed=seed + 101 * k)
+        shear = rng.uniform(-0.1, 0.1)
+        v[:, 0] += shear * v[:, 2]
+        add_cloth(s, v, f, strain=1e4, bend=1e-2, pos=1e12, density=0.2, patch_dim=n)
+    sv, sf = icosphere(6, radius=0.15, noise=0.015, seed=seed)
+    add_static(s, sv, sf, patch_dim=256)
+    gv, gf = ground_plane(y=-0.2, half=1.5)
+    add_static(s, gv, gf, patch_dim=2)
+    return s
+
+
 def scene_sizes(scene):
     nv = sum(len(c["v"]) for c in scene["cloths"]) + sum(len(t["v"]) for t in scene["tets"])
     nf = sum(len(c["f"]) for c in scene["cloths"]) + sum(len(t["f"]) for t in scene["statics"])
