@@ -125,27 +125,31 @@ class Oracle:
         self.dual = bool(dual)
         self.h = L.orc_create(_dp(prm), len(prm), 2 if dual else (1 if use_float else 0))
         self._keep = []
-        mi = 0
-        for c in scene["cloths"]:
-            v, f, fx = _f64(c["v"]), _i32(c["f"]), _i32(c["fixed"])
-            self._keep += [v, f, fx]
-            L.orc_add_cloth(self.h, _dp(v), len(v), _ip(f), len(f), c["s0"], c["s1"], c["s2"], c["density"],
-                            int(c["patch_dim"]), _ip(fx), len(fx))
-            if c.get("tri_k") is not None:
-                k = _f64(c["tri_k"])
-                tg = None
-                if tri_k_tangents is not None and mi in tri_k_tangents:
-                    tg = _f64(tri_k_tangents[mi])
-                    assert tg.shape == (dual_dirs(), len(f))
-                if L.orc_set_tri_stiffness(self.h, mi, _dp(k), len(k), _dp(tg) if tg is not None else None) != 0:
-                    raise ValueError("per-triangle stiffness does not match the cloth")
-            mi += 1
-        for t in scene["tets"]:
-            v, tt, fx = _f64(t["v"]), _i32(t["t"]), _i32(t["fixed"])
-            self._keep += [v, tt, fx]
-            L.orc_add_tet(self.h, _dp(v), len(v), _ip(tt), len(tt), t["s0"], t["s1"], t["s2"], t["density"],
-                          int(t["patch_dim"]), _ip(fx), len(fx))
-            mi += 1
+        order = scene.get("model_order")
+        if order is None:  # cloths then tets, unless the scene gives the add order
+            order = [("cloth", i) for i in range(len(scene["cloths"]))] + \
+                    [("tet", i) for i in range(len(scene["tets"]))]
+        for mi, (kind, i) in enumerate(order):
+            if kind == "cloth":
+                c = scene["cloths"][i]
+                v, f, fx = _f64(c["v"]), _i32(c["f"]), _i32(c["fixed"])
+                self._keep += [v, f, fx]
+                L.orc_add_cloth(self.h, _dp(v), len(v), _ip(f), len(f), c["s0"], c["s1"], c["s2"], c["density"],
+                                int(c["patch_dim"]), _ip(fx), len(fx))
+                if c.get("tri_k") is not None:
+                    k = _f64(c["tri_k"])
+                    tg = None
+                    if tri_k_tangents is not None and mi in tri_k_tangents:
+                        tg = _f64(tri_k_tangents[mi])
+                        assert tg.shape == (dual_dirs(), len(f))
+                    if L.orc_set_tri_stiffness(self.h, mi, _dp(k), len(k), _dp(tg) if tg is not None else None) != 0:
+                        raise ValueError("per-triangle stiffness does not match the cloth")
+            else:
+                t = scene["tets"][i]
+                v, tt, fx = _f64(t["v"]), _i32(t["t"]), _i32(t["fixed"])
+                self._keep += [v, tt, fx]
+                L.orc_add_tet(self.h, _dp(v), len(v), _ip(tt), len(tt), t["s0"], t["s1"], t["s2"], t["density"],
+                              int(t["patch_dim"]), _ip(fx), len(fx))
         for s in scene["statics"]:
             v, f = _f64(s["v"]), _i32(s["f"])
             self._keep += [v, f]

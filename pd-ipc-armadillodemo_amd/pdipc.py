@@ -134,6 +134,14 @@ def default_params(**overrides):
     return p
 
 
+def model_order(scene):
+    """The simulated models in add order: scene["model_order"] or cloths then tets."""
+    order = scene.get("model_order")
+    if order is None:
+        order = [("cloth", i) for i in range(len(scene["cloths"]))] + [("tet", i) for i in range(len(scene["tets"]))]
+    return list(order)
+
+
 def params_from_scene(scene, **extra):
     p = default_params()
     for k, v in scene["params"].items():
@@ -156,13 +164,19 @@ class Simulator:
 
     @classmethod
     def from_scene(cls, scene, init=True, **extra):
+        """Simulated models in scene["model_order"] (("cloth" | "tet", index)
+        pairs, the reference's add order) when given, else cloths then tets;
+        static models last."""
         s = cls(params_from_scene(scene, **extra))
-        for mi, c in enumerate(scene["cloths"]):
-            s.add_cloth(c["v"], c["f"], c["s0"], c["s1"], c["s2"], c["density"], c["patch_dim"], c["fixed"])
-            if c.get("tri_k") is not None:
-                s.set_cloth_stiffness(mi, c["tri_k"])
-        for t in scene["tets"]:
-            s.add_tet(t["v"], t["t"], t["s0"], t["s1"], t["s2"], t["density"], t["patch_dim"], t["fixed"])
+        for mi, (kind, i) in enumerate(model_order(scene)):
+            if kind == "cloth":
+                c = scene["cloths"][i]
+                s.add_cloth(c["v"], c["f"], c["s0"], c["s1"], c["s2"], c["density"], c["patch_dim"], c["fixed"])
+                if c.get("tri_k") is not None:
+                    s.set_cloth_stiffness(mi, c["tri_k"])
+            else:
+                t = scene["tets"][i]
+                s.add_tet(t["v"], t["t"], t["s0"], t["s1"], t["s2"], t["density"], t["patch_dim"], t["fixed"])
         for st in scene["statics"]:
             s.add_static(st["v"], st["f"], st["patch_dim"])
         if init:

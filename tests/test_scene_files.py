@@ -1,0 +1,275 @@
+"""The reader of the reference's scene files (pd-ipc-armadillodemo_amd/scene_files.py):
+transforms against closed forms (including the reference's bounding-box and
+std::unique quirks), the shipped demo's two cloths from the committed fixtures
+(their cached boundary.dat / patchInfo.dat reproduced through the reader and
+the host precompute), a tet object with its surface embedding, and the
+reference's model order (a tet object before a cloth) on the oracle and the GPU."""
+
+import os
+
+import numpy as np
+import pytest
+
+import oracle
+import pdipc
+import scene_files
+import scenes
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+F32 = np.float32
+
+
+def _write_obj(path, v, f):
+    with open(path, "w") as fh:
+        fh.write("# written by the test\no mesh\n")
+        for p in np.asarray(v, F32):
+            fh.write("v %.9g %.9g %.9g\n" % tuple(float(c) for c in p))
+        for t in f:
+            fh.write("f %d/1/1 %d/1/1 %d/1/1\n" % tuple(int(i) + 1 for i in t))
+
+
+def _write_tet(base, v, t):
+    with open(base + ".node", "w") as fh:
+        fh.write(f"{len(v)}  3  0  0\n")
+        for i, p in enumerate(np.asarray(v, F32)):
+            fh.write("   %d    %.9g  %.9g  %.9g\n" % ((i + 1,) + tuple(float(c) for c in p)))
+    with open(base + ".ele", "w") as fh:
+        fh.write(f"{len(t)}  4  0\n")
+        for i, r in enumerate(t):
+            fh.write("    %d    %d  %d  %d %d\n" % ((i + 1,) + tuple(int(x) + 1 for x in r)))
+
+
+def _object(name, type_line, patch=50, scale="1 1 1", trans="0 0 0", density=2500, static=False, **attrs):
+    tag = "static_object" if static else "object"
+    a = " ".join(f'{k} = "{v}"' for k, v in attrs.items())
+    return (f'<{tag} name = "{name}" nick_name = "{name}" {a}>\n<type> {type_line} </type>\n'
+            f"<patchDim> {patch} </patchDim>\n<scale> {scale} </scale>\n<translation> {trans} </translation>\n"
+            f"<rotation> 0 0 0 0</rotation>\n<density> {density} </density>\n</{tag}>\n")
+
+
+def _config(path, objects, trailing=""):
+    with open(os.path.join(path, "config.xml"), "w") as fh:
+        fh.write('<?xml version="1.0" ?>\n<SimFramework name = "pd_ipc_simulator" run = "CUDA">\n<simulator>\n'
+                 "<scene>\n<startFrame> 0 </startFrame>\n</scene>\n<timeStep> 0.01 </timeStep>\n"
+                 "<gravity> -9.8 </gravity>\n<dhat>5e-3</dhat>\n<tol> 1e-2 </tol>\n<kappa>5e7</kappa>\n"
+                 + "".join(objects) + "</simulator>\n</SimFramework>\n" + trailing)
+
+
+def test_bbox_quirk_and_fixed_ids(tmp_path):
+    # the first point lowers every minimum, so it is never compared with the maxima
+    pts = np.array([[5, 5, 5], [0, 1, 2], [3, 4, 4]], F32)
+    lo, hi = scene_files.bbox(pts)
+    assert lo.tolist() == [0, 1, 2] and hi.tolist() == [3, 4, 4]
+    # a strictly decreasing axis never sets its maximum
+    lo, hi = scene_files.bbox(np.array([[2, 0, 0], [1, 1, 1], [0, 2, 2]], F32))
+    assert hi[0] == -np.finfo(F32).max and lo[0] == 0
+    p = tmp_path / "fixed_points.txt"
+    p.write_text("c 6\n4\n4\n7\n7\n7\n2\n")
+    # std::unique moves [4,7,2] to the front and leaves the old tail
+    assert scene_files.read_fixed(str(p)).tolist() == [4, 7, 2, 7, 7, 2]
+
+
+def test_transforms_closed_form():
+    # a box spanning [0,2] x [0,1] x [0,4]; its first point is not extreme
+    v = np.array([[1, 0.5, 2], [0, 0, 0], [2, 1, 4], [2, 0, 0], [0, 1, 4]], F32)
+    s, t = scene_files.transform(v, v[::-1].copy(), uniform=True, origin=True, scale=[2, 2, 2],
+                                 translation=[1, 2, 3], rotation=[0, 0, 0, 0])
+    # origin: minus (1, 0.5, 2); uniform: / 4; scale 2 about the (new) centre 0; translate
+    want = (v - np.array([1, 0.5, 2], F32)) / 4 * 2 + np.array([1, 2, 3], F32)
+    assert np.array_equal(s, want.astype(F32))
+    assert np.array_equal(t, want[::-1].astype(F32))
+    # flags off: only the configured scale (about the box centre (1, 0.5, 2)) and translation
+    s, _ = scene_files.transform(v, None, uniform=False, origin=False, scale=[1, 3, 1],
+                                 translation=[0, 0, 0], rotation=[0, 0, 0, 0])
+    assert np.array_equal(s[:, 1], ((v[:, 1] - F32(0.5)) * F32(3) + F32(0.5)).astype(F32))
+    assert np.array_equal(s[:, [0, 2]], v[:, [0, 2]])
+
+
+def test_demo_cloths_reproduce_the_reference_caches(tmp_path):
+    """The demo's cloth objects (config.xml:14-29 of the shipped example:
+    type 0 1e6 25 1e12, patchDim 50, uniform / origin off) from the committed
+    fixtures; the reader's meshes equal the reference loader's and the host
+    precompute on them reproduces the reference's cached .dat files."""
+    g = np.load(os.path.join(HERE, "golden", "cloth_armadillo_fixtures.npz"))
+    objs = []
+    for name in ("cloth1", "cloth2"):
+        d = tmp_path / name
+        d.mkdir()
+        _write_obj(str(d / f"{name}.obj"), g[name + "_v"], g[name + "_f"])
+        fx = g[name + "_fixed"]
+        (d / "fixed_points.txt").write_text(f"c {len(fx)}\n" + "\n".join(str(int(i)) for i in fx) + "\n")
+        g[name + "_boundary"].astype(np.int32).tofile(str(d / "boundary.dat"))
+        g[name + "_patch"].astype(np.int32).tofile(str(d / "patchInfo.dat"))
+        objs.append(_object(name, "0 1e6 25 1e12", uniform="false", origin="false"))
+    _config(str(tmp_path), objs, trailing="\n\n" + _object("hit", "0 1e3 1e-5 1e12", static=True))
+    sc = scene_files.load_config(str(tmp_path))
+    assert sc["params"]["dt"] == 0.01 and sc["params"]["dhat"] == 5e-3 and sc["params"]["kappa"] == 5e7
+    assert len(sc["cloths"]) == 2 and not sc["statics"]  # the static object after </SimFramework> is not read
+    for i, name in enumerate(("cloth1", "cloth2")):
+        c = sc["cloths"][i]
+        # scaleModel(1, 1, 1) still maps p -> (p - c) * 1 + c in float32, which
+        # rounds: the reference's points sit within an ulp or two of the OBJ's
+        raw = g[name + "_v"]
+        want, _ = scene_files.transform(raw, None, False, False, [1, 1, 1], [0, 0, 0], [0, 0, 0, 0])
+        assert np.array_equal(c["v"], want.astype(np.float64))
+        ulp = np.spacing(np.abs(raw - scene_files.center(raw)).astype(F32))
+        assert (np.abs(want - raw) <= 2 * ulp).all()
+        assert np.array_equal(c["f"], g[name + "_f"])
+        assert np.array_equal(c["fixed"], g[name + "_fixed"])
+        assert (c["s0"], c["s1"], c["s2"], c["density"], c["patch_dim"]) == (1e6, 25.0, 1e12, 2500.0, 50)
+    sim = pdipc.Simulator.from_scene(sc, init=False)
+    sim.host_setup()
+    m = scene_files.dat_cache_matches(sim, sc)
+    assert len(m) == 4 and all(m.values()), m
+
+
+def _tet_scene_dir(path, first):
+    """A 2x2x2-cell tet block (surface = its boundary) and a 6x6 cloth, the
+    tet object first, uniform / origin on for the block."""
+    tv, tt = scenes.tet_block(2, size=0.4, origin=(-0.2, -0.19, -0.2))
+    # boundary faces of the block as its surface mesh
+    faces = {}
+    for t in tt:
+        for a, b, c in ((0, 1, 2), (0, 1, 3), (0, 2, 3), (1, 2, 3)):
+            key = tuple(sorted((t[a], t[b], t[c])))
+            faces[key] = faces.get(key, 0) + 1
+    sf = np.array([k for k, n in faces.items() if n == 1], np.int32)
+    d = path / "block"
+    d.mkdir()
+    _write_obj(str(d / "block.obj"), tv, sf)
+    _write_tet(str(d / "block"), tv, tt)
+    (d / "fixed_points.txt").write_text("c 2\n0\n3\n")
+    # every surface vertex is a tet node: embed it as a one-hot corner of its first tet
+    lines = [str(len(tv))]
+    for i in range(len(tv)):
+        e = int(np.nonzero((tt == i).any(1))[0][0])
+        w = (tt[e] == i).astype(int)
+        lines.append(f"{i} {e} {w[0]} {w[1]} {w[2]} {w[3]} -1")
+    (d / "tetMeshInterpolation.txt").write_text("\n".join(lines) + "\n")
+    cv, cf = scenes.cloth_grid(6, side=0.3, height=0.25, jitter=1e-3, seed=4)
+    c = path / "sheet"
+    c.mkdir()
+    _write_obj(str(c / "sheet.obj"), cv, cf)
+    block = _object("block", "1 71428.6 1e3 1e12", patch=4, scale="0.5 0.5 0.5", trans="0 -0.05 0",
+                    density=1000, volumetric_mesh="tet", uniform="true", origin="true")
+    sheet = _object("sheet", "0 1e4 1e-2 1e12", patch=6, density=0.2, uniform="false", origin="false")
+    _config(str(path), [block, sheet] if first == "tet" else [sheet, block])
+    return tv, tt, sf
+
+
+def test_tet_object_and_surface_embedding(tmp_path):
+    tv, tt, sf = _tet_scene_dir(tmp_path, "tet")
+    sc = scene_files.load_config(str(tmp_path))
+    assert sc["model_order"] == [("tet", 0), ("cloth", 0)]
+    t = sc["tets"][0]
+    s_want, t_want = scene_files.transform(tv.astype(F32), tv.astype(F32), True, True, [0.5] * 3, [0, -0.05, 0],
+                                           [0, 0, 0, 0])
+    assert np.array_equal(t["v"], t_want.astype(np.float64))
+    assert np.array_equal(t["t"], tt) and t["fixed"].tolist() == [0, 3]
+    # the block spans 0.4 in every axis: after origin + uniform + scale 0.5 it spans 0.5
+    ext = t["v"].max(0) - t["v"].min(0)
+    assert np.allclose(ext, 0.5, atol=1e-6)
+    surf = scene_files.surface_positions(sc, 0, t["v"])
+    assert np.array_equal(surf, t["v"].astype(F32))
+
+
+@pytest.mark.timeout(300)
+def test_model_order_tet_first_oracle(tmp_path):
+    """The reference's add order (the tet object first) is a permutation of the
+    points: the oracle gives the same motion as with the cloth first."""
+    _tet_scene_dir(tmp_path, "tet")
+    a = scene_files.load_config(str(tmp_path), lg_iters=5)
+    b = dict(a, model_order=[("cloth", 0), ("tet", 0)])
+    oa, ob = oracle.Oracle(a), oracle.Oracle(b)
+    oa.step(3)
+    ob.step(3)
+    nt, nc = len(a["tets"][0]["v"]), len(a["cloths"][0]["v"])
+    xa, xb = oa.x(), ob.x()
+    assert np.abs(xa[:nt] - xb[nc:nc + nt]).max() < 1e-12
+    assert np.abs(xa[nt:nt + nc] - xb[:nc]).max() < 1e-12
+
+
+@pytest.mark.gpu
+def test_model_order_tet_first_gpu(tmp_path):
+    _tet_scene_dir(tmp_path, "tet")
+    sc = scene_files.load_config(str(tmp_path), lg_iters=5)
+    print("case: loaded tet-first scene, 3 FP64 steps vs the oracle", flush=True)
+    ref = oracle.Oracle(sc)
+    sim = pdipc.Simulator.from_scene(sc)
+    for _ in range(3):
+        ref.step(1)
+        sim.step(1)
+    rel = np.abs(sim.x() - ref.x()).max() / np.abs(ref.x()).max()
+    assert rel < 1e-8
+
+
+def _demo_dir(path):
+    """The shipped ClothArmadillo demo directory, rebuilt from the committed
+    fixtures in the reference's file formats."""
+    g = np.load(os.path.join(HERE, "golden", "cloth_armadillo_fixtures.npz"))
+    r = np.load(os.path.join(HERE, "golden", "cloth_armadillo_demo_rest.npz"))
+    (path / "config.xml").write_bytes(r["config_xml"].tobytes())
+    for name in ("cloth1", "cloth2"):
+        d = path / name
+        d.mkdir()
+        _write_obj(str(d / f"{name}.obj"), g[name + "_v"], g[name + "_f"])
+        fx = g[name + "_fixed"]
+        (d / "fixed_points.txt").write_text(f"c {len(fx)}\n" + "\n".join(str(int(i)) for i in fx) + "\n")
+        g[name + "_boundary"].astype(np.int32).tofile(str(d / "boundary.dat"))
+        g[name + "_patch"].astype(np.int32).tofile(str(d / "patchInfo.dat"))
+    d = path / "armadillo"
+    d.mkdir()
+    _write_obj(str(d / "armadillo.obj"), r["armadillo_v"], r["armadillo_f"])
+    _write_tet(str(d / "armadillo"), r["armadillo_nodes"], r["armadillo_tets"])
+    fx = r["armadillo_fixed"]
+    (d / "fixed_points.txt").write_text(f"c {len(fx)}\n" + "\n".join(str(int(i)) for i in fx) + "\n")
+    lines = [str(len(r["armadillo_interp_eid"]))]
+    for i, (e, w) in enumerate(zip(r["armadillo_interp_eid"], r["armadillo_interp_w"])):
+        lines.append(f"{i} {int(e)} " + " ".join("%.9g" % float(x) for x in w) + " -1")
+    (d / "tetMeshInterpolation.txt").write_text("\n".join(lines) + "\n")
+    return g, r
+
+
+def test_shipped_demo_loads(tmp_path):
+    g, r = _demo_dir(tmp_path)
+    sc = scene_files.load_config(str(tmp_path))
+    assert sc["model_order"] == [("cloth", 0), ("cloth", 1), ("tet", 0)]
+    assert (sc["params"]["dt"], sc["params"]["gravity"], sc["params"]["dhat"], sc["params"]["kappa"]) == \
+        (0.01, -9.8, 5e-3, 5e7)
+    t = sc["tets"][0]
+    assert len(t["v"]) == 14779 and len(t["t"]) == 54855 and len(t["fixed"]) == 5309
+    assert (t["s0"], t["s1"], t["s2"], t["density"], t["patch_dim"]) == (2e6, 1e3, 1e12, 1000.0, 50)
+    # origin + uniform + scale 3.5 + translation (1.5, 2.3, 1): the surface's
+    # largest extent becomes 3.5 and its box centre the translation
+    s = sc["surfaces"][0]["v"].astype(np.float64)
+    ext = s.max(0) - s.min(0)
+    assert abs(ext.max() - 3.5) < 1e-5
+    assert np.abs((s.max(0) + s.min(0)) / 2 - np.array([1.5, 2.3, 1.0])).max() < 1e-5
+    sim = pdipc.Simulator.from_scene(sc, init=False)
+    sim.host_setup()
+    m = scene_files.dat_cache_matches(sim, sc)
+    assert len(m) == 4 and all(m.values()), m
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(600)
+def test_shipped_demo_steps(tmp_path):
+    """The shipped ClothArmadillo scene (two cloths on the armadillo, 26,029
+    points, 54,855 tets) through the reader: two FP64 steps against the oracle."""
+    from parity_util import check_step
+
+    _demo_dir(tmp_path)
+    sc = scene_files.load_config(str(tmp_path))
+    print("case: shipped ClothArmadillo demo, 2 FP64 steps vs the oracle", flush=True)
+    ref = oracle.Oracle(sc)
+    sim = pdipc.Simulator.from_scene(sc)
+    for k in range(2):
+        x, v = ref.x(), ref.v()
+        sim.set_x(x)
+        sim.set_v(v)
+        ref.step(1)
+        sim.step(1)
+        err, sens = check_step(sc, x, v, sim.x(), ref.x(), ref.sim, seed=k)
+        c, o = sim.counters(), ref.counters()
+        assert c["candidates"] == o["candidates"] and c["outer_iters"] == o["outer"]
+        print("demo step", k, "error", err, "sensitivity", sens, "active", o["active_pairs"])
