@@ -151,9 +151,11 @@ def cpu_baseline(scene, x, v, seconds_budget=20.0):
 def adjoint_bench(graphs, fd_dirs, log_every=8):
     """The differentiable run: 128 x 128 cloth over the config-0 obstacle, 50 FP64
     steps, loss |x_T - target|^2, gradient w.r.t. per-triangle stiffness and
-    initial velocity by the library's adjoint with checkpoints every 10 steps;
-    timed forward (rollout) and backward, then central differences (eps 1e-6)
-    on `fd_dirs` random directions."""
+    initial velocity by the library's adjoint, in its two modes: the rollout
+    keeps the whole run's tape in HBM (checkpoint_every = 0; the backward runs
+    no forward step), or keeps (x, v) every 10 steps (the backward re-runs each
+    segment recording it).  Timed forward (rollout) and backward of each, then
+    central differences (eps 1e-6) on `fd_dirs` random directions."""
     steps, every = 50, 10
     sc, v0, target = scenes.config3(0, n_cloth=128, steps=steps)
     sim = pdipc.Simulator.from_scene(sc, use_graphs=graphs)
@@ -161,49 +163,59 @@ def adjoint_bench(graphs, fd_dirs, log_every=8):
     k0 = sc["cloths"][0]["tri_k"].copy()
     ntri = len(k0)
 
-    def forward(k, v):
+    def forward(k, v, ck):
         sim.set_cloth_stiffness(0, k)
         sim.set_x(x0)
         sim.set_v(v)
         t0 = time.perf_counter()
-        sim.rollout(steps, every)
+        sim.rollout(steps, ck)
         t = time.perf_counter() - t0
         x = sim.x()[: sim.sim]
         return float(np.sum((x - target) ** 2)), x, t
 
-    forward(k0, v0)  # warm: allocations, graphs
-    _, xT, _ = forward(k0, v0)
-    sim.backward(dx=2.0 * (xT - target), n_tris=ntri)
-    fw = []
-    bw = []
-    for _ in range(2):
-        loss, xT, tf = forward(k0, v0)
-        t0 = time.perf_counter()
-        _, dv0, dk = sim.backward(dx=2.0 * (xT - target), n_tris=ntri)
-        bw.append(time.perf_counter() - t0)
-        fw.append(tf)
-    stats = sim.adjoint_stats()
-    tf, tb = min(fw), min(bw)
-    log(f"adjoint: forward {tf:.3f} s, backward {tb:.3f} s (ratio {tb / tf:.2f}); {stats}")
+    res = {}
+    grads = {}
+    for mode, ck in (("recorded", 0), ("checkpointed", every)):
+        forward(k0, v0, ck)  # warm: allocations, graphs
+        _, xT, _ = forward(k0, v0, ck)
+        sim.backward(dx=2.0 * (xT - target), n_tris=ntri)
+        fw, bw = [], []
+        for _ in range(2):
+            loss, xT, tf = forward(k0, v0, ck)
+            t0 = time.perf_counter()
+            _, dv0, dk = sim.backward(dx=2.0 * (xT - target), n_tris=ntri)
+            bw.append(time.perf_counter() - t0)
+            fw.append(tf)
+        stats = sim.adjoint_stats()
+        tf, tb = min(fw), min(bw)
+        grads[mode] = (dk, dv0)
+        res[mode] = {"checkpoint_every": ck, "forward_s": tf, "backward_s": tb, "backward_fwd_ratio": tb / tf,
+                     "tape_bytes": stats["tape_bytes"], "recorded_outer_iters": stats["outer_iters"]}
+        log(f"adjoint ({mode}): forward {tf:.3f} s, backward {tb:.3f} s (ratio {tb / tf:.2f}); {stats}")
+    (dk, dv0), (dkc, dvc) = grads["recorded"], grads["checkpointed"]
+    same = max(float(np.abs(dk - dkc).max() / np.abs(dkc).max()), float(np.abs(dv0 - dvc).max() / np.abs(dvc).max()))
     rng = np.random.default_rng(12345)
     errs = []
     eps = 1e-6
     for q in range(fd_dirs):
         dkq = rng.normal(size=ntri) * k0
         dvq = rng.normal(size=v0.shape) * 0.05
-        lp, _, _ = forward(k0 + eps * dkq, v0 + eps * dvq)
-        lm, _, _ = forward(k0 - eps * dkq, v0 - eps * dvq)
+        lp, _, _ = forward(k0 + eps * dkq, v0 + eps * dvq, every)
+        lm, _, _ = forward(k0 - eps * dkq, v0 - eps * dvq, every)
         fd = (lp - lm) / (2 * eps)
         ad = float(dk @ dkq + np.sum(dv0 * dvq))
         errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
         if q % log_every == 0:
             log(f"  direction {q}: fd {fd:.6e} adjoint {ad:.6e} rel {errs[-1]:.2e}")
     errs = np.array(errs)
+    r, c = res["recorded"], res["checkpointed"]
     return {
         "workload": "cloth 128x128 (32,258 triangles) over icosphere subdiv 4 (config-0 obstacle), 50 FP64 steps, "
-                    "checkpoints every 10, loss |x_T - target|^2, d/d(per-triangle stiffness, v0)",
-        "forward_s": tf, "backward_s": tb, "adjoint_fwd_ratio": tb / tf,
-        "recorded_outer_iters": stats["outer_iters"], "peak_tape_bytes": stats["tape_bytes"],
+                    "loss |x_T - target|^2, d/d(per-triangle stiffness, v0)",
+        "mode": "whole-run tape kept in HBM (checkpoint_every = 0)",
+        "forward_s": r["forward_s"], "backward_s": r["backward_s"], "adjoint_fwd_ratio": r["backward_fwd_ratio"],
+        "gradient_cost_vs_checkpointed_forward": (r["forward_s"] + r["backward_s"]) / c["forward_s"],
+        "recorded": r, "checkpointed": c, "modes_max_rel_diff": same,
         "fd_directions": int(fd_dirs), "fd_eps": eps,
         "fd_rel_err_median": float(np.median(errs)) if fd_dirs else None,
         "fd_rel_err_max": float(errs.max()) if fd_dirs else None,

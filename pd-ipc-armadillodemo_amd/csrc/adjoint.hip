@@ -108,6 +108,7 @@ struct AdjointState {
   AdjMesh am;
   std::vector<void*> allocs;
   int nSteps = 0, every = 1;
+  bool recorded = false;  // the rollout kept the tape of every step (every == 0)
   std::vector<double*> ckX, ckV;
   double *endX = nullptr, *endV = nullptr;
   Pool tape;
@@ -460,13 +461,33 @@ extern "C" {
 int pdipc_rollout(pdipc_handle* h, int nsteps, int every) {
   int rc = require_device(h);
   if (rc) return rc;
-  if (nsteps < 1 || every < 1) return fail(PDIPC_E_ARG, "nsteps and checkpoint_every must be positive");
+  if (nsteps < 1 || every < 0) return fail(PDIPC_E_ARG, "nsteps must be positive, checkpoint_every >= 0");
   if (h->prm.cheb_iters > kMaxCheb) return fail(PDIPC_E_ARG, "the adjoint supports at most 32 Chebyshev sweeps");
   if (h->prm.precision != PDIPC_FP64) return fail(PDIPC_E_ARG, "the adjoint runs on FP64 steps");
   if ((rc = ensure_state(h))) return rc;
   AdjointState* A = h->adj;
   DeviceMesh& m = h->m;
   size_t V3 = 3 * (size_t)m.ld;
+  if (every == 0) {  // record the whole run: the backward reverses this tape directly
+    A->nSteps = nsteps;
+    A->every = 0;
+    A->recorded = false;
+    A->tape.reset();
+    A->stats[0] = 0;
+    A->seg.assign(nsteps, StepTape());
+    for (int n = 0; n < nsteps; n++)
+      if ((rc = taped_step(h, A->seg[n]))) return rc;
+    double used = 0;
+    for (size_t c = 0; c < A->tape.ci && c < A->tape.chunks.size(); c++) used += (double)A->tape.chunks[c].size;
+    used += (double)A->tape.off;
+    A->stats[1] = used;
+    A->recorded = true;
+    launch_copy(A->endX, m.X, (int64_t)V3, h->stream);
+    launch_copy(A->endV, m.V, (int64_t)V3, h->stream);
+    HIPCHK(hipStreamSynchronize(h->stream));
+    return check_launch(h);
+  }
+  A->recorded = false;
   int nck = (nsteps + every - 1) / every;
   while ((int)A->ckX.size() < nck) {
     double *x = nullptr, *v = nullptr;
@@ -509,10 +530,17 @@ int pdipc_backward(pdipc_handle* h, const double* dx, const double* dv, double* 
   launch_zero(A->R1bar, (int64_t)hs.r1Col.size(), s);
   launch_zero(A->DbarEl, m.nSim, s);
   double *xin = A->xbarA, *vin = A->vbarA, *xout = A->xbarB, *vout = A->vbarB;
-  A->stats[0] = 0;
   A->stats[2] = 0;
-  double tapeBytes = 0;
-  int nseg = (A->nSteps + A->every - 1) / A->every;
+  if (A->recorded) {  // the rollout's own tape, every step
+    for (int n = A->nSteps - 1; n >= 0; n--) {
+      if ((rc = reverse_step(h, A->seg[n], xin, vin, xout, vout))) return rc;
+      std::swap(xin, xout);
+      std::swap(vin, vout);
+    }
+  }
+  A->stats[0] = A->recorded ? A->stats[0] : 0;
+  double tapeBytes = A->recorded ? A->stats[1] : 0;
+  int nseg = A->recorded ? 0 : (A->nSteps + A->every - 1) / A->every;
   for (int sg = nseg - 1; sg >= 0; sg--) {
     int first = sg * A->every, last = std::min(A->nSteps, first + A->every);
     launch_copy(m.X, A->ckX[sg], (int64_t)V3, s);

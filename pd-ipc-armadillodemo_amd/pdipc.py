@@ -338,6 +338,8 @@ class Simulator:
         _check(lib().pdipc_set_cloth_stiffness(self.h, int(model), _dp(k), len(k)))
 
     def rollout(self, nsteps, checkpoint_every=10):
+        """`nsteps` FP64 steps for a later backward: (x, v) kept every
+        `checkpoint_every` steps, or (0) every step's full tape kept in HBM."""
         _check(lib().pdipc_rollout(self.h, int(nsteps), int(checkpoint_every)))
 
     def backward(self, dx=None, dv=None, n_tris=None):

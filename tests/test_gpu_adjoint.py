@@ -257,3 +257,30 @@ def test_adjoint_graph_replay_matches_direct_launches():
     assert np.array_equal(out[0][0], out[1][0])
     assert np.abs(out[0][1] - out[1][1]).max() <= 1e-7 * np.abs(out[0][1]).max()
     assert np.abs(out[0][2] - out[1][2]).max() <= 1e-7 * np.abs(out[0][2]).max()
+
+
+def test_recorded_rollout_matches_checkpoints():
+    """checkpoint_every = 0 keeps the rollout's own tape: the backward runs no
+    forward step, gives the checkpointed backward's gradient (the re-run
+    segments record the same trajectory) and can be repeated."""
+    steps = 9
+    sc, v0, target = _cloth_case(1, steps=steps)
+    k0 = sc["cloths"][0]["tri_k"].copy()
+    sim = pdipc.Simulator.from_scene(sc)
+    x0 = sim.x()[: sim.sim].copy()
+    out = {}
+    for every in (4, 0):
+        print(f"case: rollout with checkpoint_every = {every}", flush=True)
+        sim.set_cloth_stiffness(0, k0)
+        sim.set_x(x0)
+        sim.set_v(v0)
+        sim.rollout(steps, every)
+        xT = sim.x()[: sim.sim].copy()
+        out[every] = (xT, sim.backward(dx=2.0 * (xT - target), n_tris=len(k0)))
+    (xa, (dxa, dva, dka)), (xb, (dxb, dvb, dkb)) = out[4], out[0]
+    assert np.array_equal(xa, xb)
+    for a, b in ((dxa, dxb), (dva, dvb), (dka, dkb)):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
+    again = sim.backward(dx=2.0 * (xb - target), n_tris=len(k0))
+    for a, b in zip(again, (dxb, dvb, dkb)):
+        assert np.array_equal(a, b)
