@@ -218,6 +218,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
   int row = slice * S + lane / G;  // < ld: slices cover at most ld rows
   int64_t off = sliceOff[slice] + lane;
   int width = sliceW[slice];
+  Row64 p = pre[row], c = cA[row];  // loaded up front, with the matrix
   double acc[3] = {0.0, 0.0, 0.0};
   for (int c0 = 0; c0 < width; c0 += U) {
     int j[U];
@@ -242,7 +243,6 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
   }
   group_reduce<G>(acc);
   if ((lane % G) != 0 || row >= n) return;
-  Row64 p = pre[row], c = cA[row];
   double x0 = W * (acc[0] + c.x - p.x) + p.x;
   double x1 = W * (acc[1] + c.y - p.y) + p.y;
   double x2 = W * (acc[2] + c.z - p.z) + p.z;
@@ -285,6 +285,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
   int row = slice * S + lane / G;  // < ld: slices cover at most ld rows
   int64_t off = sliceOff[slice] + lane;
   int width = sliceW[slice];
+  // the row's own operands, loaded up front so they arrive with the matrix
+  float4 p = dpre[row], r = r0[row];
   float acc[3] = {0.f, 0.f, 0.f};
   for (int c0 = 0; c0 < width; c0 += U) {
     int j[U];
@@ -309,7 +311,6 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
   }
   group_reduce<G>(acc);
   if ((lane % G) != 0 || row >= n) return;
-  float4 p = dpre[row], r = r0[row];
   float d0 = W * (acc[0] + r.x - p.x) + p.x;
   float d1 = W * (acc[1] + r.y - p.y) + p.y;
   float d2 = W * (acc[2] + r.z - p.z) + p.z;
