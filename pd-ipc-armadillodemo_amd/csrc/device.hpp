@@ -135,10 +135,15 @@ void launch_velocity(const DeviceMesh& m, double dt, hipStream_t s);
 void launch_zero(double* p, int64_t n, hipStream_t s);
 void launch_copy(double* dst, const double* src, int64_t n, hipStream_t s);
 // k_local.hip
-void launch_local(const DeviceMesh& m, int precision, double sigLo, double sigHi, int withCollision, hipStream_t s);
+// residualOmega >= 0 (FP32-split): also the first pass of the solve's FP64
+// residual (into xa0 / cA, for the current X and D); the solve is then
+// launched with residualReady
+void launch_local(const DeviceMesh& m, int precision, double sigLo, double sigHi, int withCollision, hipStream_t s,
+                  double residualOmega = -1.0);
 // k_solve.hip
 void launch_operators(const DeviceMesh& m, double omega, int precision, hipStream_t s);
-void launch_solve(const DeviceMesh& m, double omega, double rho, int chebIters, int precision, hipStream_t s);
+void launch_solve(const DeviceMesh& m, double omega, double rho, int chebIters, int precision, hipStream_t s,
+                  bool residualReady = false);
 // one Chebyshev sweep of the given precision (measurement support)
 void launch_one_sweep(const DeviceMesh& m, int precision, hipStream_t s);
 // k_contact.hip

@@ -6,8 +6,8 @@
 //      w * U clamp(S) V^T Dm^{-T} G (PdIpcGpuFunc.cu:439-599) into 12 outputs.
 //   2. k_rhs: per point, rhs = rhs0 + tet outputs + springs + bending gathered
 //      from the 1-ring + collision rhs + position term.
-// Every point's sum is formed by one lane in a fixed order, so the FP64 result
-// is bit-reproducible run to run.  T is the arithmetic type of the projections
+// Every point's sum is formed by a few lanes (entries dealt round-robin)
+// combined by a fixed shuffle tree, so the FP64 result is bit-reproducible.  T is the arithmetic type of the projections
 // (double, or float in the FP32-split mode); edge vectors are formed in double
 // and rounded once, so FP32 never sees absolute coordinates.
 #include "device.hpp"
@@ -208,29 +208,61 @@ __global__ void k_tet_project(int nT, int ld, T sigLo, T sigHi, const int4* __re
   }
 }
 
-template <class T>
-__global__ void k_rhs(int nSim, int ld, int nT, int withCol, const double* __restrict__ initRhs,
+// RES (FP32-split): also the first pass of the solve's FP64 residual for the
+// current X and D (as k_res_a in k_solve.hip: v = M x + w D^{-1} b and the
+// pointwise part (1 - w) M x + (2 - w) w D^{-1} b - x), saving a launch.
+struct ResidualArgs {
+  double w;
+  const int* off;
+  const int* col;
+  const double* val;
+  const double* D;
+  Row64* v;
+  Row64* pt;
+};
+
+// kRhsLanes lanes per point: ring slots, tet corners and (RES) R1 entries are
+// dealt round-robin and the partial sums combined by a fixed shuffle tree.
+constexpr int kRhsLanes = 4;
+
+template <class V>
+__device__ inline V rhs_lanes_sum(V v) {
+#pragma unroll
+  for (int o = kRhsLanes / 2; o > 0; o >>= 1) v += __shfl_xor(v, o, kWave);
+  return v;
+}
+
+template <class T, bool RES>
+__global__ void k_rhs(int nSim, int ld, int nT, int withCol, ResidualArgs ra, const double* __restrict__ initRhs,
                       const int* __restrict__ vtOff, const int* __restrict__ vtList, const double* __restrict__ tetOut,
                       const int* __restrict__ clothOfPoint, const int* __restrict__ cOff, const int* __restrict__ cNbr,
                       const double* __restrict__ cRest, const double* __restrict__ cStrain,
                       const double* __restrict__ cBendIn, const double* __restrict__ A, const double* __restrict__ X,
                       const double* __restrict__ colRhs, const double* __restrict__ fixK,
                       const double* __restrict__ Xrest, double* __restrict__ rhs) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nSim) return;
-  double r[3] = {initRhs[i], initRhs[ld + i], initRhs[2 * ld + i]};
-  for (int q = vtOff[i]; q < vtOff[i + 1]; q++) {
+  int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  int i = gt / kRhsLanes, sub = gt % kRhsLanes;
+  bool live = i < nSim;
+  double tr[3] = {0, 0, 0};  // tet outputs
+  int q0 = live ? vtOff[i] : 0, q1 = live ? vtOff[i + 1] : 0;
+  for (int q = q0 + sub; q < q1; q += kRhsLanes) {
     int tc = vtList[q];
     int t = tc >> 2, corner = tc & 3;
 #pragma unroll
-    for (int k = 0; k < 3; k++) r[k] += tetOut[(int64_t)(corner * 3 + k) * nT + t];
+    for (int k = 0; k < 3; k++) tr[k] += tetOut[(int64_t)(corner * 3 + k) * nT + t];
   }
-  int c = clothOfPoint[i];
+  double xi[3] = {0, 0, 0};
+  if (live) {
+    xi[0] = X[i];
+    xi[1] = X[ld + i];
+    xi[2] = X[2 * ld + i];
+  }
+  int c = live ? clothOfPoint[i] : -1;
+  T sp[3] = {0, 0, 0};
+  double bd[3] = {0, 0, 0};
   if (c >= 0) {
     int o = cOff[c], e = cOff[c + 1];
-    double xi[3] = {X[i], X[ld + i], X[2 * ld + i]};
-    T sp[3] = {0, 0, 0};
-    for (int s = o + 1; s < e; s++) {
+    for (int s = o + 1 + sub; s < e; s += kRhsLanes) {
       int j = cNbr[s];
       T d[3];
 #pragma unroll
@@ -240,13 +272,34 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, const double* __res
 #pragma unroll
       for (int k = 0; k < 3; k++) sp[k] += rc * d[k];
     }
-    double bd[3] = {0, 0, 0};
-    for (int s = o; s < e; s++) {
+    for (int s = o + sub; s < e; s += kRhsLanes) {
       int j = cNbr[s];
       double b = cBendIn[s];
 #pragma unroll
       for (int k = 0; k < 3; k++) bd[k] += b * A[k * ld + j];
     }
+  }
+  double sx[3] = {0, 0, 0};
+  if (RES && live)
+    for (int q = ra.off[i] + sub; q < ra.off[i + 1]; q += kRhsLanes) {
+      int j = ra.col[q];
+      double a = ra.val[q];
+      sx[0] += a * X[j];
+      sx[1] += a * X[ld + j];
+      sx[2] += a * X[2 * ld + j];
+    }
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    tr[k] = rhs_lanes_sum(tr[k]);
+    sp[k] = rhs_lanes_sum(sp[k]);
+    bd[k] = rhs_lanes_sum(bd[k]);
+    if (RES) sx[k] = rhs_lanes_sum(sx[k]);
+  }
+  if (!live || sub != 0) return;
+  double r[3] = {initRhs[i], initRhs[ld + i], initRhs[2 * ld + i]};
+#pragma unroll
+  for (int k = 0; k < 3; k++) r[k] += tr[k];
+  if (c >= 0) {
 #pragma unroll
     for (int k = 0; k < 3; k++) r[k] += (double)sp[k] + bd[k];
   }
@@ -261,19 +314,39 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, const double* __res
   }
 #pragma unroll
   for (int k = 0; k < 3; k++) rhs[k * ld + i] = r[k];
+  if (RES) {
+    double di = ra.D[i], w = ra.w;
+    double vv[3], pp[3];
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      double y = (1.0 - w) * xi[k] + w * sx[k] / di;
+      double u = r[k] / di;
+      vv[k] = y + w * u;
+      pp[k] = (1.0 - w) * y + (2.0 - w) * w * u - xi[k];
+    }
+    ra.v[i] = Row64{vv[0], vv[1], vv[2], 0.0};
+    ra.pt[i] = Row64{pp[0], pp[1], pp[2], 0.0};
+  }
 }
 
 template <class T>
-void local_impl(const DeviceMesh& m, T lo, T hi, int withCol, hipStream_t s) {
+void local_impl(const DeviceMesh& m, T lo, T hi, int withCol, double resW, hipStream_t s) {
   if (m.nCloth > 0)
     hipLaunchKernelGGL(k_cloth_curvature<T>, dim3(grid_for(m.nCloth)), dim3(kBlock), 0, s, m.nCloth, m.ld, m.cCenter,
                        m.cOff, m.cNbr, m.cLbo, m.cH0, m.X, m.clothA);
   if (m.nTets > 0)
     hipLaunchKernelGGL(k_tet_project<T>, dim3(grid_for(m.nTets)), dim3(kBlock), 0, s, m.nTets, m.ld, lo, hi,
                        m.tetIdx, m.tetDminv, m.tetW, m.X, m.tetOut);
-  hipLaunchKernelGGL(k_rhs<T>, dim3(grid_for(m.nSim)), dim3(kBlock), 0, s, m.nSim, m.ld, m.nTets, withCol, m.initRhs,
-                     m.vtOff, m.vtList, m.tetOut, m.clothOfPoint, m.cOff, m.cNbr, m.cRest, m.cStrain, m.cBendIn,
-                     m.clothA, m.X, m.colRhs, m.fixK, m.Xrest, m.sysRhs);
+  const ResidualArgs ra{resW, m.r1Off, m.r1Col, m.r1Val, m.sysDiag, m.xa0, m.cA};
+  const dim3 grid(grid_for((int64_t)m.nSim * kRhsLanes));
+  if (resW >= 0.0)
+    hipLaunchKernelGGL((k_rhs<T, true>), grid, dim3(kBlock), 0, s, m.nSim, m.ld, m.nTets, withCol, ra, m.initRhs,
+                       m.vtOff, m.vtList, m.tetOut, m.clothOfPoint, m.cOff, m.cNbr, m.cRest, m.cStrain, m.cBendIn,
+                       m.clothA, m.X, m.colRhs, m.fixK, m.Xrest, m.sysRhs);
+  else
+    hipLaunchKernelGGL((k_rhs<T, false>), grid, dim3(kBlock), 0, s, m.nSim, m.ld, m.nTets, withCol, ra, m.initRhs,
+                       m.vtOff, m.vtList, m.tetOut, m.clothOfPoint, m.cOff, m.cNbr, m.cRest, m.cStrain, m.cBendIn,
+                       m.clothA, m.X, m.colRhs, m.fixK, m.Xrest, m.sysRhs);
 }
 
 // ---- adjoint of the local step (FP64), evaluated at the recorded x
@@ -508,9 +581,10 @@ void launch_adj_local(const DeviceMesh& m, const AdjMesh& a, double sigLo, doubl
                      bbar, ax, sbar, rhs0bar, colRbar);
 }
 
-void launch_local(const DeviceMesh& m, int precision, double sigLo, double sigHi, int withCollision, hipStream_t s) {
-  if (precision == 1) local_impl<float>(m, (float)sigLo, (float)sigHi, withCollision, s);
-  else local_impl<double>(m, sigLo, sigHi, withCollision, s);
+void launch_local(const DeviceMesh& m, int precision, double sigLo, double sigHi, int withCollision, hipStream_t s,
+                  double residualOmega) {
+  if (precision == 1) local_impl<float>(m, (float)sigLo, (float)sigHi, withCollision, residualOmega, s);
+  else local_impl<double>(m, sigLo, sigHi, withCollision, residualOmega, s);
 }
 
 }  // namespace pdipc

@@ -373,7 +373,8 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
 }
 
 template <int G>
-void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precision, hipStream_t s) {
+void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precision, bool residualReady,
+                hipStream_t s) {
   if (precision == 0) {
     solve64<G>(m, w, rho, iters, nullptr, s);
     return;
@@ -385,8 +386,9 @@ void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precis
   // delta_0 = 0 -> pre, delta_1 = r0 -> cur
   constexpr int GR = 4;  // lanes per R1 row
   const int gridR = (m.nSim * GR + kBlock - 1) / kBlock;
-  hipLaunchKernelGGL((k_res_a<GR>), dim3(gridR), dim3(kBlock), 0, s, m.nSim, m.ld, w, m.r1Off, m.r1Col, m.r1Val,
-                     m.sysDiag, m.X, m.sysRhs, m.xa0, m.cA);
+  if (!residualReady)  // else the local step's k_rhs wrote v and pt
+    hipLaunchKernelGGL((k_res_a<GR>), dim3(gridR), dim3(kBlock), 0, s, m.nSim, m.ld, w, m.r1Off, m.r1Col, m.r1Val,
+                       m.sysDiag, m.X, m.sysRhs, m.xa0, m.cA);
   hipLaunchKernelGGL((k_res_b<GR>), dim3(gridR), dim3(kBlock), 0, s, m.nSim, w, m.r1Off, m.r1Col, m.r1Val, m.sysDiag,
                      m.xa0, m.cA, pre, cur, m.r0);
   if (iters <= 1) {
@@ -681,12 +683,13 @@ void launch_adj_ops_gather(const DeviceMesh& m, const AdjMesh& a, double omega, 
                        m.sysDiag, Gbar, omega, R1bar);
 }
 
-void launch_solve(const DeviceMesh& m, double omega, double rho, int iters, int precision, hipStream_t s) {
+void launch_solve(const DeviceMesh& m, double omega, double rho, int iters, int precision, hipStream_t s,
+                  bool residualReady) {
   switch (m.sellLanes) {
-    case 1: solve_impl<1>(m, omega, rho, iters, precision, s); break;
-    case 2: solve_impl<2>(m, omega, rho, iters, precision, s); break;
-    case 8: solve_impl<8>(m, omega, rho, iters, precision, s); break;
-    default: solve_impl<4>(m, omega, rho, iters, precision, s); break;
+    case 1: solve_impl<1>(m, omega, rho, iters, precision, residualReady, s); break;
+    case 2: solve_impl<2>(m, omega, rho, iters, precision, residualReady, s); break;
+    case 8: solve_impl<8>(m, omega, rho, iters, precision, residualReady, s); break;
+    default: solve_impl<4>(m, omega, rho, iters, precision, residualReady, s); break;
   }
 }
 

@@ -328,8 +328,9 @@ void enqueue_local_global(pdipc_handle* h, int withCol) {
   launch_diag(m, withCol, s);
   launch_operators(m, p.omega, p.precision, s);
   for (int it = 0; it < p.lg_iters; it++) {
-    launch_local(m, p.precision, p.sigma_lo, p.sigma_hi, withCol, s);
-    launch_solve(m, p.omega, p.rho, p.cheb_iters, p.precision, s);
+    bool fused = p.precision == PDIPC_FP32_SPLIT;  // k_rhs also runs the residual's first pass
+    launch_local(m, p.precision, p.sigma_lo, p.sigma_hi, withCol, s, fused ? p.omega : -1.0);
+    launch_solve(m, p.omega, p.rho, p.cheb_iters, p.precision, s, fused);
   }
 }
 
