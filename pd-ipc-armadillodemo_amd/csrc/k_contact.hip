@@ -196,6 +196,14 @@ __host__ __device__ inline T accd(int type, P3T<T>* x, P3T<T>* dx, int iters, do
   T toc = T(0.0);
   for (int i = 0; i < iters; i++) {
     T tlb = T(step) * dF / (dcur * maxdisp);
+    // First iteration with tlb > 1: the loop returns 1.  The move shifts the
+    // primitives relative to each other by at most tlb * maxdisp = step * d0
+    // (dF = d^2 for both record types), so the moved distance is at least
+    // (1 - step) d0 > 0 and the touch test cannot fire; for d0 above 1e-9 no
+    // rounding can turn that distance into 0.  The second distance
+    // evaluation is skipped (most records are far apart relative to their
+    // motion).
+    if (i == 0 && tlb > T(1.0) && dcur > T(1e-9)) return T(1.0);
 #pragma unroll
     for (int j = 0; j < 4; j++)
 #pragma unroll
