@@ -180,7 +180,9 @@ int pdipc_set_cloth_stiffness(pdipc_handle* h, int model, const double* k, int n
  * cloth triangle (all cloth models in add order, nk >= their total), treating
  * candidate records, barrier active sets, minimum-TOI records and iteration
  * counts as fixed.  It re-runs one checkpoint segment at a time with every
- * solver iterate recorded; afterwards the handle is left at the final state.
+ * solver iterate recorded (the rollout records the last segment as it runs,
+ * so the first backward after it re-runs all but that one); afterwards the
+ * handle is left at the final state.
  * checkpoint_every = 0 records every step during the rollout instead (device
  * memory for the whole run's tape, ~0.4 GB per step of a 128^2 cloth): the
  * backward then runs no forward step and may be called again on the same

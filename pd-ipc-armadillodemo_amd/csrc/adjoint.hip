@@ -78,7 +78,9 @@ struct StageLayout {
   double* xold(double* b) const { return b; }
   double* xstart(double* b) const { return b + V3; }
   double* rhs(double* b, int l) const { return b + (2 + (size_t)l) * V3; }
-  double* it(double* b, int l, int k) const {  // k-th iterate of pass l; k = 0 is the pass's input
+  // k-th iterate of pass l; k = 0 is the pass's input.  Iterates 1 .. K of a
+  // pass are consecutive vectors (the reverse pass packs them by that stride).
+  double* it(double* b, int l, int k) const {
     if (k == 0) return l == 0 ? xstart(b) : it(b, l - 1, K);
     return b + (2 + (size_t)LG + (size_t)l * K + (k - 1)) * V3;
   }
@@ -109,6 +111,7 @@ struct AdjointState {
   std::vector<void*> allocs;
   int nSteps = 0, every = 1;
   bool recorded = false;  // the rollout kept the tape of every step (every == 0)
+  bool lastKept = false;  // seg holds the last checkpoint segment, recorded by the rollout
   std::vector<double*> ckX, ckV;
   double *endX = nullptr, *endV = nullptr;
   Pool tape;
@@ -219,6 +222,41 @@ int ensure_state(pdipc_handle* h) {
         (rc = dupload(A, &A->am.midA, midA)) || (rc = dupload(A, &A->am.springQ, springQ)) ||
         (rc = dupload(A, &A->am.springOff, sOff)) || (rc = dupload(A, &A->am.springEnt, sEnt)) ||
         (rc = dupload(A, &A->am.springMid, sMid)) || (rc = dupload(A, &A->am.springOther, sOther)))
+      return rc;
+  }
+  // per slice: the sorted union of the columns of its entries, and per union
+  // column and row the entry's offset in the slice's SELL block
+  {
+    const SellLayout& L = h->sell;
+    const int R = 64 / L.lanes, n = m.nSim;
+    std::vector<int> uOff(L.nSlices + 1, 0), uCol, cols;
+    std::vector<uint16_t> uPos;
+    auto real = [&](int64_t e, int r) {  // padding slots repeat the row's own index unflagged
+      return r < n && (L.col[e] != r || (L.rowFlag[e] & (int)0x80000000) != 0);
+    };
+    for (int sl = 0; sl < L.nSlices; sl++) {
+      const int64_t e0 = L.sliceOff[sl], e1 = L.sliceOff[sl + 1];
+      if (e1 - e0 > 65535) return fail(PDIPC_E_ARG, "adjoint: a SELL slice holds more than 65535 entries");
+      cols.clear();
+      for (int64_t e = e0; e < e1; e++) {
+        const int r = sl * R + (int)((e - e0) % 64) / L.lanes;
+        if (real(e, r)) cols.push_back(L.col[e]);
+      }
+      std::sort(cols.begin(), cols.end());
+      cols.erase(std::unique(cols.begin(), cols.end()), cols.end());
+      const size_t p0 = uPos.size();
+      uPos.resize(p0 + cols.size() * R, (uint16_t)0xFFFF);
+      for (int64_t e = e0; e < e1; e++) {
+        const int r = sl * R + (int)((e - e0) % 64) / L.lanes;
+        if (!real(e, r)) continue;
+        const size_t t = std::lower_bound(cols.begin(), cols.end(), L.col[e]) - cols.begin();
+        uPos[p0 + t * R + (r - sl * R)] = (uint16_t)(e - e0);
+      }
+      uCol.insert(uCol.end(), cols.begin(), cols.end());
+      uOff[sl + 1] = (int)uCol.size();
+    }
+    if ((rc = dupload(A, &A->am.uOff, uOff)) || (rc = dupload(A, &A->am.uCol, uCol)) ||
+        (rc = dupload(A, &A->am.uPos, uPos)))
       return rc;
   }
   for (double** p : {&A->ax, &A->axo, &A->asol, &A->uA, &A->uB, &A->uC, &A->uc, &A->bbar, &A->aA, &A->rhs0bar,
@@ -423,12 +461,11 @@ int reverse_step(pdipc_handle* h, StepTape& st, const double* xbarIn, const doub
     HIPCHK(hipMemsetAsync(m.dStale, 1, m.nSim, s));  // the next plain step recomputes every row
     launch_operators(m, p.omega, PDIPC_FP64, s);
     int rc = run_captured(h, A->revGraph[ot.withCol], [&]() {
-      std::vector<const double*> x(K);
       for (int l = LG - 1; l >= 0; l--) {
         launch_div_rows(m, A->asol, m.sysDiag, A->uS[K], s);  // u_K
         launch_zero(A->uc, (int64_t)V3, s);
-        for (int k = 0; k < K; k++) x[k] = L.it(sg, l, k);
-        launch_adj_pass(m, W.data(), K, A->uS.data(), x.data(), A->Gbar, A->uc, s);
+        launch_adj_pass(m, A->am, W.data(), K, A->uS.data(), L.it(sg, l, 0), L.it(sg, l, 1), (int64_t)V3,
+                        A->Gbar, A->uc, s);
         launch_adj_first(m, p.omega, L.rhs(sg, l), A->uc, A->bbar, A->Dbar, A->R1bar, s);
         launch_adj_local(m, A->am, p.sigma_lo, p.sigma_hi, L.it(sg, l, 0), A->bbar, A->uS[0], A->aA, A->tetAdj,
                          A->asol, A->sbar, A->rhs0bar, ot.withCol ? A->colRacc : nullptr, s);
@@ -472,6 +509,7 @@ int pdipc_rollout(pdipc_handle* h, int nsteps, int every) {
     A->nSteps = nsteps;
     A->every = 0;
     A->recorded = false;
+    A->lastKept = false;
     A->tape.reset();
     A->stats[0] = 0;
     A->seg.assign(nsteps, StepTape());
@@ -488,6 +526,7 @@ int pdipc_rollout(pdipc_handle* h, int nsteps, int every) {
     return check_launch(h);
   }
   A->recorded = false;
+  A->lastKept = false;
   int nck = (nsteps + every - 1) / every;
   while ((int)A->ckX.size() < nck) {
     double *x = nullptr, *v = nullptr;
@@ -498,13 +537,23 @@ int pdipc_rollout(pdipc_handle* h, int nsteps, int every) {
   }
   A->nSteps = nsteps;
   A->every = every;
+  // the last segment runs recorded (the same trajectory bit for bit), so the
+  // backward reverses it without re-running it
+  const int keepFrom = (nck - 1) * every;
+  A->tape.reset();
+  A->seg.assign(nsteps - keepFrom, StepTape());
   for (int n = 0; n < nsteps; n++) {
     if (n % every == 0) {
       launch_copy(A->ckX[n / every], m.X, (int64_t)V3, h->stream);
       launch_copy(A->ckV[n / every], m.V, (int64_t)V3, h->stream);
     }
-    if ((rc = advance(h))) return rc;
+    if (n >= keepFrom)
+      rc = taped_step(h, A->seg[n - keepFrom]);
+    else
+      rc = advance(h);
+    if (rc) return rc;
   }
+  A->lastKept = true;
   launch_copy(A->endX, m.X, (int64_t)V3, h->stream);
   launch_copy(A->endV, m.V, (int64_t)V3, h->stream);
   HIPCHK(hipStreamSynchronize(h->stream));
@@ -543,12 +592,17 @@ int pdipc_backward(pdipc_handle* h, const double* dx, const double* dv, double* 
   int nseg = A->recorded ? 0 : (A->nSteps + A->every - 1) / A->every;
   for (int sg = nseg - 1; sg >= 0; sg--) {
     int first = sg * A->every, last = std::min(A->nSteps, first + A->every);
-    launch_copy(m.X, A->ckX[sg], (int64_t)V3, s);
-    launch_copy(m.V, A->ckV[sg], (int64_t)V3, s);
-    A->tape.reset();
-    A->seg.assign(last - first, StepTape());
-    for (int n = first; n < last; n++)
-      if ((rc = taped_step(h, A->seg[n - first]))) return rc;
+    if (A->lastKept && sg == nseg - 1) {
+      for (const StepTape& st : A->seg) A->stats[0] += (double)st.outers.size();
+    } else {
+      launch_copy(m.X, A->ckX[sg], (int64_t)V3, s);
+      launch_copy(m.V, A->ckV[sg], (int64_t)V3, s);
+      A->lastKept = false;  // the tape now holds this segment
+      A->tape.reset();
+      A->seg.assign(last - first, StepTape());
+      for (int n = first; n < last; n++)
+        if ((rc = taped_step(h, A->seg[n - first]))) return rc;
+    }
     double used = 0;
     for (size_t c = 0; c < A->tape.ci && c < A->tape.chunks.size(); c++) used += (double)A->tape.chunks[c].size;
     used += (double)A->tape.off;
@@ -616,8 +670,7 @@ int pdipc_debug_solve_vjp(pdipc_handle* h, const double* diag, const double* rhs
   launch_div_rows(m, A->asol, m.sysDiag, A->uS[K], s);
   launch_zero(A->uc, (int64_t)V3, s);
   launch_zero(A->Gbar, m.sellEntries, s);
-  std::vector<const double*> x(K, A->axo);
-  launch_adj_pass(m, W.data(), K, A->uS.data(), x.data(), A->Gbar, A->uc, s);
+  launch_adj_pass(m, A->am, W.data(), K, A->uS.data(), A->axo, A->axo, 0, A->Gbar, A->uc, s);
   launch_zero(A->Dbar, m.nSim, s);
   launch_adj_first(m, p.omega, A->bbar, A->uc, A->ax, A->Dbar, A->R1bar, s);
   launch_mul_rows(m, A->uS[0], m.sysDiag, A->asol, s);

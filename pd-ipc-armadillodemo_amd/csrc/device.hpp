@@ -162,13 +162,19 @@ struct AdjMesh {
   int nSpringQ = 0;
   int *springQ = nullptr, *springOff = nullptr, *springMid = nullptr, *springOther = nullptr;
   int64_t* springEnt = nullptr;
+  // per slice: the sorted union of the columns of its entries (uOff, uCol), and
+  // per union column and row of the slice the entry's offset in the slice's
+  // SELL block (uPos, 0xFFFF where the row has no entry in that column)
+  int *uOff = nullptr, *uCol = nullptr;
+  uint16_t* uPos = nullptr;
 };
 void launch_solve_taped(const DeviceMesh& m, double omega, double rho, int iters, double* const* outs, hipStream_t s);
 // the whole reverse of one K-sweep solve: u[k] for k = K-1 .. 0 from u[K] (given)
-// and u[K+1] (zero), then one SELL pass adding sum_k W_k u_{k+1} x_k^T to Gbar
+// and u[K+1] (zero), then one SELL pass adding sum_k W_k u_{k+1} x_k^T to Gbar,
+// with the forward iterates x_0 and x_k = x1 + (k-1) xStride (k >= 1)
 constexpr int kMaxCheb = 32;
-void launch_adj_pass(const DeviceMesh& m, const double* W, int K, double* const* u, const double* const* x,
-                     double* Gbar, double* uc, hipStream_t s);
+void launch_adj_pass(const DeviceMesh& m, const AdjMesh& a, const double* W, int K, double* const* u,
+                     const double* x0, const double* x1, int64_t xStride, double* Gbar, double* uc, hipStream_t s);
 void launch_adj_first(const DeviceMesh& m, double omega, const double* b, const double* uc, double* bbar, double* Dbar,
                       double* R1bar, hipStream_t s);
 // adjoint of the operator update (M^2 from D and R1) as three deterministic

@@ -410,7 +410,7 @@ void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precis
 // (M^T = D M D^{-1} because R1 is symmetric, so M^T xbar = D M u): each is the
 // forward SELL SpMV on u,
 //   u_out = a M u_in + b u_oth,   uc[row] += a u_in[row]   (uc: D^{-1}-scaled first-term adjoint);
-// the M^2-entry adjoints are gathered once per solve by k_adj_gbar_e.
+// the M^2-entry adjoints are gathered once per solve by k_adj_gbar.
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_adj_spmv(int n, int nSlices, int ld, double a, double b,
                                                      const int64_t* __restrict__ sliceOff,
@@ -447,34 +447,75 @@ __global__ void __launch_bounds__(kBlock) k_adj_spmv(int n, int nSlices, int ld,
 }
 
 // Gbar_e += sum_k W_k u_{k+1}[row] . x_k[col] over the K sweeps of one pass
-// (u_{k+1} and x_k from the pass's reverse and recorded forward iterates)
+// (u_{k+1} and x_k from the pass's reverse and recorded forward iterates;
+// x_0, and x_k = x1 + (k-1) xStride for k >= 1).  One workgroup per slice, one
+// lane per column of the union of the slice's columns (sorted, listed per slice
+// at set-up): the lane loads its column's x_k kGbarChunk sweeps at a time (all
+// loads of a chunk in flight together) and sums over k, in order, for all R
+// rows of the slice at once; the rows' u values are wave-uniform (R consecutive
+// doubles per component, scalar loads; rows past n read the vectors' ld
+// padding, and their sums are never stored).  So each x value is read once per
+// slice, where a lane per entry reads it for every entry of the column.  uPos
+// gives, per union column and row, the entry's offset in the slice's SELL block
+// (0xFFFF: no entry; padding slots are never written).
 struct GbarArgs {
-  const double* u[kMaxCheb + 1];  // u_1 .. u_K
-  const double* x[kMaxCheb];      // x_0 .. x_{K-1}
+  const double* u[kMaxCheb];  // u_1 .. u_K
   double w[kMaxCheb];
   int K;
 };
-// one thread per SELL entry (the row comes from rowFlag): enough threads to
-// hide the latency of the 3K gathered loads each entry makes
-__global__ void __launch_bounds__(kBlock) k_adj_gbar_e(int64_t nE, int ld, const int* __restrict__ rowFlag,
-                                                       const int* __restrict__ col, GbarArgs ga,
-                                                       double* __restrict__ Gbar) {
-  int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= nE) return;
-  int row = rowFlag[e] & 0x7fffffff;
-  int j = col[e];
-  double g = 0.0;
-  for (int k = 0; k < ga.K; k++) {
-    const double* u = ga.u[k + 1];
-    const double* x = ga.x[k];
-    g += ga.w[k] * (u[row] * x[j] + u[ld + row] * x[ld + j] + u[2 * ld + row] * x[2 * ld + j]);
+constexpr int kGbarThreads = 128;
+constexpr int kGbarChunk = 8;
+template <int G>
+__global__ void __launch_bounds__(kGbarThreads) k_adj_gbar(int ld, const int64_t* __restrict__ sliceOff,
+                                                           const int* __restrict__ uOff, const int* __restrict__ uCol,
+                                                           const uint16_t* __restrict__ uPos,
+                                                           const double* __restrict__ x0, const double* __restrict__ x1,
+                                                           int64_t xStride, GbarArgs ga, double* __restrict__ Gbar) {
+  constexpr int R = kWave / G;
+  const int slice = blockIdx.x;
+  const int K = ga.K;
+  const int row0 = slice * R;
+  const int u0 = uOff[slice], nU = uOff[slice + 1] - u0;
+  const int64_t sb = sliceOff[slice];
+  for (int t = threadIdx.x; t < nU; t += kGbarThreads) {
+    const int j = uCol[u0 + t];
+    double g[R];
+#pragma unroll
+    for (int i = 0; i < R; i++) g[i] = 0.0;
+    for (int k0 = 0; k0 < K; k0 += kGbarChunk) {
+      double xv[kGbarChunk][3];
+#pragma unroll
+      for (int q = 0; q < kGbarChunk; q++)
+        if (k0 + q < K) {
+          const int k = k0 + q;
+          const double* x = k == 0 ? x0 : x1 + (k - 1) * xStride;
+          xv[q][0] = x[j];
+          xv[q][1] = x[ld + j];
+          xv[q][2] = x[2 * ld + j];
+        }
+#pragma unroll
+      for (int q = 0; q < kGbarChunk; q++)
+        if (k0 + q < K) {
+          const int k = k0 + q;
+          const double w = ga.w[k];
+          const double* uk = ga.u[k] + row0;
+#pragma unroll
+          for (int i = 0; i < R; i++)
+            g[i] += w * (uk[i] * xv[q][0] + uk[ld + i] * xv[q][1] + uk[2 * ld + i] * xv[q][2]);
+        }
+    }
+    const uint16_t* pos = uPos + (int64_t)(u0 + t) * R;
+#pragma unroll
+    for (int i = 0; i < R; i++) {
+      const int p = pos[i];
+      if (p != 0xFFFF) Gbar[sb + p] += g[i];
+    }
   }
-  Gbar[e] += g;
 }
 
 template <int G>
-void adj_pass_impl(const DeviceMesh& m, const double* W, int K, double* const* u, const double* const* x,
-                   double* Gbar, double* uc, hipStream_t s) {
+void adj_pass_impl(const DeviceMesh& m, const AdjMesh& a, const double* W, int K, double* const* u, const double* x0,
+                   const double* x1, int64_t xStride, double* Gbar, double* uc, hipStream_t s) {
   const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
   for (int k = K - 1; k >= 0; k--) {
     double b = k + 1 <= K - 1 ? 1.0 - W[k + 1] : 0.0;
@@ -483,29 +524,33 @@ void adj_pass_impl(const DeviceMesh& m, const double* W, int K, double* const* u
   }
   GbarArgs ga;
   ga.K = K;
-  for (int k = 0; k <= K; k++) ga.u[k] = u[k];
   for (int k = 0; k < K; k++) {
-    ga.x[k] = x[k];
+    ga.u[k] = u[k + 1];
     ga.w[k] = W[k];
   }
-  int64_t nE = m.sellEntries;
-  hipLaunchKernelGGL(k_adj_gbar_e, dim3((unsigned)((nE + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nE, m.ld,
-                     m.m2RowFlag, m.m2Col, ga, Gbar);
+  hipLaunchKernelGGL((k_adj_gbar<G>), dim3(m.nSlices), dim3(kGbarThreads), 0, s, m.ld, m.sliceOff, a.uOff, a.uCol,
+                     a.uPos, x0, x1, xStride, ga, Gbar);
 }
 
 // Adjoint of the first term c_i = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
 // given its scaled adjoint uc = D^{-1} cbar:  bbar = w^2 D^{-1} R1 uc + (2 - w) w uc,
 // and the D and R1 parts of the derivative accumulated into Dbar and R1bar.
-__global__ void k_adj_first(int n, int ld, double w, const int* __restrict__ off, const int* __restrict__ col,
-                            const double* __restrict__ r1, const double* __restrict__ D, const double* __restrict__ b,
-                            const double* __restrict__ uc, double* __restrict__ bbar, double* __restrict__ Dbar,
-                            double* __restrict__ R1bar) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n) return;
+// kFirstLanes lanes per row share its R1 entries; rows past n run on row n - 1
+// without writing, so every lane takes part in the reduction.
+constexpr int kFirstLanes = 8;
+__global__ void __launch_bounds__(kBlock) k_adj_first(int n, int ld, double w, const int* __restrict__ off,
+                                                      const int* __restrict__ col, const double* __restrict__ r1,
+                                                      const double* __restrict__ D, const double* __restrict__ b,
+                                                      const double* __restrict__ uc, double* __restrict__ bbar,
+                                                      double* __restrict__ Dbar, double* __restrict__ R1bar) {
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sub = gt % kFirstLanes;
+  const bool live = gt / kFirstLanes < n;
+  const int i = live ? gt / kFirstLanes : n - 1;
   double ui[3] = {uc[i], uc[ld + i], uc[2 * ld + i]};
   double s1[3] = {0, 0, 0}, s2[3] = {0, 0, 0};
   const double w2 = w * w;
-  for (int q = off[i]; q < off[i + 1]; q++) {
+  for (int q = off[i] + sub; q < off[i + 1]; q += kFirstLanes) {
     int j = col[q];
     double a = r1[q], dj = D[j];
     double bj[3] = {b[j] / dj, b[ld + j] / dj, b[2 * ld + j] / dj};
@@ -514,8 +559,11 @@ __global__ void k_adj_first(int n, int ld, double w, const int* __restrict__ off
       s1[k] += a * uc[k * ld + j];
       s2[k] += a * bj[k];
     }
-    R1bar[q] += w2 * (ui[0] * bj[0] + ui[1] * bj[1] + ui[2] * bj[2]);
+    if (live) R1bar[q] += w2 * (ui[0] * bj[0] + ui[1] * bj[1] + ui[2] * bj[2]);
   }
+  group_reduce<kFirstLanes>(s1);
+  group_reduce<kFirstLanes>(s2);
+  if (!live || sub != 0) return;
   double di = D[i];
   double db = 0.0;
 #pragma unroll
@@ -645,20 +693,21 @@ void launch_solve_taped(const DeviceMesh& m, double omega, double rho, int iters
   }
 }
 
-void launch_adj_pass(const DeviceMesh& m, const double* W, int K, double* const* u, const double* const* x,
-                     double* Gbar, double* uc, hipStream_t s) {
+void launch_adj_pass(const DeviceMesh& m, const AdjMesh& a, const double* W, int K, double* const* u,
+                     const double* x0, const double* x1, int64_t xStride, double* Gbar, double* uc, hipStream_t s) {
   switch (m.sellLanes) {
-    case 1: adj_pass_impl<1>(m, W, K, u, x, Gbar, uc, s); break;
-    case 2: adj_pass_impl<2>(m, W, K, u, x, Gbar, uc, s); break;
-    case 8: adj_pass_impl<8>(m, W, K, u, x, Gbar, uc, s); break;
-    default: adj_pass_impl<4>(m, W, K, u, x, Gbar, uc, s); break;
+    case 1: adj_pass_impl<1>(m, a, W, K, u, x0, x1, xStride, Gbar, uc, s); break;
+    case 2: adj_pass_impl<2>(m, a, W, K, u, x0, x1, xStride, Gbar, uc, s); break;
+    case 8: adj_pass_impl<8>(m, a, W, K, u, x0, x1, xStride, Gbar, uc, s); break;
+    default: adj_pass_impl<4>(m, a, W, K, u, x0, x1, xStride, Gbar, uc, s); break;
   }
 }
 
 void launch_adj_first(const DeviceMesh& m, double omega, const double* b, const double* uc, double* bbar, double* Dbar,
                       double* R1bar, hipStream_t s) {
-  hipLaunchKernelGGL(k_adj_first, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, omega,
-                     m.r1Off, m.r1Col, m.r1Val, m.sysDiag, b, uc, bbar, Dbar, R1bar);
+  const int rowsPerBlock = kBlock / kFirstLanes;
+  hipLaunchKernelGGL(k_adj_first, dim3((m.nSim + rowsPerBlock - 1) / rowsPerBlock), dim3(kBlock), 0, s, m.nSim, m.ld,
+                     omega, m.r1Off, m.r1Col, m.r1Val, m.sysDiag, b, uc, bbar, Dbar, R1bar);
 }
 
 void launch_adj_ops_gather(const DeviceMesh& m, const AdjMesh& a, double omega, const double* Gbar, double* Dbar,

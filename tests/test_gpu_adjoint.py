@@ -262,7 +262,9 @@ def test_adjoint_graph_replay_matches_direct_launches():
 def test_recorded_rollout_matches_checkpoints():
     """checkpoint_every = 0 keeps the rollout's own tape: the backward runs no
     forward step, gives the checkpointed backward's gradient (the re-run
-    segments record the same trajectory) and can be repeated."""
+    segments record the same trajectory) and can be repeated.  A checkpointed
+    rollout records its last segment as it runs: the first backward reverses
+    that tape, a second one re-runs the segment, and both give the same bits."""
     steps = 9
     sc, v0, target = _cloth_case(1, steps=steps)
     k0 = sc["cloths"][0]["tri_k"].copy()
@@ -277,6 +279,9 @@ def test_recorded_rollout_matches_checkpoints():
         sim.rollout(steps, every)
         xT = sim.x()[: sim.sim].copy()
         out[every] = (xT, sim.backward(dx=2.0 * (xT - target), n_tris=len(k0)))
+        if every:
+            for a, b in zip(sim.backward(dx=2.0 * (xT - target), n_tris=len(k0)), out[every][1]):
+                assert np.array_equal(a, b)
     (xa, (dxa, dva, dka)), (xb, (dxb, dvb, dkb)) = out[4], out[0]
     assert np.array_equal(xa, xb)
     for a, b in ((dxa, dxb), (dva, dvb), (dka, dkb)):
