@@ -1,0 +1,75 @@
+"""The SELL matrix layout's variants against the oracle: every supported lane
+count per row (1, 2, 4; 8 is the default every other test runs), and 32-bit
+columns (a cloth whose vertex ids are randomly permuted, so that the offsets
+col - row of M^2 leave the 16-bit range the FP32 sweep otherwise stores).
+
+Bars as in the parity tests: FP64 within 1e-8 relative per step (teacher
+forced), FP32-split within 1e-5 of the FP64 oracle per step."""
+
+import numpy as np
+import pytest
+
+import oracle
+import pdipc
+import scenes
+from parity_util import check_step, rel
+
+pytestmark = pytest.mark.gpu
+
+
+def _teacher_forced(sc, ref, steps, **extra):
+    """Steps of the HIP library from the oracle's states; returns the worst
+    FP64-bar error (checked inside) or the worst FP32-split relative error."""
+    sim = pdipc.Simulator.from_scene(sc, **extra)
+    fp64 = extra.get("precision", pdipc.FP64) == pdipc.FP64
+    worst = 0.0
+    for k in range(steps):
+        x, v = ref.x(), ref.v()
+        sim.set_x(x)
+        sim.set_v(v)
+        ref.step(1)
+        sim.step(1)
+        if fp64:
+            err, _ = check_step(sc, x, v, sim.x(), ref.x(), ref.sim, seed=k)
+        else:
+            err = rel(sim.x()[: ref.sim], ref.x()[: ref.sim])
+        worst = max(worst, err)
+    return worst
+
+
+@pytest.mark.parametrize("lanes", [1, 2, 4])
+def test_lane_counts_match_oracle(lanes):
+    sc = scenes.cloth_over_sphere(12, 2, ground=True, seed=3, patch_dim=12, sphere_patch_dim=12, height=0.16)
+    start = oracle.Oracle(sc)
+    start.step(6)  # into contact with the obstacle
+    for prec in (pdipc.FP64, pdipc.FP32_SPLIT):
+        print(f"case: {lanes} lanes per SELL row, precision {prec}", flush=True)
+        ref = oracle.Oracle(sc)
+        ref.set_x(start.x())
+        ref.set_v(start.v())
+        worst = _teacher_forced(sc, ref, 4, sell_lanes=lanes, precision=prec)
+        print("worst step error", worst, flush=True)
+        if prec == pdipc.FP32_SPLIT:
+            assert worst < 1e-5
+
+
+def test_32bit_columns_scrambled_cloth():
+    n = 190  # 36,100 vertices: a random numbering puts most M^2 offsets past 32767
+    sc = scenes.cloth_over_sphere(n, 2, ground=True, seed=4, patch_dim=16, sphere_patch_dim=16, height=0.16)
+    cl = sc["cloths"][0]
+    perm = np.random.default_rng(4).permutation(len(cl["v"]))
+    v = np.empty_like(cl["v"])
+    v[perm] = cl["v"]
+    cl["v"] = v
+    cl["f"] = perm[cl["f"]].astype(np.int32)
+    f = cl["f"]
+    edges = np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]])
+    assert np.abs(edges[:, 0] - edges[:, 1]).max() > 32767  # the 1-ring alone is out of 16-bit range
+    for prec in (pdipc.FP64, pdipc.FP32_SPLIT):
+        print(f"case: scrambled {n}x{n} cloth, precision {prec}", flush=True)
+        ref = oracle.Oracle(sc)
+        ref.step(2)
+        worst = _teacher_forced(sc, ref, 2, precision=prec)
+        print("worst step error", worst, flush=True)
+        if prec == pdipc.FP32_SPLIT:
+            assert worst < 1e-5
