@@ -121,6 +121,16 @@ int pdipc_get_counters(pdipc_handle* h, pdipc_counters* out);
  * column-major data) in float (the reference's qeal) or double. */
 int pdipc_save_frame(pdipc_handle* h, const char* path, int as_double);
 
+/* --- full-state checkpoint (no counterpart in the reference, which only dumps
+ * frames): the step reads nothing from earlier steps but x and v, so a saved
+ * state restarts a run bit for bit.  Binary file: "PDIPCST1", int32 version 1,
+ * int32 nSim, int32 nTotal, int32 0, uint64 topology fingerprint, then x (3
+ * nTotal doubles, AoS) and v (3 nSim doubles, AoS).  Loading checks the magic,
+ * the point counts and the fingerprint (the scene's ring, tet and R1
+ * patterns) and rejects a short or over-long file with PDIPC_E_ARG. */
+int pdipc_save_state(pdipc_handle* h, const char* path);
+int pdipc_load_state(pdipc_handle* h, const char* path);
+
 /* --- candidate records of the last broad phase (int[5]: type, a, b, c, d as in
  * getCcdPotentialICPairsListKernel, PdIpcGpuFunc.cu:1915-1945); returns the
  * count in *count and copies min(count, cap) records. */

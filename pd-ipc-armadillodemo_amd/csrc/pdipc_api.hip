@@ -423,6 +423,39 @@ ModelInput make_input(int kind, const double* v, int nv, const int* el, int nel,
 
 }  // namespace pdipc
 
+// Full-state checkpoint: the step reads nothing from earlier steps but the
+// positions and velocities, so (x, v) restart a run bit for bit.  File:
+// "PDIPCST1", int32 version 1, int32 nSim, int32 nTotal, int32 0, uint64
+// topology fingerprint (FNV-1a over the point counts and the ring, tet and R1
+// patterns), then x (3 nTotal doubles, AoS) and v (3 nSim doubles, AoS).
+namespace {
+constexpr char kStateMagic[8] = {'P', 'D', 'I', 'P', 'C', 'S', 'T', '1'};
+struct StateHeader {
+  char magic[8];
+  int32_t version, nSim, nTotal, pad;
+  uint64_t topo;
+};
+template <class T>
+void fnv(uint64_t& hsh, const T* p, size_t n) {
+  const unsigned char* b = reinterpret_cast<const unsigned char*>(p);
+  for (size_t i = 0; i < n * sizeof(T); i++) {
+    hsh ^= b[i];
+    hsh *= 1099511628211ull;
+  }
+}
+uint64_t topology_fingerprint(const HostSetup& hs) {
+  uint64_t hsh = 1469598103934665603ull;
+  const int counts[4] = {hs.nTotal, hs.nSim, hs.nCloth, hs.nTets};
+  fnv(hsh, counts, 4);
+  fnv(hsh, hs.cOff.data(), hs.cOff.size());
+  fnv(hsh, hs.cNbr.data(), hs.cNbr.size());
+  fnv(hsh, hs.tetIdx.data(), hs.tetIdx.size());
+  fnv(hsh, hs.r1Off.data(), hs.r1Off.size());
+  fnv(hsh, hs.r1Col.data(), hs.r1Col.size());
+  return hsh;
+}
+}  // namespace
+
 extern "C" {
 
 void pdipc_default_params(pdipc_params* o) {
@@ -630,6 +663,57 @@ int pdipc_save_frame(pdipc_handle* h, const char* path, int as_double) {
     fwrite(xf.data(), sizeof(float), xf.size(), f);
   }
   fclose(f);
+  return PDIPC_OK;
+}
+
+int pdipc_save_state(pdipc_handle* h, const char* path) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (!path) return fail(PDIPC_E_ARG, "null path");
+  const DeviceMesh& m = h->m;
+  std::vector<double> x(3 * (size_t)m.nTotal), v(3 * (size_t)m.nSim);
+  if ((rc = get_soa(h, m.X, x.data(), m.nTotal)) || (rc = get_soa(h, m.V, v.data(), m.nSim))) return rc;
+  StateHeader hd;
+  memcpy(hd.magic, kStateMagic, 8);
+  hd.version = 1;
+  hd.nSim = m.nSim;
+  hd.nTotal = m.nTotal;
+  hd.pad = 0;
+  hd.topo = topology_fingerprint(h->hs);
+  FILE* f = fopen(path, "wb");
+  if (!f) return fail(PDIPC_E_ARG, std::string("cannot open ") + path);
+  bool ok = fwrite(&hd, sizeof(hd), 1, f) == 1 && fwrite(x.data(), sizeof(double), x.size(), f) == x.size() &&
+            fwrite(v.data(), sizeof(double), v.size(), f) == v.size();
+  ok = (fclose(f) == 0) && ok;
+  return ok ? PDIPC_OK : fail(PDIPC_E_ARG, std::string("short write to ") + path);
+}
+
+int pdipc_load_state(pdipc_handle* h, const char* path) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (!path) return fail(PDIPC_E_ARG, "null path");
+  const DeviceMesh& m = h->m;
+  FILE* f = fopen(path, "rb");
+  if (!f) return fail(PDIPC_E_ARG, std::string("cannot open ") + path);
+  StateHeader hd;
+  std::vector<double> x(3 * (size_t)m.nTotal), v(3 * (size_t)m.nSim);
+  bool ok = fread(&hd, sizeof(hd), 1, f) == 1;
+  std::string why;
+  if (!ok || memcmp(hd.magic, kStateMagic, 8) != 0 || hd.version != 1)
+    why = "not a state file";
+  else if (hd.nSim != m.nSim || hd.nTotal != m.nTotal || hd.topo != topology_fingerprint(h->hs))
+    why = "saved from a different scene";
+  else if (fread(x.data(), sizeof(double), x.size(), f) != x.size() ||
+           fread(v.data(), sizeof(double), v.size(), f) != v.size())
+    why = "truncated";
+  else {
+    char extra;
+    if (fread(&extra, 1, 1, f) != 0) why = "trailing data";
+  }
+  fclose(f);
+  if (!why.empty()) return fail(PDIPC_E_ARG, std::string(path) + ": " + why);
+  if ((rc = put_soa(h, h->m.X, x.data(), m.nTotal)) || (rc = put_soa(h, h->m.V, v.data(), m.nSim))) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
   return PDIPC_OK;
 }
 

@@ -68,6 +68,8 @@ SIGNATURES = {
     "pdipc_set_v": (ctypes.c_int, [_H, _D, ctypes.c_int]),
     "pdipc_get_counters": (ctypes.c_int, [_H, ctypes.POINTER(Counters)]),
     "pdipc_save_frame": (ctypes.c_int, [_H, ctypes.c_char_p, ctypes.c_int]),
+    "pdipc_save_state": (ctypes.c_int, [_H, ctypes.c_char_p]),
+    "pdipc_load_state": (ctypes.c_int, [_H, ctypes.c_char_p]),
     "pdipc_get_candidates": (ctypes.c_int, [_H, _I, ctypes.c_int, _I]),
     "pdipc_get_patch_pairs": (ctypes.c_int, [_H, _I, ctypes.c_int, _I]),
     "pdipc_get_model_dat": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int, _I]),
@@ -265,6 +267,14 @@ class Simulator:
 
     def save_simulator(self, path, as_double=False):
         _check(lib().pdipc_save_frame(self.h, path.encode(), 1 if as_double else 0))
+
+    def save_state(self, path):
+        """Full-state checkpoint: x and v in FP64 with the scene's topology fingerprint."""
+        _check(lib().pdipc_save_state(self.h, os.fspath(path).encode()))
+
+    def load_state(self, path):
+        """Restores a save_state file of the same scene (raises for another scene)."""
+        _check(lib().pdipc_load_state(self.h, os.fspath(path).encode()))
 
     def candidates(self):
         n = ctypes.c_int()
