@@ -134,6 +134,7 @@ def test_cloth_adjoint_matches_dual_oracle(seed, lanes):
     print("adjoint", got, "\ndual   ", want, "\nstats", sim.adjoint_stats(), flush=True)
     assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
     # the handle is left at the final state and a second pass repeats the first
+    # (to the rounding of the FP64 atomics that gather the barrier adjoint)
     assert np.array_equal(sim.x()[: sim.sim], xg)
     _, dv0b, dkb = sim.backward(dx=2.0 * (xg - target), n_tris=ntri)
     assert np.abs(dkb - dkg).max() <= 1e-7 * np.abs(dkg).max()
@@ -264,7 +265,9 @@ def test_recorded_rollout_matches_checkpoints():
     forward step, gives the checkpointed backward's gradient (the re-run
     segments record the same trajectory) and can be repeated.  A checkpointed
     rollout records its last segment as it runs: the first backward reverses
-    that tape, a second one re-runs the segment, and both give the same bits."""
+    that tape, a second one re-runs the segment (the same trajectory bit for
+    bit), and both give the same gradient up to the rounding of the FP64
+    atomics that gather the barrier adjoint."""
     steps = 9
     sc, v0, target = _cloth_case(1, steps=steps)
     k0 = sc["cloths"][0]["tri_k"].copy()
@@ -281,11 +284,11 @@ def test_recorded_rollout_matches_checkpoints():
         out[every] = (xT, sim.backward(dx=2.0 * (xT - target), n_tris=len(k0)))
         if every:
             for a, b in zip(sim.backward(dx=2.0 * (xT - target), n_tris=len(k0)), out[every][1]):
-                assert np.array_equal(a, b)
+                assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
     (xa, (dxa, dva, dka)), (xb, (dxb, dvb, dkb)) = out[4], out[0]
     assert np.array_equal(xa, xb)
     for a, b in ((dxa, dxb), (dva, dvb), (dka, dkb)):
         assert np.abs(a - b).max() <= 1e-12 * np.abs(a).max()
     again = sim.backward(dx=2.0 * (xb - target), n_tris=len(k0))
     for a, b in zip(again, (dxb, dvb, dkb)):
-        assert np.array_equal(a, b)
+        assert np.abs(a - b).max() <= 1e-13 * np.abs(b).max()
