@@ -17,7 +17,7 @@
 //                an invariant or brute force;
 //   "unpinned" = no independent check exists (stated where it applies).
 //
-// Deliberate deviations from the reference (each also listed in DESIGN.md):
+// Deliberate deviations from the reference (the GPU library makes the same ones):
 //   D1 inflation radius: the reference's per-step broad phase reads an
 //      uninitialised member (PdIpcSimulator.cpp:1154 shadows PdIpcSimulator.h:302);
 //      we use 0.5*dHat in both places, the evident intent.
@@ -307,7 +307,7 @@ static void tet_register_boundary(Model& m) {
 // reference computes in float (qeal = float); so do we, always.  Pinned by
 // patchInfo.dat.  When no extent is strictly smallest the reference leaves
 // (u, v) uninitialised; we then keep the last assignment the chain would make
-// with u = 0, v = 2 (the vertical axis y excluded) and say so in DESIGN.md.
+// with u = 0, v = 2 (the vertical axis y excluded), a stated deviation.
 static void cloth_set_patches(Model& m) {
   float mn[3] = {FLT_MAX, FLT_MAX, FLT_MAX}, mx[3] = {-FLT_MAX, -FLT_MAX, -FLT_MAX};
   auto P = [&](int i, int k) { return (float)m.V[3 * i + k]; };
