@@ -60,6 +60,7 @@ struct DeviceMesh {
   double* r1Val = nullptr;
   // M^2 in SELL layout
   int sellLanes = 4, rowsPerSlice = 16, nSlices = 0;
+  int sellUniformW = 0;  // > 0: every slice has this width (offsets by arithmetic)
   int64_t sellEntries = 0;
   int64_t* sliceOff = nullptr;
   int* sliceW = nullptr;

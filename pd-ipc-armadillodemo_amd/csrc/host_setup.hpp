@@ -95,6 +95,7 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
 // Padding entries point at their own row with no 2-paths.
 struct SellLayout {
   int lanes = 4, rowsPerSlice = 16, nSlices = 0;
+  int uniformW = 0;               // > 0: every slice padded to this width
   std::vector<int64_t> sliceOff;   // nSlices + 1
   std::vector<int> sliceW;         // nSlices: width in 64-entry columns
   std::vector<int> col;            // per SELL entry

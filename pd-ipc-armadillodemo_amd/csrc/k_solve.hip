@@ -80,7 +80,7 @@ struct FirstTermArgs {  // c = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
 // c -> cA, x0 -> xa0, x1 = M^2 x0 + c -> xa1 (and out, SoA, when given).
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_first64(int n, int nSlices, int ld, const int64_t* __restrict__ sliceOff,
-                                                    const int* __restrict__ sliceW, const int* __restrict__ col,
+                                                    const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
                                                     const double* __restrict__ val, const double* __restrict__ X,
                                                     Row64* __restrict__ cA, Row64* __restrict__ xa0,
                                                     Row64* __restrict__ xa1, double* __restrict__ out,
@@ -91,8 +91,8 @@ __global__ void __launch_bounds__(kBlock) k_first64(int n, int nSlices, int ld, 
   constexpr int S = kWave / G;
   int row = slice * S + lane / G;
   int sub = lane % G;
-  int64_t off = sliceOff[slice] + lane;
-  int width = sliceW[slice];
+  const int width = uw > 0 ? uw : sliceW[slice];
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
@@ -206,7 +206,7 @@ __global__ void __launch_bounds__(kBlock) k_res_b(int n, double w, const int* __
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, double W,
                                                     const int64_t* __restrict__ sliceOff,
-                                                    const int* __restrict__ sliceW, const int* __restrict__ col,
+                                                    const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
                                                     const double* __restrict__ val, const Row64* __restrict__ cur,
                                                     const Row64* __restrict__ pre, const Row64* __restrict__ cA,
                                                     Row64* __restrict__ next, double* __restrict__ out) {
@@ -216,8 +216,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
   if (slice >= nSlices) return;
   constexpr int S = kWave / G;
   int row = slice * S + lane / G;  // < ld: slices cover at most ld rows
-  int64_t off = sliceOff[slice] + lane;
-  int width = sliceW[slice];
+  const int width = uw > 0 ? uw : sliceW[slice];
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
   Row64 p = pre[row], c = cA[row];  // loaded up front, with the matrix
   double acc[3] = {0.0, 0.0, 0.0};
   for (int c0 = 0; c0 < width; c0 += U) {
@@ -272,7 +272,7 @@ __global__ void k_row64_to_soa(int n, int ld, const Row64* __restrict__ a, doubl
 template <int G, int LAST, bool C16>
 __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, float W,
                                                     const int64_t* __restrict__ sliceOff,
-                                                    const int* __restrict__ sliceW, const int* __restrict__ col,
+                                                    const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
                                                     const int16_t* __restrict__ col16, const float* __restrict__ val,
                                                     const float4* __restrict__ dcur, const float4* __restrict__ dpre,
                                                     const float4* __restrict__ r0, float4* __restrict__ dnext,
@@ -283,8 +283,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
   if (slice >= nSlices) return;
   constexpr int S = kWave / G;
   int row = slice * S + lane / G;  // < ld: slices cover at most ld rows
-  int64_t off = sliceOff[slice] + lane;
-  int width = sliceW[slice];
+  const int width = uw > 0 ? uw : sliceW[slice];
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
   // the row's own operands, loaded up front so they arrive with the matrix
   float4 p = dpre[row], r = r0[row];
   float acc[3] = {0.f, 0.f, 0.f};
@@ -337,10 +337,10 @@ void launch_sweep32(const DeviceMesh& m, int grid, float W, const float4* cur, c
                     hipStream_t s) {
   if (m.m2Col16)
     hipLaunchKernelGGL((k_sweep32<G, LAST, true>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W,
-                       m.sliceOff, m.sliceW, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
+                       m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
   else
     hipLaunchKernelGGL((k_sweep32<G, LAST, false>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W,
-                       m.sliceOff, m.sliceW, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
+                       m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
 }
 
 // The FP64 solve: x_0 = X, iterates rotate through xa0..xa2, the last one is
@@ -354,7 +354,7 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   double r = rho * rho;
   double W = 2.0 / (2.0 - r);
   hipLaunchKernelGGL((k_first64<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff, m.sliceW,
-                     m.m2Col, m.m2Val, m.X, m.cA, m.xa0, m.xa1, outs ? outs[1] : nullptr, ft);
+                     m.sellUniformW, m.m2Col, m.m2Val, m.X, m.cA, m.xa0, m.xa1, outs ? outs[1] : nullptr, ft);
   Row64 *pre = m.xa0, *cur = m.xa1, *next = m.xa2;
   if (iters <= 1) {
     hipLaunchKernelGGL(k_row64_to_soa, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, cur,
@@ -364,7 +364,7 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   for (int it = 1; it < iters; it++) {
     double* out = outs ? outs[it + 1] : (it == iters - 1 ? m.X : nullptr);
     hipLaunchKernelGGL((k_sweep64<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff,
-                       m.sliceW, m.m2Col, m.m2Val, cur, pre, m.cA, next, out);
+                       m.sliceW, m.sellUniformW, m.m2Col, m.m2Val, cur, pre, m.cA, next, out);
     W = 4.0 / (4.0 - r * W);
     Row64* t = pre; pre = cur; cur = next; next = t;
   }
@@ -414,7 +414,7 @@ void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precis
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_adj_spmv(int n, int nSlices, int ld, double a, double b,
                                                      const int64_t* __restrict__ sliceOff,
-                                                     const int* __restrict__ sliceW, const int* __restrict__ col,
+                                                     const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
                                                      const double* __restrict__ val, const double* __restrict__ uin,
                                                      const double* __restrict__ uoth, double* __restrict__ uout,
                                                      double* __restrict__ uc) {
@@ -424,8 +424,8 @@ __global__ void __launch_bounds__(kBlock) k_adj_spmv(int n, int nSlices, int ld,
   constexpr int S = kWave / G;
   int row = slice * S + lane / G;
   int sub = lane % G;
-  int64_t off = sliceOff[slice] + lane;
-  int width = sliceW[slice];
+  const int width = uw > 0 ? uw : sliceW[slice];
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
@@ -520,7 +520,7 @@ void adj_pass_impl(const DeviceMesh& m, const AdjMesh& a, const double* W, int K
   for (int k = K - 1; k >= 0; k--) {
     double b = k + 1 <= K - 1 ? 1.0 - W[k + 1] : 0.0;
     hipLaunchKernelGGL((k_adj_spmv<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W[k], b, m.sliceOff,
-                       m.sliceW, m.m2Col, m.m2Val, u[k + 1], u[k + 2], u[k], uc);
+                       m.sliceW, m.sellUniformW, m.m2Col, m.m2Val, u[k + 1], u[k + 2], u[k], uc);
   }
   GbarArgs ga;
   ga.K = K;
@@ -659,7 +659,7 @@ void one_sweep_impl(const DeviceMesh& m, int precision, hipStream_t s) {
   const double W = 1.9;
   if (precision == 0)
     hipLaunchKernelGGL((k_sweep64<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff,
-                       m.sliceW, m.m2Col, m.m2Val, m.xa1, m.xa0, m.cA, m.xa2, nullptr);
+                       m.sliceW, m.sellUniformW, m.m2Col, m.m2Val, m.xa1, m.xa0, m.cA, m.xa2, nullptr);
   else
     launch_sweep32<G, 0>(m, grid, (float)W, m.dCur, m.dPre, m.dNext, s);
 }

@@ -121,6 +121,7 @@ int upload_all(pdipc_handle* h) {
   UP(r1Off, hs.r1Off); UP(r1Col, hs.r1Col); UP(r1Val, hs.r1Val);
   const SellLayout& L = h->sell;
   m.sellLanes = L.lanes;
+  m.sellUniformW = L.uniformW;
   m.rowsPerSlice = L.rowsPerSlice;
   m.nSlices = L.nSlices;
   m.sellEntries = L.sliceOff.back();
