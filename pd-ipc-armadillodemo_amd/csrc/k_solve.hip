@@ -78,10 +78,10 @@ struct FirstTermArgs {  // c = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
 // The first FP64 sweep of a solve, reading x0 = X (SoA) and computing c on the
 // fly (the first term is fused: the G lanes of a row split its R1 entries):
 // c -> cA, x0 -> xa0, x1 = M^2 x0 + c -> xa1 (and out, SoA, when given).
-template <int G>
+template <int G, bool C16>
 __global__ void __launch_bounds__(kBlock) k_first64(int n, int nSlices, int ld, const int64_t* __restrict__ sliceOff,
                                                     const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
-                                                    const double* __restrict__ val, const double* __restrict__ X,
+                                                    const int16_t* __restrict__ col16, const double* __restrict__ val, const double* __restrict__ X,
                                                     Row64* __restrict__ cA, Row64* __restrict__ xa0,
                                                     Row64* __restrict__ xa1, double* __restrict__ out,
                                                     FirstTermArgs ft) {
@@ -97,7 +97,7 @@ __global__ void __launch_bounds__(kBlock) k_first64(int n, int nSlices, int ld, 
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
     int64_t e = off + (int64_t)c * kWave;
-    int j = col[e];
+    int j = C16 ? row + (int)col16[e] : col[e];
     double v = val[e];
     acc[0] += v * X[j];
     acc[1] += v * X[ld + j];
@@ -203,11 +203,11 @@ __global__ void __launch_bounds__(kBlock) k_res_b(int n, double w, const int* __
 // FP64 Chebyshev step x_{k+1} = W (M^2 x_k + c - x_{k-1}) + x_{k-1} on Row64
 // vectors (one 32-byte gather per matrix entry, U entries of a lane fetched
 // together); out (SoA), when given, receives x_{k+1} as well.
-template <int G>
+template <int G, bool C16>
 __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, double W,
                                                     const int64_t* __restrict__ sliceOff,
                                                     const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
-                                                    const double* __restrict__ val, const Row64* __restrict__ cur,
+                                                    const int16_t* __restrict__ col16, const double* __restrict__ val, const Row64* __restrict__ cur,
                                                     const Row64* __restrict__ pre, const Row64* __restrict__ cA,
                                                     Row64* __restrict__ next, double* __restrict__ out) {
   constexpr int U = 4;
@@ -229,7 +229,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
       v[u] = 0.0;
       if (c0 + u < width) {
         int64_t e = off + (int64_t)(c0 + u) * kWave;
-        j[u] = col[e];
+        j[u] = C16 ? row + (int)col16[e] : col[e];
         v[u] = val[e];
       }
     }
@@ -353,8 +353,10 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   const FirstTermArgs ft{m.r1Off, m.r1Col, m.r1Val, m.sysDiag, m.sysRhs, w};
   double r = rho * rho;
   double W = 2.0 / (2.0 - r);
-  hipLaunchKernelGGL((k_first64<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, m.sliceOff, m.sliceW,
-                     m.sellUniformW, m.m2Col, m.m2Val, m.X, m.cA, m.xa0, m.xa1, outs ? outs[1] : nullptr, ft);
+  const bool c16 = m.m2Col16 != nullptr;
+  hipLaunchKernelGGL((c16 ? k_first64<G, true> : k_first64<G, false>), dim3(grid), dim3(kBlock), 0, s, m.nSim,
+                     m.nSlices, m.ld, m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16, m.m2Val, m.X, m.cA,
+                     m.xa0, m.xa1, outs ? outs[1] : nullptr, ft);
   Row64 *pre = m.xa0, *cur = m.xa1, *next = m.xa2;
   if (iters <= 1) {
     hipLaunchKernelGGL(k_row64_to_soa, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, cur,
@@ -363,8 +365,9 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   }
   for (int it = 1; it < iters; it++) {
     double* out = outs ? outs[it + 1] : (it == iters - 1 ? m.X : nullptr);
-    hipLaunchKernelGGL((k_sweep64<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff,
-                       m.sliceW, m.sellUniformW, m.m2Col, m.m2Val, cur, pre, m.cA, next, out);
+    hipLaunchKernelGGL((c16 ? k_sweep64<G, true> : k_sweep64<G, false>), dim3(grid), dim3(kBlock), 0, s, m.nSim,
+                       m.nSlices, m.ld, W, m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16, m.m2Val, cur,
+                       pre, m.cA, next, out);
     W = 4.0 / (4.0 - r * W);
     Row64* t = pre; pre = cur; cur = next; next = t;
   }
@@ -411,10 +414,11 @@ void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precis
 // forward SELL SpMV on u,
 //   u_out = a M u_in + b u_oth,   uc[row] += a u_in[row]   (uc: D^{-1}-scaled first-term adjoint);
 // the M^2-entry adjoints are gathered once per solve by k_adj_gbar.
-template <int G>
+template <int G, bool C16>
 __global__ void __launch_bounds__(kBlock) k_adj_spmv(int n, int nSlices, int ld, double a, double b,
                                                      const int64_t* __restrict__ sliceOff,
                                                      const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
+                                                     const int16_t* __restrict__ col16,
                                                      const double* __restrict__ val, const double* __restrict__ uin,
                                                      const double* __restrict__ uoth, double* __restrict__ uout,
                                                      double* __restrict__ uc) {
@@ -430,7 +434,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_spmv(int n, int nSlices, int ld,
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
     int64_t e = off + (int64_t)c * kWave;
-    int j = col[e];
+    int j = C16 ? row + (int)col16[e] : col[e];
     double v = val[e];
     acc[0] += v * uin[j];
     acc[1] += v * uin[ld + j];
@@ -519,8 +523,9 @@ void adj_pass_impl(const DeviceMesh& m, const AdjMesh& a, const double* W, int K
   const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
   for (int k = K - 1; k >= 0; k--) {
     double b = k + 1 <= K - 1 ? 1.0 - W[k + 1] : 0.0;
-    hipLaunchKernelGGL((k_adj_spmv<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W[k], b, m.sliceOff,
-                       m.sliceW, m.sellUniformW, m.m2Col, m.m2Val, u[k + 1], u[k + 2], u[k], uc);
+    hipLaunchKernelGGL((m.m2Col16 ? k_adj_spmv<G, true> : k_adj_spmv<G, false>), dim3(grid), dim3(kBlock), 0, s,
+                       m.nSim, m.nSlices, m.ld, W[k], b, m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16,
+                       m.m2Val, u[k + 1], u[k + 2], u[k], uc);
   }
   GbarArgs ga;
   ga.K = K;
@@ -658,8 +663,9 @@ void one_sweep_impl(const DeviceMesh& m, int precision, hipStream_t s) {
   const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
   const double W = 1.9;
   if (precision == 0)
-    hipLaunchKernelGGL((k_sweep64<G>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff,
-                       m.sliceW, m.sellUniformW, m.m2Col, m.m2Val, m.xa1, m.xa0, m.cA, m.xa2, nullptr);
+    hipLaunchKernelGGL((m.m2Col16 ? k_sweep64<G, true> : k_sweep64<G, false>), dim3(grid), dim3(kBlock), 0, s,
+                       m.nSim, m.nSlices, m.ld, W, m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16, m.m2Val,
+                       m.xa1, m.xa0, m.cA, m.xa2, nullptr);
   else
     launch_sweep32<G, 0>(m, grid, (float)W, m.dCur, m.dPre, m.dNext, s);
 }

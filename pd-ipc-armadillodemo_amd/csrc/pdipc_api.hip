@@ -886,7 +886,7 @@ int pdipc_time_kernel(pdipc_handle* h, int which, int reps, double* avg_us, doub
   double N = m.nSim, nnz = (double)h->hs.nnzR2, D3 = 3.0 * N;
   double b = 0;
   switch (which) {
-    case 0: b = nnz * 12.0 + D3 * 8.0 * 4.0; break;            // col+val FP64; x_k, x_{k-1}, c, x_{k+1}
+    case 0: b = nnz * (m.m2Col16 ? 10.0 : 12.0) + D3 * 8.0 * 4.0; break;  // col (16/32-bit)+val FP64; x_k, x_{k-1}, c, x_{k+1}
     case 1: b = nnz * (m.m2Col16 ? 6.0 : 8.0) + D3 * 4.0 * 4.0; break;  // col (16/32-bit)+val FP32; d_k, d_{k-1}, r0, d_{k+1}
     case 2: {
       double slots = (double)h->hs.cNbr.size();
