@@ -81,8 +81,9 @@ struct FirstTermArgs {  // c = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
 template <int G, bool C16>
 __global__ void __launch_bounds__(kBlock) k_first64(int n, int nSlices, int ld, const int64_t* __restrict__ sliceOff,
                                                     const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
-                                                    const int16_t* __restrict__ col16, const double* __restrict__ val, const double* __restrict__ X,
-                                                    Row64* __restrict__ cA, Row64* __restrict__ xa0,
+                                                    const int16_t* __restrict__ col16, const double* __restrict__ val,
+                                                    const double* __restrict__ X, Row64* __restrict__ cA,
+                                                    Row64* __restrict__ xa0,
                                                     Row64* __restrict__ xa1, double* __restrict__ out,
                                                     FirstTermArgs ft) {
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -181,6 +182,10 @@ __global__ void __launch_bounds__(kBlock) k_res_b(int n, double w, const int* __
                                                   float4* __restrict__ dnextF, float4* __restrict__ r0F) {
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int row = gt / G, sub = gt % G;
+  // the row's own operands first, so that they arrive with the gathers
+  const int rr = row < n ? row : n - 1;
+  const double dr = D[rr];
+  const Row64 p = pt[rr];
   double sv[3] = {0.0, 0.0, 0.0};
   if (row < n)
     for (int q = off[row] + sub; q < off[row + 1]; q += G) {
@@ -192,8 +197,7 @@ __global__ void __launch_bounds__(kBlock) k_res_b(int n, double w, const int* __
     }
   group_reduce<G>(sv);
   if (sub != 0 || row >= n) return;
-  double f = w / D[row];
-  Row64 p = pt[row];
+  double f = w / dr;
   float4 rv = make_float4((float)(p.x + f * sv[0]), (float)(p.y + f * sv[1]), (float)(p.z + f * sv[2]), 0.0f);
   r0F[row] = rv;
   dpreF[row] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
@@ -207,8 +211,9 @@ template <int G, bool C16>
 __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, double W,
                                                     const int64_t* __restrict__ sliceOff,
                                                     const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
-                                                    const int16_t* __restrict__ col16, const double* __restrict__ val, const Row64* __restrict__ cur,
-                                                    const Row64* __restrict__ pre, const Row64* __restrict__ cA,
+                                                    const int16_t* __restrict__ col16, const double* __restrict__ val,
+                                                    const Row64* __restrict__ cur, const Row64* __restrict__ pre,
+                                                    const Row64* __restrict__ cA,
                                                     Row64* __restrict__ next, double* __restrict__ out) {
   constexpr int U = 4;
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
@@ -286,7 +291,14 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
   const int width = uw > 0 ? uw : sliceW[slice];
   const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
   // the row's own operands, loaded up front so they arrive with the matrix
+  // (the last sweep of a solve also adds into X: its X too)
   float4 p = dpre[row], r = r0[row];
+  double xo[3] = {0.0, 0.0, 0.0};
+  if (LAST) {
+    xo[0] = X[row];
+    xo[1] = X[ld + row];
+    xo[2] = X[2 * ld + row];
+  }
   float acc[3] = {0.f, 0.f, 0.f};
   for (int c0 = 0; c0 < width; c0 += U) {
     int j[U];
@@ -315,9 +327,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
   float d1 = W * (acc[1] + r.y - p.y) + p.y;
   float d2 = W * (acc[2] + r.z - p.z) + p.z;
   if (LAST) {
-    X[row] += (double)d0;
-    X[ld + row] += (double)d1;
-    X[2 * ld + row] += (double)d2;
+    X[row] = xo[0] + (double)d0;
+    X[ld + row] = xo[1] + (double)d1;
+    X[2 * ld + row] = xo[2] + (double)d2;
   } else {
     dnext[row] = make_float4(d0, d1, d2, 0.0f);
   }
@@ -417,8 +429,8 @@ void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precis
 template <int G, bool C16>
 __global__ void __launch_bounds__(kBlock) k_adj_spmv(int n, int nSlices, int ld, double a, double b,
                                                      const int64_t* __restrict__ sliceOff,
-                                                     const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
-                                                     const int16_t* __restrict__ col16,
+                                                     const int* __restrict__ sliceW, int uw,
+                                                     const int* __restrict__ col, const int16_t* __restrict__ col16,
                                                      const double* __restrict__ val, const double* __restrict__ uin,
                                                      const double* __restrict__ uoth, double* __restrict__ uout,
                                                      double* __restrict__ uc) {
