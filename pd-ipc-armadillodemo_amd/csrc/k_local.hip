@@ -358,27 +358,38 @@ __global__ void k_adj_cloth_a(int nCloth, int ld, const int* __restrict__ cCente
                               const double* __restrict__ cH0, const double* __restrict__ cBendOut,
                               const double* __restrict__ X, const double* __restrict__ bbar,
                               double* __restrict__ Aadj) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= nCloth) return;
+  // kRhsLanes lanes per cloth vertex share its ring slots (lanes past nCloth
+  // run on the last vertex without writing)
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sub = gt % kRhsLanes;
+  const bool live = gt / kRhsLanes < nCloth;
+  const int c = live ? gt / kRhsLanes : nCloth - 1;
   int i = cCenter[c];
   int o = cOff[c], e = cOff[c + 1];
   double ab[3] = {0, 0, 0};
-  for (int s = o; s < e; s++) {
+  for (int s = o + sub; s < e; s += kRhsLanes) {
     int j = cNbr[s];
     double b = cBendOut[s];
 #pragma unroll
     for (int k = 0; k < 3; k++) ab[k] += b * bbar[k * ld + j];
   }
   double xi[3] = {X[i], X[ld + i], X[2 * ld + i]};
-  double lsum = cLbo[o];
+  double lsum = sub == 0 ? cLbo[o] : 0.0;
   double a[3] = {0, 0, 0};
-  for (int s = o + 1; s < e; s++) {
+  for (int s = o + 1 + sub; s < e; s += kRhsLanes) {
     int j = cNbr[s];
     double l = cLbo[s];
     lsum += l;
 #pragma unroll
     for (int k = 0; k < 3; k++) a[k] += l * (X[k * ld + j] - xi[k]);
   }
+  lsum = rhs_lanes_sum(lsum);
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    ab[k] = rhs_lanes_sum(ab[k]);
+    a[k] = rhs_lanes_sum(a[k]);
+  }
+  if (!live || sub != 0) return;
 #pragma unroll
   for (int k = 0; k < 3; k++) a[k] += lsum * xi[k];
   double n = sqrt(a[0] * a[0] + a[1] * a[1] + a[2] * a[2]);
@@ -519,12 +530,21 @@ __global__ void k_adj_gather(int nSim, int ld, int nT, const double* __restrict_
                              const double* __restrict__ cLboIn, const double* __restrict__ Aadj,
                              const double* __restrict__ X, const double* __restrict__ bbar, double* __restrict__ ax,
                              double* __restrict__ sbar, double* __restrict__ rhs0bar, double* __restrict__ colRbar) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= nSim) return;
+  // kRhsLanes lanes per point share its tet corners and ring slots; lanes past
+  // nSim run on the last point without writing, so every lane takes part in the
+  // reduction; the point's own term enters once, on lane 0
+  const int gt = blockIdx.x * blockDim.x + threadIdx.x;
+  const int sub = gt % kRhsLanes;
+  const bool live = gt / kRhsLanes < nSim;
+  const int i = live ? gt / kRhsLanes : nSim - 1;
   double di = D[i];
   double bi[3] = {bbar[i], bbar[ld + i], bbar[2 * ld + i]};
-  double r[3] = {di * u0[i], di * u0[ld + i], di * u0[2 * ld + i]};
-  for (int q = vtOff[i]; q < vtOff[i + 1]; q++) {
+  double r[3] = {0.0, 0.0, 0.0};
+  if (sub == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) r[k] = di * u0[k * ld + i];
+  }
+  for (int q = vtOff[i] + sub; q < vtOff[i + 1]; q += kRhsLanes) {
     int tc = vtList[q];
     int t = tc >> 2, corner = tc & 3;
 #pragma unroll
@@ -534,7 +554,7 @@ __global__ void k_adj_gather(int nSim, int ld, int nT, const double* __restrict_
   if (c >= 0) {
     int o = cOff[c], e = cOff[c + 1];
     double xi[3] = {X[i], X[ld + i], X[2 * ld + i]};
-    for (int s = o + 1; s < e; s++) {
+    for (int s = o + 1 + sub; s < e; s += kRhsLanes) {
       int j = cNbr[s];
       double d[3], db[3];
 #pragma unroll
@@ -548,15 +568,18 @@ __global__ void k_adj_gather(int nSim, int ld, int nT, const double* __restrict_
       double pr = n[0] * db[0] + n[1] * db[1] + n[2] * db[2];
 #pragma unroll
       for (int k = 0; k < 3; k++) r[k] += coef * (db[k] - n[k] * pr);
-      sbar[s] += cRest[s] * (n[0] * bi[0] + n[1] * bi[1] + n[2] * bi[2]);
+      if (live) sbar[s] += cRest[s] * (n[0] * bi[0] + n[1] * bi[1] + n[2] * bi[2]);
     }
-    for (int s = o; s < e; s++) {
+    for (int s = o + sub; s < e; s += kRhsLanes) {
       int j = cNbr[s];
       double l = cLboIn[s];
 #pragma unroll
       for (int k = 0; k < 3; k++) r[k] += l * Aadj[k * ld + j];
     }
   }
+#pragma unroll
+  for (int k = 0; k < 3; k++) r[k] = rhs_lanes_sum(r[k]);
+  if (!live || sub != 0) return;
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     ax[k * ld + i] = r[k];
@@ -571,14 +594,14 @@ void launch_adj_local(const DeviceMesh& m, const AdjMesh& a, double sigLo, doubl
                       const double* bbar, const double* u0, double* aA, double* tetAdj, double* ax, double* sbar,
                       double* rhs0bar, double* colRbar, hipStream_t s) {
   if (m.nCloth > 0)
-    hipLaunchKernelGGL(k_adj_cloth_a, dim3(grid_for(m.nCloth)), dim3(kBlock), 0, s, m.nCloth, m.ld, m.cCenter, m.cOff,
-                       m.cNbr, m.cLbo, m.cH0, a.cBendOut, x, bbar, aA);
+    hipLaunchKernelGGL(k_adj_cloth_a, dim3(grid_for((int64_t)m.nCloth * kRhsLanes)), dim3(kBlock), 0, s, m.nCloth,
+                       m.ld, m.cCenter, m.cOff, m.cNbr, m.cLbo, m.cH0, a.cBendOut, x, bbar, aA);
   if (m.nTets > 0)
     hipLaunchKernelGGL(k_adj_tet, dim3(grid_for(m.nTets)), dim3(kBlock), 0, s, m.nTets, m.ld, sigLo, sigHi, m.tetIdx,
                        m.tetDminv, m.tetW, x, bbar, tetAdj);
-  hipLaunchKernelGGL(k_adj_gather, dim3(grid_for(m.nSim)), dim3(kBlock), 0, s, m.nSim, m.ld, m.nTets, m.sysDiag, u0,
-                     m.vtOff, m.vtList, tetAdj, m.clothOfPoint, m.cOff, m.cNbr, m.cRest, m.cStrain, a.cLboIn, aA, x,
-                     bbar, ax, sbar, rhs0bar, colRbar);
+  hipLaunchKernelGGL(k_adj_gather, dim3(grid_for((int64_t)m.nSim * kRhsLanes)), dim3(kBlock), 0, s, m.nSim, m.ld,
+                     m.nTets, m.sysDiag, u0, m.vtOff, m.vtList, tetAdj, m.clothOfPoint, m.cOff, m.cNbr, m.cRest,
+                     m.cStrain, a.cLboIn, aA, x, bbar, ax, sbar, rhs0bar, colRbar);
 }
 
 void launch_local(const DeviceMesh& m, int precision, double sigLo, double sigHi, int withCollision, hipStream_t s,
