@@ -113,7 +113,7 @@ def _host_threads():
     return len(os.sched_getaffinity(0))
 
 
-def cpu_baseline(scene, x, v, seconds_budget=20.0):
+def cpu_baseline(scene, x, v, seconds_budget=20.0, label="256x256 cloth"):
     """The FP64 oracle (the C++ restatement of the reference step; OpenMP over
     the solve, operator update, TOI and barrier-activity loops) on this host, on
     a bounded sample: the full-size scene from the same warmed-up state the GPU
@@ -143,7 +143,7 @@ def cpu_baseline(scene, x, v, seconds_budget=20.0):
     oracle.set_threads(0)
     return dict(value=n / t, unit="steps/s", cores=threads, kind="port",
                 single_thread_steps_per_s=n1 / t1,
-                sample=f"FP64 oracle steps of the same 256x256 scene from the GPU timing's warmed-up state: "
+                sample=f"FP64 oracle steps of the same scene ({label}) from the GPU timing's warmed-up state: "
                        f"{n} steps on {threads} threads ({t:.1f} s), {n1} steps on 1 thread ({t1:.1f} s); "
                        f"setup {init_s:.1f} s each")
 
@@ -228,8 +228,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=60)
     ap.add_argument("--warmup", type=int, default=20)
-    ap.add_argument("--scene", choices=("cloth", "tet_block", "stacked"), default="cloth",
-                    help="cloth: the n x n cloth over the sphere (default); tet_block: the 40^3-cell block "
+    ap.add_argument("--scene", choices=("cloth", "config0", "tet_block", "stacked"), default="cloth",
+                    help="cloth: the n x n cloth over the sphere (default); config0: the 64x64 cloth over the "
+                         "subdivision-4 sphere without ground; tet_block: the 40^3-cell block "
                          "(320,000 tets) under a 128x128 cloth, dt = 1/200, 15 local/global iterations; "
                          "stacked: the scale run, 3 stacked 1024^2 cloths, FP32-split only, --steps steps")
     ap.add_argument("--n", type=int, default=256, help="cloth vertices per side")
@@ -267,7 +268,11 @@ def main():
         print(json.dumps(adjoint_bench(args.graphs, args.adjoint_fd)), flush=True)
         return
 
-    if args.scene == "tet_block":
+    if args.scene == "config0":
+        scene = scenes.config0()
+        workload = ("cloth 64x64 (7,938 triangles) over icosphere subdiv 4 (5,120 triangles), no ground, "
+                    "dt=1/100, 10 LG x 15 Chebyshev, dHat=1e-3, kappa=1e3")
+    elif args.scene == "tet_block":
         scene = scenes.config2()
         workload = ("tet block 40^3 cells (320,000 tets) + 128x128 cloth on the ground, dt=1/200, "
                     "15 LG x 15 Chebyshev, dHat=1e-3, kappa=1e3")
@@ -330,7 +335,9 @@ def main():
 
     cpu = None
     if not args.no_cpu:
-        cpu = cpu_baseline(scene, x_w, v_w, args.cpu_seconds)
+        label = {"config0": "64x64 cloth", "tet_block": "tet block + 128x128 cloth"}.get(args.scene,
+                                                                                    f"{args.n}x{args.n} cloth")
+        cpu = cpu_baseline(scene, x_w, v_w, args.cpu_seconds, label)
         log(f"cpu baseline {cpu}")
 
     steps = args.steps

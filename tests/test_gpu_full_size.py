@@ -1,6 +1,7 @@
 """GPU parity at the benchmark sizes: one step of the 256x256 cloth over the
-subdivision-6 sphere and ground in contact, and one step of the 40^3-cell tet
-block (320,000 tets) with a 128x128 cloth, against the oracle from identical
+subdivision-6 sphere and ground in contact, one step of the 40^3-cell tet
+block (320,000 tets) with a 128x128 cloth, and all 100 steps of the 64x64
+cloth over the subdivision-4 sphere, against the oracle from identical
 states.  Bars as in test_gpu_parity: patch pairs bit-exact as sets, FP64
 positions within 1e-8 relative per step (or 20x the oracle's own sensitivity
 at an ill-conditioned state), FP32-split within 1e-5 of the FP64 oracle."""
@@ -75,3 +76,30 @@ def test_config2_tet_block_step():
     e32 = rel(sim32.x()[: ref.sim], ref.x()[: ref.sim])
     print("config-2 FP32-split step error", e32)
     assert e32 < 1e-5
+
+
+def test_config0_hundred_steps_teacher_forced():
+    """The CPU reference path's scene (64x64 cloth over the subdivision-4
+    sphere, no ground) for its full 100 steps: every FP64 step of the HIP path
+    from the oracle's state at the FP64 bar, with the oracle's outer-iteration
+    and candidate counts."""
+    sc = scenes.config0()
+    ref = oracle.Oracle(sc)
+    sim = pdipc.Simulator.from_scene(sc)
+    worst, contact = 0.0, 0
+    for k in range(100):
+        if k % 20 == 0:
+            print(f"case: config-0 FP64 step {k}", flush=True)
+        x, v = ref.x(), ref.v()
+        sim.set_x(x)
+        sim.set_v(v)
+        ref.step(1)
+        sim.step(1)
+        err, _ = check_step(sc, x, v, sim.x(), ref.x(), ref.sim, seed=k)
+        worst = max(worst, err)
+        c, o = sim.counters(), ref.counters()
+        assert c["outer_iters"] == o["outer"], (k, c["outer_iters"], o["outer"])
+        assert c["candidates"] == o["candidates"], k
+        contact += c["active_pairs"] > 0
+    print("config-0: worst FP64 step error", worst, "steps with active barrier pairs", contact)
+    assert contact > 0  # the cloth reaches the obstacle within the 100 steps
