@@ -76,11 +76,13 @@ __device__ inline void svd3(const T F[9], T U[9], T sig[3], T V[9]) {
   if (lam[o1] > lam[o0]) { tmp = o0; o0 = o1; o1 = tmp; }
   if (lam[o2] > lam[o0]) { tmp = o0; o0 = o2; o2 = tmp; }
   if (lam[o2] > lam[o1]) { tmp = o1; o1 = o2; o2 = tmp; }
-  int ord[3] = {o0, o1, o2};
+  // columns reordered with selects (a run-time index into W would put it in
+  // scratch memory)
+  const int ord[3] = {o0, o1, o2};
 #pragma unroll
   for (int k = 0; k < 3; k++)
 #pragma unroll
-    for (int c = 0; c < 3; c++) V[3 * k + c] = W[3 * k + ord[c]];
+    for (int c = 0; c < 3; c++) V[3 * k + c] = ord[c] == 0 ? W[3 * k] : (ord[c] == 1 ? W[3 * k + 1] : W[3 * k + 2]);
   T detV = V[0] * (V[4] * V[8] - V[5] * V[7]) - V[1] * (V[3] * V[8] - V[5] * V[6]) + V[2] * (V[3] * V[7] - V[4] * V[6]);
   if (detV < 0) { V[2] = -V[2]; V[5] = -V[5]; V[8] = -V[8]; }
   T b[3][3];
@@ -99,8 +101,8 @@ __device__ inline void svd3(const T F[9], T U[9], T sig[3], T V[9]) {
     u1[0] /= s1; u1[1] /= s1; u1[2] /= s1;
   } else {
     T ax = fabs(u0[0]), ay = fabs(u0[1]), az = fabs(u0[2]);
-    T t[3] = {0, 0, 0};
-    t[(ax < ay) ? (ax < az ? 0 : 2) : (ay < az ? 1 : 2)] = 1;
+    const int m = (ax < ay) ? (ax < az ? 0 : 2) : (ay < az ? 1 : 2);
+    const T t[3] = {m == 0 ? (T)1 : (T)0, m == 1 ? (T)1 : (T)0, m == 2 ? (T)1 : (T)0};
     u1[0] = u0[1] * t[2] - u0[2] * t[1];
     u1[1] = u0[2] * t[0] - u0[0] * t[2];
     u1[2] = u0[0] * t[1] - u0[1] * t[0];
@@ -411,9 +413,10 @@ __global__ void k_adj_cloth_a(int nCloth, int ld, const int* __restrict__ cCente
 //   A = (g_j - g_i)/(s_j - s_i),  B = (g_i + g_j)/(s_i + s_j)
 // (the differential of U diag(g) V^T through dU = U Omega_U, dV = V Omega_V);
 // the adjoint swaps Pbar_ij, Pbar_ji in the same combination.
-__global__ void k_adj_tet(int nT, int ld, double sigLo, double sigHi, const int4* __restrict__ tets,
-                          const double* __restrict__ dminv, const double* __restrict__ w,
-                          const double* __restrict__ X, const double* __restrict__ bbar, double* __restrict__ out) {
+__global__ void __launch_bounds__(kBlock) k_adj_tet(int nT, int ld, double sigLo, double sigHi,
+                                                   const int4* __restrict__ tets, const double* __restrict__ dminv,
+                                                   const double* __restrict__ w, const double* __restrict__ X,
+                                                   const double* __restrict__ bbar, double* __restrict__ out) {
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= nT) return;
   int4 q = tets[t];
@@ -452,11 +455,13 @@ __global__ void k_adj_tet(int nT, int ld, double sigLo, double sigHi, const int4
   }
   double g[3], dg[3];
 #pragma unroll
-  for (int k = 0; k < 3; k++) {
-    double v = sv[src[k]];
+  for (int k = 0; k < 3; k++) {  // selects, not run-time indices (which would use scratch memory)
+    const int sk = src[k];
+    double v = sk == 0 ? sv[0] : (sk == 1 ? sv[1] : sv[2]);
+    double sg = sk == 0 ? sgn[0] : (sk == 1 ? sgn[1] : sgn[2]);
     double r = v < sigHi ? v : sigHi;
     g[k] = r < sigLo ? sigLo : r;
-    dg[k] = (v < sigHi && !(v < sigLo)) ? sgn[src[k]] : 0.0;  // d g_k / d s_{src[k]}
+    dg[k] = (v < sigHi && !(v < sigLo)) ? sg : 0.0;  // d g_k / d s_{src[k]}
   }
   // Pbar
   double h[4][3];
@@ -491,7 +496,10 @@ __global__ void k_adj_tet(int nT, int ld, double sigLo, double sigHi, const int4
     for (int j = 0; j < 3; j++) Pt[3 * i + j] = T1[3 * i] * V[j] + T1[3 * i + 1] * V[3 + j] + T1[3 * i + 2] * V[6 + j];
   double Ft[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
 #pragma unroll
-  for (int k = 0; k < 3; k++) Ft[4 * src[k]] += dg[k] * Pt[4 * k];
+  for (int k = 0; k < 3; k++)
+#pragma unroll
+    for (int j = 0; j < 3; j++)
+      if (src[k] == j) Ft[4 * j] += dg[k] * Pt[4 * k];
 #pragma unroll
   for (int i = 0; i < 3; i++)
 #pragma unroll
