@@ -82,13 +82,15 @@ struct FirstTermArgs {  // c = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
 // fly (the first term is fused: the G lanes of a row split its R1 entries):
 // c -> cA, x0 -> xa0, x1 = M^2 x0 + c -> xa1 (and out, SoA, when given).
 template <int G, bool C16>
-__global__ void __launch_bounds__(kBlock) k_first64(int n, int nSlices, int ld, const int64_t* __restrict__ sliceOff,
-                                                    const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
-                                                    const int16_t* __restrict__ col16, const double* __restrict__ val,
-                                                    const double* __restrict__ X, Row64* __restrict__ cA,
-                                                    Row64* __restrict__ xa0,
-                                                    Row64* __restrict__ xa1, double* __restrict__ out,
-                                                    FirstTermArgs ft) {
+// (at most 64 registers: eight wavefronts per SIMD, so a 65 k-row sweep runs in one round)
+__global__ void __launch_bounds__(kBlock, 8) k_first64(int n, int nSlices, int ld,
+                                                       const int64_t* __restrict__ sliceOff,
+                                                       const int* __restrict__ sliceW, int uw,
+                                                       const int* __restrict__ col, const int16_t* __restrict__ col16,
+                                                       const double* __restrict__ val, const double* __restrict__ X,
+                                                       Row64* __restrict__ cA, Row64* __restrict__ xa0,
+                                                       Row64* __restrict__ xa1, double* __restrict__ out,
+                                                       FirstTermArgs ft) {
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int slice = gt / kWave, lane = gt % kWave;
   if (slice >= nSlices) return;  // whole wavefront
@@ -430,13 +432,13 @@ void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precis
 //   u_out = a M u_in + b u_oth,   uc[row] += a u_in[row]   (uc: D^{-1}-scaled first-term adjoint);
 // the M^2-entry adjoints are gathered once per solve by k_adj_gbar.
 template <int G, bool C16>
-__global__ void __launch_bounds__(kBlock) k_adj_spmv(int n, int nSlices, int ld, double a, double b,
-                                                     const int64_t* __restrict__ sliceOff,
-                                                     const int* __restrict__ sliceW, int uw,
-                                                     const int* __restrict__ col, const int16_t* __restrict__ col16,
-                                                     const double* __restrict__ val, const double* __restrict__ uin,
-                                                     const double* __restrict__ uoth, double* __restrict__ uout,
-                                                     double* __restrict__ uc) {
+__global__ void __launch_bounds__(kBlock, 8) k_adj_spmv(int n, int nSlices, int ld, double a, double b,
+                                                        const int64_t* __restrict__ sliceOff,
+                                                        const int* __restrict__ sliceW, int uw,
+                                                        const int* __restrict__ col, const int16_t* __restrict__ col16,
+                                                        const double* __restrict__ val, const double* __restrict__ uin,
+                                                        const double* __restrict__ uoth, double* __restrict__ uout,
+                                                        double* __restrict__ uc) {
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int slice = gt / kWave, lane = gt % kWave;
   if (slice >= nSlices) return;
