@@ -61,6 +61,8 @@ struct DeviceMesh {
   // M^2 in SELL layout
   int sellLanes = 4, rowsPerSlice = 16, nSlices = 0;
   int sellUniformW = 0;  // > 0: every slice has this width (offsets by arithmetic)
+  int* sellPerm = nullptr;    // SELL slot -> row (nSlices * rowsPerSlice); null: identity
+  int* sellSlotOf = nullptr;  // row -> SELL slot (nSim); null: identity
   int64_t sellEntries = 0;
   int64_t* sliceOff = nullptr;
   int* sliceW = nullptr;

@@ -797,14 +797,17 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
   L.rowsPerSlice = 64 / lanes;
   int n = hs.nSim;
   L.nSlices = (n + L.rowsPerSlice - 1) / L.rowsPerSlice;
-  for (int s = 0; s < L.nSlices; s++) {
-    int w = 0;
-    for (int r = s * L.rowsPerSlice; r < std::min(n, (s + 1) * L.rowsPerSlice); r++) {
-      int nnz = hs.m2Off[r + 1] - hs.m2Off[r];
-      w = std::max(w, (nnz + lanes - 1) / lanes);
+  const int S = L.rowsPerSlice, nSlots = L.nSlices * S;
+  std::vector<int> rowW(n);
+  for (int r = 0; r < n; r++) rowW[r] = (hs.m2Off[r + 1] - hs.m2Off[r] + lanes - 1) / lanes;
+  auto slice_widths = [&](auto rowOfSlot) {
+    L.sliceW.assign(L.nSlices, 0);
+    for (int p = 0; p < nSlots; p++) {
+      const int r = rowOfSlot(p);
+      if (r < n) L.sliceW[p / S] = std::max(L.sliceW[p / S], rowW[r]);
     }
-    L.sliceW.push_back(w);
-  }
+  };
+  slice_widths([](int p) { return p; });
   // When padding every slice to the widest costs at most 2% more slots (a
   // regular mesh: only boundary slices are narrower), do so: the sweeps then
   // find a slice's entries by arithmetic instead of a dependent table load.
@@ -816,7 +819,24 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
   if (L.nSlices > 0 && (wmax * L.nSlices - slots) * 50 <= slots) {
     L.uniformW = (int)wmax;
     std::fill(L.sliceW.begin(), L.sliceW.end(), L.uniformW);
+  } else if (L.nSlices > 0) {
+    // Otherwise order the rows by width (widest first, stable) within windows
+    // of kSellSortSlices slices, so that each slice holds rows of similar
+    // width (a tet mesh of 5-tet cells alternates two vertex valences from one
+    // row to the next: two slices hold one kind each) while the rows of a
+    // slice stay near each other; every row keeps its entries and their order,
+    // so its sums are unchanged.
+    const int win = kSellSortSlices * S;
+    L.perm.resize(nSlots);
+    for (int p = 0; p < nSlots; p++) L.perm[p] = p;
+    for (int w0 = 0; w0 < n; w0 += win)
+      std::stable_sort(L.perm.begin() + w0, L.perm.begin() + std::min(n, w0 + win),
+                       [&](int a, int b) { return rowW[a] > rowW[b]; });
+    L.slotOf.assign(n, 0);
+    for (int p = 0; p < n; p++) L.slotOf[L.perm[p]] = p;
+    slice_widths([&](int p) { return L.perm[p]; });
   }
+  auto rowOf = [&](int p) { return L.perm.empty() ? p : L.perm[p]; };
   L.sliceOff.assign(1, 0);
   for (int s = 0; s < L.nSlices; s++) L.sliceOff.push_back(L.sliceOff.back() + (int64_t)L.sliceW[s] * 64);
   int64_t total = L.sliceOff.back();
@@ -829,7 +849,7 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
   parallel_ranges(L.nSlices, nt, [&](int64_t s0, int64_t s1, int) {
     for (int s = (int)s0; s < (int)s1; s++)
       for (int lr = 0; lr < L.rowsPerSlice; lr++) {
-        int r = s * L.rowsPerSlice + lr;
+        int r = rowOf(s * L.rowsPerSlice + lr);
         int rr = r < n ? r : n - 1;  // rows past the end pad with a valid index
         for (int c = 0; c < L.sliceW[s] * lanes; c++) {
           int64_t e = L.sliceOff[s] + (int64_t)(c / lanes) * 64 + lr * lanes + (c % lanes);
@@ -850,7 +870,7 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
   L.col16ok = true;
   for (int s = 0; s < L.nSlices && L.col16ok; s++)
     for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++) {
-      int r = s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes;
+      int r = rowOf(s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes);
       int d = L.col[e] - r;
       if (d < -32768 || d > 32767) {
         L.col16ok = false;
@@ -862,7 +882,7 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
     parallel_ranges(L.nSlices, nt, [&](int64_t s0, int64_t s1, int) {
       for (int s = (int)s0; s < (int)s1; s++)
         for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++)
-          L.col16[e] = (int16_t)(L.col[e] - (s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes));
+          L.col16[e] = (int16_t)(L.col[e] - rowOf(s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes));
     });
   }
   L.pOff.assign(total + 1, 0);

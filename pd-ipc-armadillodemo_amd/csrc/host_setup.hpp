@@ -96,6 +96,10 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
 struct SellLayout {
   int lanes = 4, rowsPerSlice = 16, nSlices = 0;
   int uniformW = 0;               // > 0: every slice padded to this width
+  // non-uniform layouts: slot -> row (rows sorted by width within windows of
+  // kSellSortSlices slices; slots past the last row map to themselves) and its
+  // inverse over the rows; both empty for the identity order
+  std::vector<int> perm, slotOf;
   std::vector<int64_t> sliceOff;   // nSlices + 1
   std::vector<int> sliceW;         // nSlices: width in 64-entry columns
   std::vector<int> col;            // per SELL entry
@@ -111,6 +115,7 @@ struct SellLayout {
   std::vector<int> r1Idx;          // per SELL entry: R1 index of (i,j), -1 if none or padding
   std::vector<int> pIa, pIb;       // per 2-path: R1 indices of its two factors
 };
+constexpr int kSellSortSlices = 2;  // slices per row-sorting window
 void build_sell(const HostSetup& hs, int lanes, SellLayout& out);
 
 }  // namespace pdipc

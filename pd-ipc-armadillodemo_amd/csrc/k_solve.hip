@@ -85,7 +85,7 @@ template <int G, bool C16>
 // (at most 64 registers: eight wavefronts per SIMD, so a 65 k-row sweep runs in one round)
 __global__ void __launch_bounds__(kBlock, 8) k_first64(int n, int nSlices, int ld,
                                                        const int64_t* __restrict__ sliceOff,
-                                                       const int* __restrict__ sliceW, int uw,
+                                                       const int* __restrict__ sliceW, int uw, const int* __restrict__ perm,
                                                        const int* __restrict__ col, const int16_t* __restrict__ col16,
                                                        const double* __restrict__ val, const double* __restrict__ X,
                                                        Row64* __restrict__ cA, Row64* __restrict__ xa0,
@@ -95,7 +95,8 @@ __global__ void __launch_bounds__(kBlock, 8) k_first64(int n, int nSlices, int l
   int slice = gt / kWave, lane = gt % kWave;
   if (slice >= nSlices) return;  // whole wavefront
   constexpr int S = kWave / G;
-  int row = slice * S + lane / G;
+  const int slot = slice * S + lane / G;
+  const int row = perm ? perm[slot] : slot;
   int sub = lane % G;
   const int width = uw > 0 ? uw : sliceW[slice];
   const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
@@ -215,7 +216,7 @@ __global__ void __launch_bounds__(kBlock) k_res_b(int n, double w, const int* __
 template <int G, bool C16>
 __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, double W,
                                                     const int64_t* __restrict__ sliceOff,
-                                                    const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
+                                                    const int* __restrict__ sliceW, int uw, const int* __restrict__ perm, const int* __restrict__ col,
                                                     const int16_t* __restrict__ col16, const double* __restrict__ val,
                                                     const Row64* __restrict__ cur, const Row64* __restrict__ pre,
                                                     const Row64* __restrict__ cA,
@@ -225,7 +226,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
   int slice = gt / kWave, lane = gt % kWave;
   if (slice >= nSlices) return;
   constexpr int S = kWave / G;
-  int row = slice * S + lane / G;  // < ld: slices cover at most ld rows
+  const int slot = slice * S + lane / G;  // < ld: slices cover at most ld rows
+  const int row = perm ? perm[slot] : slot;
   const int width = uw > 0 ? uw : sliceW[slice];
   const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
   Row64 p = pre[row], c = cA[row];  // loaded up front, with the matrix
@@ -282,7 +284,7 @@ __global__ void k_row64_to_soa(int n, int ld, const Row64* __restrict__ a, doubl
 template <int G, int LAST, bool C16>
 __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, float W,
                                                     const int64_t* __restrict__ sliceOff,
-                                                    const int* __restrict__ sliceW, int uw, const int* __restrict__ col,
+                                                    const int* __restrict__ sliceW, int uw, const int* __restrict__ perm, const int* __restrict__ col,
                                                     const int16_t* __restrict__ col16, const float* __restrict__ val,
                                                     const float4* __restrict__ dcur, const float4* __restrict__ dpre,
                                                     const float4* __restrict__ r0, float4* __restrict__ dnext,
@@ -292,7 +294,8 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
   int slice = gt / kWave, lane = gt % kWave;
   if (slice >= nSlices) return;
   constexpr int S = kWave / G;
-  int row = slice * S + lane / G;  // < ld: slices cover at most ld rows
+  const int slot = slice * S + lane / G;  // < ld: slices cover at most ld rows
+  const int row = perm ? perm[slot] : slot;
   const int width = uw > 0 ? uw : sliceW[slice];
   const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
   // the row's own operands, loaded up front so they arrive with the matrix
@@ -354,10 +357,10 @@ void launch_sweep32(const DeviceMesh& m, int grid, float W, const float4* cur, c
                     hipStream_t s) {
   if (m.m2Col16)
     hipLaunchKernelGGL((k_sweep32<G, LAST, true>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W,
-                       m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
+                       m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
   else
     hipLaunchKernelGGL((k_sweep32<G, LAST, false>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W,
-                       m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
+                       m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
 }
 
 // The FP64 solve: x_0 = X, iterates rotate through xa0..xa2, the last one is
@@ -372,7 +375,7 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   double W = 2.0 / (2.0 - r);
   const bool c16 = m.m2Col16 != nullptr;
   hipLaunchKernelGGL((c16 ? k_first64<G, true> : k_first64<G, false>), dim3(grid), dim3(kBlock), 0, s, m.nSim,
-                     m.nSlices, m.ld, m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16, m.m2Val, m.X, m.cA,
+                     m.nSlices, m.ld, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2Val, m.X, m.cA,
                      m.xa0, m.xa1, outs ? outs[1] : nullptr, ft);
   Row64 *pre = m.xa0, *cur = m.xa1, *next = m.xa2;
   if (iters <= 1) {
@@ -383,7 +386,7 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   for (int it = 1; it < iters; it++) {
     double* out = outs ? outs[it + 1] : (it == iters - 1 ? m.X : nullptr);
     hipLaunchKernelGGL((c16 ? k_sweep64<G, true> : k_sweep64<G, false>), dim3(grid), dim3(kBlock), 0, s, m.nSim,
-                       m.nSlices, m.ld, W, m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16, m.m2Val, cur,
+                       m.nSlices, m.ld, W, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2Val, cur,
                        pre, m.cA, next, out);
     W = 4.0 / (4.0 - r * W);
     Row64* t = pre; pre = cur; cur = next; next = t;
@@ -434,7 +437,7 @@ void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precis
 template <int G, bool C16>
 __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv(int n, int nSlices, int ld, double a, double b,
                                                         const int64_t* __restrict__ sliceOff,
-                                                        const int* __restrict__ sliceW, int uw,
+                                                        const int* __restrict__ sliceW, int uw, const int* __restrict__ perm,
                                                         const int* __restrict__ col, const int16_t* __restrict__ col16,
                                                         const double* __restrict__ val, const double* __restrict__ uin,
                                                         const double* __restrict__ uoth, double* __restrict__ uout,
@@ -443,7 +446,8 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv(int n, int nSlices, int 
   int slice = gt / kWave, lane = gt % kWave;
   if (slice >= nSlices) return;
   constexpr int S = kWave / G;
-  int row = slice * S + lane / G;
+  const int slot = slice * S + lane / G;
+  const int row = perm ? perm[slot] : slot;
   int sub = lane % G;
   const int width = uw > 0 ? uw : sliceW[slice];
   const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
@@ -473,9 +477,10 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv(int n, int nSlices, int 
 // lane per column of the union of the slice's columns (sorted, listed per slice
 // at set-up): the lane loads its column's x_k kGbarChunk sweeps at a time (all
 // loads of a chunk in flight together) and sums over k, in order, for all R
-// rows of the slice at once; the rows' u values are wave-uniform (R consecutive
-// doubles per component, scalar loads; rows past n read the vectors' ld
-// padding, and their sums are never stored).  So each x value is read once per
+// rows of the slice at once; the rows' u values are wave-uniform (scalar
+// loads: R consecutive doubles per component, or with P the rows that the
+// slice's slots hold under the SELL row order perm; rows past n read the
+// vectors' ld padding, and their sums are never stored).  So each x value is read once per
 // slice, where a lane per entry reads it for every entry of the column.  uPos
 // gives, per union column and row, the entry's offset in the slice's SELL block
 // (0xFFFF: no entry; padding slots are never written).
@@ -486,8 +491,9 @@ struct GbarArgs {
 };
 constexpr int kGbarThreads = 128;
 constexpr int kGbarChunk = 8;
-template <int G>
+template <int G, bool P>
 __global__ void __launch_bounds__(kGbarThreads) k_adj_gbar(int ld, const int64_t* __restrict__ sliceOff,
+                                                           const int* __restrict__ perm,
                                                            const int* __restrict__ uOff, const int* __restrict__ uCol,
                                                            const uint16_t* __restrict__ uPos,
                                                            const double* __restrict__ x0, const double* __restrict__ x1,
@@ -496,6 +502,9 @@ __global__ void __launch_bounds__(kGbarThreads) k_adj_gbar(int ld, const int64_t
   const int slice = blockIdx.x;
   const int K = ga.K;
   const int row0 = slice * R;
+  int rows[R];  // the slice's rows, relative to row0 without P
+#pragma unroll
+  for (int i = 0; i < R; i++) rows[i] = P ? perm[row0 + i] : i;
   const int u0 = uOff[slice], nU = uOff[slice + 1] - u0;
   const int64_t sb = sliceOff[slice];
   for (int t = threadIdx.x; t < nU; t += kGbarThreads) {
@@ -519,10 +528,10 @@ __global__ void __launch_bounds__(kGbarThreads) k_adj_gbar(int ld, const int64_t
         if (k0 + q < K) {
           const int k = k0 + q;
           const double w = ga.w[k];
-          const double* uk = ga.u[k] + row0;
+          const double* uk = ga.u[k] + (P ? 0 : row0);
 #pragma unroll
           for (int i = 0; i < R; i++)
-            g[i] += w * (uk[i] * xv[q][0] + uk[ld + i] * xv[q][1] + uk[2 * ld + i] * xv[q][2]);
+            g[i] += w * (uk[rows[i]] * xv[q][0] + uk[ld + rows[i]] * xv[q][1] + uk[2 * ld + rows[i]] * xv[q][2]);
         }
     }
     const uint16_t* pos = uPos + (int64_t)(u0 + t) * R;
@@ -541,7 +550,7 @@ void adj_pass_impl(const DeviceMesh& m, const AdjMesh& a, const double* W, int K
   for (int k = K - 1; k >= 0; k--) {
     double b = k + 1 <= K - 1 ? 1.0 - W[k + 1] : 0.0;
     hipLaunchKernelGGL((m.m2Col16 ? k_adj_spmv<G, true> : k_adj_spmv<G, false>), dim3(grid), dim3(kBlock), 0, s,
-                       m.nSim, m.nSlices, m.ld, W[k], b, m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16,
+                       m.nSim, m.nSlices, m.ld, W[k], b, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16,
                        m.m2Val, u[k + 1], u[k + 2], u[k], uc);
   }
   GbarArgs ga;
@@ -550,8 +559,8 @@ void adj_pass_impl(const DeviceMesh& m, const AdjMesh& a, const double* W, int K
     ga.u[k] = u[k + 1];
     ga.w[k] = W[k];
   }
-  hipLaunchKernelGGL((k_adj_gbar<G>), dim3(m.nSlices), dim3(kGbarThreads), 0, s, m.ld, m.sliceOff, a.uOff, a.uCol,
-                     a.uPos, x0, x1, xStride, ga, Gbar);
+  hipLaunchKernelGGL((m.sellPerm ? k_adj_gbar<G, true> : k_adj_gbar<G, false>), dim3(m.nSlices), dim3(kGbarThreads),
+                     0, s, m.ld, m.sliceOff, m.sellPerm, a.uOff, a.uCol, a.uPos, x0, x1, xStride, ga, Gbar);
 }
 
 // Adjoint of the first term c_i = (w^2 sum_j R1_ij b_j / D_j + (2 - w) w b_i) / D_i
@@ -613,9 +622,11 @@ __device__ inline double wave_sum(double v) {
 }
 
 // one wavefront per row; lane t takes the row's entries t, t + 64, ...
+// (slotOf: the row's SELL slot when rows are reordered)
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_adj_ops_rows(int n, const int64_t* __restrict__ sliceOff,
-                                                         const int* __restrict__ sliceW, const double* __restrict__ r1e,
+                                                         const int* __restrict__ sliceW,
+                                                         const int* __restrict__ slotOf, const double* __restrict__ r1e,
                                                          const int* __restrict__ r1Idx,
                                                          const int64_t* __restrict__ pOff, const int* __restrict__ pK,
                                                          const double* __restrict__ pA, const double* __restrict__ D,
@@ -625,7 +636,8 @@ __global__ void __launch_bounds__(kBlock) k_adj_ops_rows(int n, const int64_t* _
   int lane = threadIdx.x % kWave;
   if (row >= n) return;  // whole wavefront
   constexpr int S = kWave / G;
-  int slice = row / S, lr = row % S;
+  const int slot = slotOf ? slotOf[row] : row;
+  int slice = slot / S, lr = slot % S;
   int64_t base = sliceOff[slice] + lr * G;
   int cnt = sliceW[slice] * G;
   const double w2 = w * w, c1 = 2.0 * (1.0 - w) * w;
@@ -681,7 +693,7 @@ void one_sweep_impl(const DeviceMesh& m, int precision, hipStream_t s) {
   const double W = 1.9;
   if (precision == 0)
     hipLaunchKernelGGL((m.m2Col16 ? k_sweep64<G, true> : k_sweep64<G, false>), dim3(grid), dim3(kBlock), 0, s,
-                       m.nSim, m.nSlices, m.ld, W, m.sliceOff, m.sliceW, m.sellUniformW, m.m2Col, m.m2Col16, m.m2Val,
+                       m.nSim, m.nSlices, m.ld, W, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2Val,
                        m.xa1, m.xa0, m.cA, m.xa2, nullptr);
   else
     launch_sweep32<G, 0>(m, grid, (float)W, m.dCur, m.dPre, m.dNext, s);
@@ -738,7 +750,7 @@ void launch_adj_ops_gather(const DeviceMesh& m, const AdjMesh& a, double omega, 
   const int rowsPerBlock = kBlock / kWave;
   const int gr = (m.nSim + rowsPerBlock - 1) / rowsPerBlock;
 #define ROWS(G)                                                                                                    \
-  hipLaunchKernelGGL(k_adj_ops_rows<G>, dim3(gr), dim3(kBlock), 0, s, m.nSim, m.sliceOff, m.sliceW, m.m2R1,        \
+  hipLaunchKernelGGL(k_adj_ops_rows<G>, dim3(gr), dim3(kBlock), 0, s, m.nSim, m.sliceOff, m.sliceW, m.sellSlotOf, m.m2R1,        \
                      a.sellR1Idx, m.p2Off, m.p2K, m.p2A, m.sysDiag, Gbar, omega, Dbar, R1bar)
   switch (m.sellLanes) {
     case 1: ROWS(1); break;

@@ -127,6 +127,10 @@ int upload_all(pdipc_handle* h) {
   m.sellEntries = L.sliceOff.back();
   UP(sliceOff, L.sliceOff); UP(sliceW, L.sliceW); UP(m2Col, L.col); UP(m2RowFlag, L.rowFlag); UP(m2R1, L.r1);
   if (L.col16ok) UP(m2Col16, L.col16);
+  if (!L.perm.empty()) {
+    UP(sellPerm, L.perm);
+    UP(sellSlotOf, L.slotOf);
+  }
   UP(p2Off, L.pOff); UP(p2K, L.pK); UP(p2A, L.pA);
   AL(m2Val, m.sellEntries); AL(m2ValF, m.sellEntries);
   // collision

@@ -1,7 +1,9 @@
 """The SELL matrix layout's variants against the oracle: every supported lane
-count per row (1, 2, 4; 8 is the default every other test runs), and 32-bit
+count per row (1, 2, 4; 8 is the default every other test runs), 32-bit
 columns (a cloth whose vertex ids are randomly permuted, so that the offsets
-col - row of M^2 leave the 16-bit range the FP32 sweep otherwise stores).
+col - row of M^2 leave the 16-bit range the FP32 sweep otherwise stores), and
+the reordered rows of a layout whose rows differ in width (a tet block: its
+rows are sorted by width within windows, so slot and row differ).
 
 Bars as in the parity tests: FP64 within 1e-8 relative per step (teacher
 forced), FP32-split within 1e-5 of the FP64 oracle per step."""
@@ -70,6 +72,22 @@ def test_32bit_columns_scrambled_cloth():
         ref = oracle.Oracle(sc)
         ref.step(2)
         worst = _teacher_forced(sc, ref, 2, precision=prec)
+        print("worst step error", worst, flush=True)
+        if prec == pdipc.FP32_SPLIT:
+            assert worst < 1e-5
+
+
+@pytest.mark.parametrize("lanes", [8, 4])
+def test_reordered_rows_tet_block(lanes):
+    sc = scenes.config2(cells=6, cloth_n=16)  # 343 tet + 256 cloth vertices; tet rows 3 to 12 slots wide at 8 lanes
+    start = oracle.Oracle(sc)
+    start.step(4)
+    for prec in (pdipc.FP64, pdipc.FP32_SPLIT):
+        print(f"case: tet block, reordered SELL rows, {lanes} lanes, precision {prec}", flush=True)
+        ref = oracle.Oracle(sc)
+        ref.set_x(start.x())
+        ref.set_v(start.v())
+        worst = _teacher_forced(sc, ref, 3, sell_lanes=lanes, precision=prec)
         print("worst step error", worst, flush=True)
         if prec == pdipc.FP32_SPLIT:
             assert worst < 1e-5
