@@ -225,7 +225,11 @@ struct ResidualArgs {
 
 // kRhsLanes lanes per point: ring slots, tet corners and (RES) R1 entries are
 // dealt round-robin and the partial sums combined by a fixed shuffle tree.
+// A lane fetches kRhsBatch of its tet corners (R1 entries) at a time, all
+// loads first, then adds them in order, so that several memory round trips
+// are in flight per lane; the sums are those of the one-at-a-time loop.
 constexpr int kRhsLanes = 4;
+constexpr int kRhsBatch = 4;
 
 template <class V>
 __device__ inline V rhs_lanes_sum(V v) {
@@ -247,11 +251,21 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, ResidualArgs ra, co
   bool live = i < nSim;
   double tr[3] = {0, 0, 0};  // tet outputs
   int q0 = live ? vtOff[i] : 0, q1 = live ? vtOff[i + 1] : 0;
-  for (int q = q0 + sub; q < q1; q += kRhsLanes) {
-    int tc = vtList[q];
-    int t = tc >> 2, corner = tc & 3;
+  for (int q = q0 + sub; q < q1; q += kRhsBatch * kRhsLanes) {
+    int tc[kRhsBatch];
 #pragma unroll
-    for (int k = 0; k < 3; k++) tr[k] += tetOut[(int64_t)(corner * 3 + k) * nT + t];
+    for (int b = 0; b < kRhsBatch; b++) tc[b] = q + b * kRhsLanes < q1 ? vtList[q + b * kRhsLanes] : -1;
+    double a[kRhsBatch][3];
+#pragma unroll
+    for (int b = 0; b < kRhsBatch; b++)
+#pragma unroll
+      for (int k = 0; k < 3; k++)
+        a[b][k] = tc[b] >= 0 ? tetOut[(int64_t)((tc[b] & 3) * 3 + k) * nT + (tc[b] >> 2)] : 0.0;
+#pragma unroll
+    for (int b = 0; b < kRhsBatch; b++)
+      if (tc[b] >= 0)
+#pragma unroll
+        for (int k = 0; k < 3; k++) tr[k] += a[b][k];
   }
   double xi[3] = {0, 0, 0};
   if (live) {
@@ -282,14 +296,28 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, ResidualArgs ra, co
     }
   }
   double sx[3] = {0, 0, 0};
-  if (RES && live)
-    for (int q = ra.off[i] + sub; q < ra.off[i + 1]; q += kRhsLanes) {
-      int j = ra.col[q];
-      double a = ra.val[q];
-      sx[0] += a * X[j];
-      sx[1] += a * X[ld + j];
-      sx[2] += a * X[2 * ld + j];
+  if (RES && live) {
+    const int r1 = ra.off[i + 1];
+    for (int q = ra.off[i] + sub; q < r1; q += kRhsBatch * kRhsLanes) {
+      int j[kRhsBatch];
+      double a[kRhsBatch], xv[kRhsBatch][3];
+#pragma unroll
+      for (int b = 0; b < kRhsBatch; b++) {
+        const bool in = q + b * kRhsLanes < r1;
+        j[b] = in ? ra.col[q + b * kRhsLanes] : i;
+        a[b] = in ? ra.val[q + b * kRhsLanes] : 0.0;
+      }
+#pragma unroll
+      for (int b = 0; b < kRhsBatch; b++)
+#pragma unroll
+        for (int k = 0; k < 3; k++) xv[b][k] = X[k * ld + j[b]];
+#pragma unroll
+      for (int b = 0; b < kRhsBatch; b++)
+        if (q + b * kRhsLanes < r1)
+#pragma unroll
+          for (int k = 0; k < 3; k++) sx[k] += a[b] * xv[b][k];
     }
+  }
 #pragma unroll
   for (int k = 0; k < 3; k++) {
     tr[k] = rhs_lanes_sum(tr[k]);
