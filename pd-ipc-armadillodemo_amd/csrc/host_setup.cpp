@@ -866,12 +866,14 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
         }
       }
   });
-  // 16-bit row-relative columns when every offset fits
+  // 16-bit columns relative to the entry's slot (its row, unless rows are
+  // reordered: then the sweeps' gathers need not wait for the row index)
+  // when every offset fits
   L.col16ok = true;
   for (int s = 0; s < L.nSlices && L.col16ok; s++)
     for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++) {
-      int r = rowOf(s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes);
-      int d = L.col[e] - r;
+      int slot = s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes;
+      int d = L.col[e] - slot;
       if (d < -32768 || d > 32767) {
         L.col16ok = false;
         break;
@@ -882,7 +884,7 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
     parallel_ranges(L.nSlices, nt, [&](int64_t s0, int64_t s1, int) {
       for (int s = (int)s0; s < (int)s1; s++)
         for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++)
-          L.col16[e] = (int16_t)(L.col[e] - rowOf(s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes));
+          L.col16[e] = (int16_t)(L.col[e] - (s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes));
     });
   }
   L.pOff.assign(total + 1, 0);

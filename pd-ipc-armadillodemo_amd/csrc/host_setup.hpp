@@ -103,7 +103,8 @@ struct SellLayout {
   std::vector<int64_t> sliceOff;   // nSlices + 1
   std::vector<int> sliceW;         // nSlices: width in 64-entry columns
   std::vector<int> col;            // per SELL entry
-  // per SELL entry: col - row of the entry's slot (padding rows included);
+  // per SELL entry: col - the entry's slot index (its row without perm; padding
+  // rows included);
   // filled only when every offset fits in 16 bits (col16ok)
   std::vector<int16_t> col16;
   bool col16ok = false;

@@ -13,8 +13,10 @@
 // matrix row is read once for all three coordinates (the reference re-reads it
 // per coordinate).  When all slices have one width uw (set-up pads them when
 // that costs at most 2% more slots) a sweep computes its slice's offset
-// instead of loading it; columns are 16-bit row-relative offsets whenever every
-// entry is within 32767 rows of its own.  The operator update rebuilds the M^2
+// instead of loading it; otherwise rows are sorted by width within windows of
+// two slices and a slot's row is read from perm.  Columns are 16-bit offsets
+// from the entry's slot (its row, unless reordered) whenever every one fits, so
+// the gathers never wait for perm.  The operator update rebuilds the M^2
 // values from the 2-path lists once per outer iteration.
 //
 // FP32-split: the sweeps run in FP32 on the correction delta = x - x0, driven
@@ -104,7 +106,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_first64(int n, int nSlices, int l
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
     int64_t e = off + (int64_t)c * kWave;
-    int j = C16 ? row + (int)col16[e] : col[e];
+    int j = C16 ? slot + (int)col16[e] : col[e];
     double v = val[e];
     acc[0] += v * X[j];
     acc[1] += v * X[ld + j];
@@ -241,7 +243,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
       v[u] = 0.0;
       if (c0 + u < width) {
         int64_t e = off + (int64_t)(c0 + u) * kWave;
-        j[u] = C16 ? row + (int)col16[e] : col[e];
+        j[u] = C16 ? slot + (int)col16[e] : col[e];
         v[u] = val[e];
       }
     }
@@ -280,7 +282,7 @@ __global__ void k_row64_to_soa(int n, int ld, const Row64* __restrict__ a, doubl
 // one 16-byte gather; the entries of a lane are fetched U at a time (all
 // column / value loads first, then all gathers) to keep several memory
 // round trips in flight per wavefront.  C16: columns as 16-bit offsets from
-// the row.
+// the slot.
 template <int G, int LAST, bool C16>
 __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, float W,
                                                     const int64_t* __restrict__ sliceOff,
@@ -317,7 +319,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
       v[u] = 0.0f;
       if (c0 + u < width) {
         int64_t e = off + (int64_t)(c0 + u) * kWave;
-        j[u] = C16 ? row + (int)col16[e] : col[e];
+        j[u] = C16 ? slot + (int)col16[e] : col[e];
         v[u] = val[e];
       }
     }
@@ -455,7 +457,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv(int n, int nSlices, int 
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
     int64_t e = off + (int64_t)c * kWave;
-    int j = C16 ? row + (int)col16[e] : col[e];
+    int j = C16 ? slot + (int)col16[e] : col[e];
     double v = val[e];
     acc[0] += v * uin[j];
     acc[1] += v * uin[ld + j];
