@@ -33,6 +33,9 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
 # a plain 1 GiB copy gives): profiles/k_sweep_config1_hbm_traffic.md.
 SWEEP32_TRAFFIC_BYTES = {(256, 6, 8): 29.86e6}
+# steps of each scene's run from rest; by default the warm-up steps and the
+# timed steps together make up the whole run
+RUN_STEPS = {"cloth": 200, "config0": 100, "tet_block": 100, "stacked": 50}
 
 
 def log(msg):
@@ -129,23 +132,24 @@ def cpu_baseline(scene, x, v, seconds_budget=20.0, label="256x256 cloth"):
         ref.set_x(x)
         ref.set_v(v)
         init_s = time.perf_counter() - t0
-        n, t = 0, 0.0
+        n, t, outer = 0, 0.0, 0
         while t < budget and n < 50:
             t1 = time.perf_counter()
             ref.step(1)
             t += time.perf_counter() - t1
             n += 1
-        return n, t, init_s
+            outer += int(ref.counters()["outer"])
+        return n, t, init_s, outer
 
     threads = _host_threads()
-    n, t, init_s = sample(threads, 0.5 * seconds_budget)
-    n1, t1, _ = sample(1, seconds_budget)
+    n, t, init_s, outer = sample(threads, 0.5 * seconds_budget)
+    n1, t1, _, _ = sample(1, seconds_budget)
     oracle.set_threads(0)
     return dict(value=n / t, unit="steps/s", cores=threads, kind="port",
-                single_thread_steps_per_s=n1 / t1,
+                single_thread_steps_per_s=n1 / t1, outer_iters_per_step=outer / n,
                 sample=f"FP64 oracle steps of the same scene ({label}) from the GPU timing's warmed-up state: "
-                       f"{n} steps on {threads} threads ({t:.1f} s), {n1} steps on 1 thread ({t1:.1f} s); "
-                       f"setup {init_s:.1f} s each")
+                       f"{n} steps on {threads} threads ({t:.1f} s, {outer / n:.1f} outer iterations per step), "
+                       f"{n1} steps on 1 thread ({t1:.1f} s); setup {init_s:.1f} s each")
 
 
 def adjoint_bench(graphs, fd_dirs, log_every=8):
@@ -226,7 +230,9 @@ def adjoint_bench(graphs, fd_dirs, log_every=8):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=60)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (default: the rest of the scene's run after the warm-up: 200 steps in all "
+                         "for the 256^2 cloth, 100 for config0 and tet_block, 50 for stacked)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--scene", choices=("cloth", "config0", "tet_block", "stacked"), default="cloth",
                     help="cloth: the n x n cloth over the sphere (default); config0: the 64x64 cloth over the "
@@ -247,6 +253,9 @@ def main():
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--only-adjoint", action="store_true", help="run only the differentiable-run measurement")
     args = ap.parse_args()
+    run_steps = RUN_STEPS[args.scene]
+    if args.steps is None:
+        args.steps = run_steps if args.scene == "stacked" else max(1, run_steps - args.warmup)
     if args.copy_probe:
         import torch
         a = torch.ones(1 << 28, dtype=torch.float32, device="cuda")
@@ -359,6 +368,8 @@ def main():
             "workload": workload or f"cloth {args.n}x{args.n} ({c['cloth_points']} vertices) over icosphere subdiv "
                                     f"{args.subdiv} + ground, dt=1/100, 10 LG x 15 Chebyshev, dHat=1e-3, kappa=1e3",
             "precision": "FP32-split (FP32 local step + sweeps on an FP64 residual; FP64 CCD/barrier)",
+            "timed_steps": f"steps {args.warmup + 1}-{args.warmup + steps} of a run from rest "
+                           f"({RUN_STEPS[args.scene]} steps configured), each precision from the same state",
             "fp64_steps_per_s": steps / res["fp64"]["seconds"],
             "fp32_split_speedup_vs_fp64": res["fp64"]["seconds"] / t32,
             "fp32_split_step_error_vs_fp64": max(errs),
