@@ -164,6 +164,20 @@ int pdipc_count_intersections(pdipc_handle* h, long long* count);
 /* LBVH of the last broad phase: sorted 64-bit keys (P), internal nodes
  * (left, right child as node ids: leaves are P-1+leaf) and parents (2P-1) */
 int pdipc_debug_lbvh(pdipc_handle* h, unsigned long long* keys, int* left, int* right, int* parent, int* npatches);
+/* Validation, no counterpart in the reference (the reference keeps R2 as
+ * per-row lists, PdIpcGpuFunc.cu:976-1059): the SELL layout of the M^2 rows
+ * that the device upload builds, for `lanes` lanes per row (1, 2, 4 or 8),
+ * from the host precompute alone (after pdipc_host_setup; no device needed).
+ * sizes[0..5] receive the slice count, the entry count (padding included), the
+ * common slice width (0: per-slice widths), 1 if 16-bit columns are used, the
+ * slot count of the row order (0: slot = row) and the rows per slice.  The
+ * arrays may be null (sizes only); otherwise they hold slice count + 1 offsets,
+ * slice count widths (in 64-entry columns), entry count columns and row flags
+ * (row | 0x80000000 on the diagonal; padding entries repeat their row's index
+ * unflagged), entry count 16-bit columns (col - slot; only if used) and the
+ * slot -> row order (only if rows are reordered). */
+int pdipc_debug_sell_layout(pdipc_handle* h, int lanes, long long* sizes, long long* slice_off, int* slice_w,
+                            int* col, int* row_flag, short* col16, int* perm);
 
 /* --- measurement support (no counterpart in the reference): replays one
  * kernel of the step `reps` times on the handle's stream, bracketed by HIP

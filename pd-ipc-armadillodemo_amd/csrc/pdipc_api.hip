@@ -765,6 +765,30 @@ int pdipc_get_model_dat(pdipc_handle* h, int model, int which, int* out, int cap
   return PDIPC_OK;
 }
 
+int pdipc_debug_sell_layout(pdipc_handle* h, int lanes, long long* sizes, long long* slice_off, int* slice_w,
+                            int* col, int* row_flag, short* col16, int* perm) {
+  if (!h) return fail(PDIPC_E_ARG, "null handle");
+  if (!h->hostReady) return fail(PDIPC_E_ARG, "pdipc_host_setup has not run");
+  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) return fail(PDIPC_E_ARG, "lanes must be 1, 2, 4 or 8");
+  if (!sizes) return fail(PDIPC_E_ARG, "null sizes");
+  SellLayout L;
+  build_sell(h->hs, lanes, L);
+  const int64_t nE = L.sliceOff.back();
+  sizes[0] = L.nSlices;
+  sizes[1] = nE;
+  sizes[2] = L.uniformW;
+  sizes[3] = L.col16ok ? 1 : 0;
+  sizes[4] = (long long)L.perm.size();
+  sizes[5] = L.rowsPerSlice;
+  if (slice_off) std::copy(L.sliceOff.begin(), L.sliceOff.end(), slice_off);
+  if (slice_w) std::copy(L.sliceW.begin(), L.sliceW.end(), slice_w);
+  if (col) std::copy(L.col.begin(), L.col.begin() + nE, col);
+  if (row_flag) std::copy(L.rowFlag.begin(), L.rowFlag.begin() + nE, row_flag);
+  if (col16 && L.col16ok) std::copy(L.col16.begin(), L.col16.begin() + nE, col16);
+  if (perm) std::copy(L.perm.begin(), L.perm.end(), perm);
+  return PDIPC_OK;
+}
+
 int pdipc_debug_broad_phase(pdipc_handle* h, const double* xold, const double* dx, int* count) {
   int rc = require_device(h);
   if (rc) return rc;

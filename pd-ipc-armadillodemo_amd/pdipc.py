@@ -79,6 +79,9 @@ SIGNATURES = {
     "pdipc_debug_local": (ctypes.c_int, [_H, _D, _D, _D]),
     "pdipc_debug_solve": (ctypes.c_int, [_H, _D, _D, _D, _D]),
     "pdipc_debug_lbvh": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_ulonglong), _I, _I, _I, _I]),
+    "pdipc_debug_sell_layout": (ctypes.c_int, [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
+                                               ctypes.POINTER(ctypes.c_longlong), _I, _I, _I,
+                                               ctypes.POINTER(ctypes.c_short), _I]),
     "pdipc_count_intersections": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_longlong)]),
     "pdipc_time_kernel": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _D, _D]),
     "pdipc_set_cloth_stiffness": (ctypes.c_int, [_H, ctypes.c_int, _D, ctypes.c_int]),
@@ -289,6 +292,26 @@ class Simulator:
         out = np.zeros(2 * max(n.value, 1), np.int32)
         _check(lib().pdipc_get_patch_pairs(self.h, _ip(out), n.value, ctypes.byref(n)))
         return out[: 2 * n.value].reshape(-1, 2)
+
+    def sell_layout(self, lanes=8):
+        """The SELL layout of M^2 the device upload builds for `lanes` lanes per
+        row, from the host precompute (after host_setup; no device needed)."""
+        sz = np.zeros(6, np.int64)
+        szp = sz.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+        _check(lib().pdipc_debug_sell_layout(self.h, lanes, szp, None, None, None, None, None, None))
+        ns, ne, uw, c16, nperm, rows = (int(v) for v in sz)
+        off = np.zeros(ns + 1, np.int64)
+        w = np.zeros(max(ns, 1), np.int32)
+        col = np.zeros(max(ne, 1), np.int32)
+        flag = np.zeros(max(ne, 1), np.int32)
+        col16 = np.zeros(max(ne, 1), np.int16)
+        perm = np.zeros(max(nperm, 1), np.int32)
+        _check(lib().pdipc_debug_sell_layout(self.h, lanes, szp, off.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)),
+                                             _ip(w), _ip(col), _ip(flag),
+                                             col16.ctypes.data_as(ctypes.POINTER(ctypes.c_short)), _ip(perm)))
+        return dict(slice_off=off, slice_w=w[:ns], col=col[:ne], row_flag=flag[:ne],
+                    col16=col16[:ne] if c16 else None, perm=perm[:nperm] if nperm else None,
+                    uniform_w=uw, rows_per_slice=rows)
 
     def model_dat(self, model, which):
         n = ctypes.c_int()
