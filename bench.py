@@ -5,9 +5,10 @@ noise-displaced icosphere (subdivision 6, 81,920 triangles) above a ground
 plane; dt = 1/100, 10 local/global passes x 15 Chebyshev sweeps per outer
 iteration, dHat = 1e-3, kappa = 1e3.  A "step" is one full PD-IPC time step
 (prediction, outer CCD iterations, velocity update) with all data resident in
-HBM.  The FP32-split step is the headline value; the plain FP64 step is timed
-from the same state, and the per-step error of FP32-split against FP64 is
-measured from identical states.
+HBM.  By default 20 warm-up steps and 180 timed steps: steps 21-200 of the
+200-step run from rest.  The FP32-split step is the headline value; the plain
+FP64 step is timed from the same state, and the per-step error of FP32-split
+against FP64 is measured from identical states.
 
 Prints one JSON line.  --gpus above 1 exits: this build runs on one GPU.
 """
