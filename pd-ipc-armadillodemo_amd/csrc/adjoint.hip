@@ -109,6 +109,9 @@ struct StepTape {
 struct AdjointState {
   AdjMesh am;
   std::vector<void*> allocs;
+  // 2-path term behind each midA entry: midA holds the SELL pA values, which
+  // scale with the spring stiffness, so a stiffness change re-gathers them
+  std::vector<int64_t> midQ;
   int nSteps = 0, every = 1;
   bool recorded = false;  // the rollout kept the tape of every step (every == 0)
   bool lastKept = false;  // seg holds the last checkpoint segment, recorded by the rollout
@@ -189,12 +192,14 @@ int ensure_state(pdipc_handle* h) {
     for (int k = 0; k < n; k++) midOff[k + 1] += midOff[k];
     std::vector<int64_t> midEnt(midOff[n]);
     std::vector<double> midA(midOff[n]);
+    A->midQ.assign(midOff[n], 0);
     std::vector<int> fill(midOff.begin(), midOff.end() - 1);
     for (int64_t e = 0; e < nE; e++)
       for (int64_t q = L.pOff[e]; q < L.pOff[e + 1]; q++) {
         int at = fill[L.pK[q]]++;
         midEnt[at] = e;
         midA[at] = L.pA[q];
+        A->midQ[at] = q;
       }
     std::vector<int> qt(hs.r1Col.size(), -1), springQ;
     for (int q : hs.cSlotR1)
@@ -284,6 +289,23 @@ int ensure_state(pdipc_handle* h) {
     if ((rc = dalloc(A, &u, V3))) return rc;
   return PDIPC_OK;
 }
+
+}  // namespace
+
+// Re-uploads the adjoint's copies of stiffness-dependent values after
+// pdipc_set_cloth_stiffness has replaced the SELL 2-path products (h->sell.pA).
+int adjoint_refresh_values(pdipc_handle* h) {
+  AdjointState* A = h->adj;
+  if (!A || A->midQ.empty()) return PDIPC_OK;
+  const SellLayout& L = h->sell;
+  std::vector<double> midA(A->midQ.size());
+  for (size_t at = 0; at < midA.size(); at++) midA[at] = L.pA[A->midQ[at]];
+  HIPCHK(hipStreamSynchronize(h->stream));
+  HIPCHK(hipMemcpy(A->am.midA, midA.data(), midA.size() * sizeof(double), hipMemcpyHostToDevice));
+  return PDIPC_OK;
+}
+
+namespace {
 
 // replays `g` (capturing `body` on first use) or, without graphs, runs `body` directly
 template <class F>

@@ -103,4 +103,5 @@ int broad_phase(pdipc_handle* h, const double* x, const double* dx);
 int collision(pdipc_handle* h);
 int require_device(pdipc_handle* h);
 int advance(pdipc_handle* h);
+int adjoint_refresh_values(pdipc_handle* h);  // adjoint.hip: after a stiffness change
 }  // namespace pdipc

@@ -548,13 +548,14 @@ int pdipc_set_cloth_stiffness(pdipc_handle* h, int model, const double* k, int n
   build_sell(h->hs, h->sell.lanes, h->sell);
   DeviceMesh& m = h->m;
   const HostSetup& hs = h->hs;
+  HIPCHK(hipStreamSynchronize(h->stream));  // the uploads below go through the null stream
   HIPCHK(hipMemcpy(m.elasticDiag, hs.elasticDiag.data(), hs.elasticDiag.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(m.r1Val, hs.r1Val.data(), hs.r1Val.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(m.cStrain, hs.cStrain.data(), hs.cStrain.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(m.m2R1, h->sell.r1.data(), h->sell.r1.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemcpy(m.p2A, h->sell.pA.data(), h->sell.pA.size() * sizeof(double), hipMemcpyHostToDevice));
   HIPCHK(hipMemset(m.dStale, 1, std::max(1, hs.nSim)));  // every M^2 row is recomputed next
-  return PDIPC_OK;
+  return pdipc::adjoint_refresh_values(h);
 }
 
 int pdipc_host_setup(pdipc_handle* h) {

@@ -25,3 +25,20 @@ def gpu_available():
 def golden():
     import numpy as np
     return np.load(os.path.join(ROOT, "tests", "golden", "cloth_armadillo_fixtures.npz"))
+
+
+def pytest_terminal_summary(terminalreporter):
+    """States how many FP64 steps took the relaxed (sensitivity-scaled) bar."""
+    try:
+        import parity_util
+    except Exception:
+        return
+    fb = parity_util.FALLBACKS
+    n = parity_util.CHECKED[0]
+    if not n:
+        return
+    tr = terminalreporter
+    tr.write_line(f"FP64 step parity: {n} step(s) checked, relaxed bar taken on {len(fb)} "
+                  f"(ceiling {parity_util.RELAXED_CEILING:g})")
+    for test, seed, err, sens in fb:
+        tr.write_line(f"  {test} seed/step {seed}: err {err:.3e}, oracle sensitivity {sens:.3e}")
