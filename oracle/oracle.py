@@ -10,7 +10,9 @@ import subprocess
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "_build", "liboracle.so")
+# ORACLE_LIB names another build of the same source (the mutation tests load
+# builds with planted mistakes this way, in a separate process)
+LIB_PATH = os.environ.get("ORACLE_LIB") or os.path.join(HERE, "_build", "liboracle.so")
 
 # order of the flat parameter array (enum ParamIndex in pdipc_oracle.cpp)
 PARAM_ORDER = ["dt", "gravity", "dhat", "kappa", "lg_iters", "cheb_iters", "max_outer",
@@ -80,7 +82,7 @@ def lib():
         L.orc_math_pt_dist2.restype = ctypes.c_double
         L.orc_math_ee_dist2.argtypes = [_D, _D, _D, _D, _D]
         L.orc_math_ee_dist2.restype = ctypes.c_double
-        L.orc_math_ccd.argtypes = [ctypes.c_int, _D, _D, ctypes.c_int]
+        L.orc_math_ccd.argtypes = [ctypes.c_int, _D, _D, ctypes.c_int, ctypes.c_int, ctypes.c_double, ctypes.c_double]
         L.orc_math_ccd.restype = ctypes.c_double
         L.orc_math_barrier.argtypes = [ctypes.c_double, ctypes.c_double, ctypes.c_double]
         L.orc_math_barrier.restype = ctypes.c_double
@@ -320,9 +322,10 @@ def ee_dist2(a0, a1, b0, b1):
     return d, rs
 
 
-def ccd(typ, x, dx, touch_policy=1):
+def ccd(typ, x, dx, touch_policy=1, iters=10, step=0.9, gap=0.1):
+    """ACCD time of impact of one record (the reference constants by default)."""
     x, dx = _f64(np.asarray(x).reshape(12)), _f64(np.asarray(dx).reshape(12))
-    return lib().orc_math_ccd(typ, _dp(x), _dp(dx), touch_policy)
+    return lib().orc_math_ccd(typ, _dp(x), _dp(dx), touch_policy, iters, step, gap)
 
 
 def barrier(d2, dh2, kappa):

@@ -37,6 +37,14 @@
 
 #include <omp.h>
 
+// Planted mistakes for the mutation tests (tests/test_oracle_mutants.py): a
+// build with -DORC_MUTANT=k carries misreading k of the reference, and the
+// closed-form tests must fail on it.  The default build has none.
+#ifndef ORC_MUTANT
+#define ORC_MUTANT 0
+#endif
+#define ORC_MUT(k) (ORC_MUTANT == (k))
+
 #include <algorithm>
 #include <array>
 #include <cfloat>
@@ -677,7 +685,10 @@ static bool ee_cd_filter(const R* a0, const R* a1, const R* b0, const R* b1, R d
 
 // ACCD of ccdVertexTriangleToi / ccdEdgeEdgeToi (PdIpcGpuFunc.cu:2460-2607),
 // with D4 under touch_policy = 1.  Pinned by conservativeness against exact
-// cubic-root TOIs (tests/test_oracle_math.py).
+// cubic-root TOIs (tests/test_oracle_math.py) and by the value of head-on
+// approaches, a partial geometric series in closed form, for the reference
+// constants and other step / gap / iteration settings, with mutation tests
+// (tests/test_oracle_closed_forms.py, tests/test_oracle_mutants.py).
 template <class R>
 static R ccd_toi(int type, R x[4][3], R dx[4][3], int iters, R step, R gapf, int touch_policy) {
   R mov[3];
@@ -719,6 +730,7 @@ static R ccd_toi(int type, R x[4][3], R dx[4][3], int iters, R step, R gapf, int
   if (touch_policy == 1 && !(dcur > 0)) return 0;
   R gap = gapf * dF / dcur;  // = gapf * dcur
   R toc = 0;
+  if (ORC_MUT(3)) step = step * (R)0.8 / (R)0.9;
   for (int i = 0; i < iters; i++) {
     R tlb = step * dF / (dcur * maxdisp);
     for (int j = 0; j < 4; j++)
@@ -726,8 +738,13 @@ static R ccd_toi(int type, R x[4][3], R dx[4][3], int iters, R step, R gapf, int
     dF = dfun(&d2);
     dcur = sqrt(d2);
     if (touch_policy == 1 && !(dcur > 0)) return toc;
-    if (toc != 0 && (dF / dcur) < gap) break;
-    toc += tlb;
+    if (ORC_MUT(4)) {
+      toc += tlb;
+      if ((dF / dcur) < gap) break;
+    } else {
+      if (toc != 0 && (dF / dcur) < gap) break;
+      toc += tlb;
+    }
     if (toc > 1) { toc = 1; break; }
   }
   return toc;
@@ -1178,7 +1195,7 @@ struct Sim {
       patchPoints[i].assign(ps.begin(), ps.end());
       patchExPoints[i].assign(xs.begin(), xs.end());
     }
-    colMass.assign(totalPoints, (R)prm.v[P_STATIC_MASS]);
+    colMass.assign(totalPoints, ORC_MUT(7) ? (R)1 : (R)prm.v[P_STATIC_MASS]);
     for (int i = 0; i < simPoints; i++) colMass[i] = mass[i];
   }
 
@@ -1236,7 +1253,11 @@ struct Sim {
     }
   }
 
-  // solveClothPointsStrainAndBendingConstraintKernel (PdIpcGpuFunc.cu:627-728)
+  // solveClothPointsStrainAndBendingConstraintKernel (PdIpcGpuFunc.cu:627-728).
+  // Pinned at rest (rhs = K_elastic x) and at uniformly scaled and shifted rest
+  // shapes of a curved sheet, where springs return l0 * d_hat and the
+  // curvature vector is rescaled back to its rest value (closed forms, with
+  // mutation tests of both ratios).
   void local_cloth(const std::vector<R>& x, std::vector<R>& rhs) {
     for (int tid = 0; tid < clothPoints; tid++) {
       const auto& nb = cNbr[tid];
@@ -1249,14 +1270,14 @@ struct Sim {
         const R* nx = &x[3 * nb[i]];
         R d[3] = {xi[0] - nx[0], xi[1] - nx[1], xi[2] - nx[2]};
         R cur = sqrt(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-        R rc = cStrain[tid][i] * (cRest[tid][i] / cur);
+        R rc = cStrain[tid][i] * (ORC_MUT(5) ? cur / cRest[tid][i] : cRest[tid][i] / cur);
         for (int k = 0; k < 3; k++) tx[k] += rc * d[k];
         lbo = cLbo[tid][i];
         for (int k = 0; k < 3; k++) aq[k] += nx[k] * lbo;
       }
       R an = sqrt(dot3(aq, aq));
       if (an >= (R)1e-10) {
-        R c = cH0[tid] / an;
+        R c = ORC_MUT(6) ? an / cH0[tid] : cH0[tid] / an;
         for (int k = 0; k < 3; k++) aq[k] *= c;
       }
       R b = cBend[tid][0];
@@ -1487,7 +1508,10 @@ struct Sim {
     return toiMin;
   }
 
-  // setCollisionConstraintsKernel (PdIpcGpuFunc.cu:2756-2892)
+  // setCollisionConstraintsKernel (PdIpcGpuFunc.cu:2756-2892).  Pinned by
+  // closed forms (a point over a static triangle, a point over a simulated
+  // triangle of equal mass, edges crossing at right angles: weights, diagonal
+  // sums and targets from the geometry) with mutation tests.
   // The activity test (filter and distance) runs over the records in
   // parallel; the accumulation then visits the active records in record order,
   // so the sums are those of the serial loop.
@@ -1540,16 +1564,19 @@ struct Sim {
         colDiag[t0] += w; colDiag[t1] += w; colDiag[t2] += w; colDiag[t3] += w;
         rp = mq / (mp + mq);
         rq = mp / (mp + mq);
+        if (ORC_MUT(1)) std::swap(rp, rq);
         dist = dHat - dist;
         dp = dist * rp;
         dq = dist * rq;
         R len = sqrt(dot3(dir, dir));
         for (int k = 0; k < 3; k++) dir[k] /= len;
+        R wt[3] = {1, 1, 1};
+        if (ORC_MUT(2)) { wt[0] = a; wt[1] = b; wt[2] = cc; }
         for (int k = 0; k < 3; k++) {
           colRhs[3 * t0 + k] += w * (x0[k] + dp * dir[k]);
-          colRhs[3 * t1 + k] += w * (x1[k] - dq * dir[k]);
-          colRhs[3 * t2 + k] += w * (x2[k] - dq * dir[k]);
-          colRhs[3 * t3 + k] += w * (x3[k] - dq * dir[k]);
+          colRhs[3 * t1 + k] += w * (x1[k] - wt[0] * dq * dir[k]);
+          colRhs[3 * t2 + k] += w * (x2[k] - wt[1] * dq * dir[k]);
+          colRhs[3 * t3 + k] += w * (x3[k] - wt[2] * dq * dir[k]);
         }
       } else {
         if (t0 == t2 || t0 == t3 || t1 == t2 || t1 == t3) continue;
@@ -1564,6 +1591,7 @@ struct Sim {
         colDiag[t0] += w; colDiag[t1] += w; colDiag[t2] += w; colDiag[t3] += w;
         rp = mq / (mp + mq);
         rq = mp / (mp + mq);
+        if (ORC_MUT(1)) std::swap(rp, rq);
         R len = sqrt(dot3(dir, dir));
         for (int k = 0; k < 3; k++) dir[k] /= len;
         dist = dHat - dist;
@@ -1985,11 +2013,12 @@ double orc_math_ee_dist2(const double* a0, const double* a1, const double* b0, c
   double dir[3];
   return edge_edge_sqd<double>(a0, a1, b0, b1, &rs[0], &rs[1], dir);
 }
-double orc_math_ccd(int type, const double* x, const double* dx, int touch_policy) {
+double orc_math_ccd(int type, const double* x, const double* dx, int touch_policy, int iters, double step,
+                    double gap) {
   double X[4][3], D[4][3];
   for (int i = 0; i < 4; i++)
     for (int k = 0; k < 3; k++) { X[i][k] = x[3 * i + k]; D[i][k] = dx[3 * i + k]; }
-  return ccd_toi<double>(type, X, D, 10, 0.9, 0.1, touch_policy);
+  return ccd_toi<double>(type, X, D, iters, step, gap, touch_policy);
 }
 double orc_math_barrier(double d2, double dh2, double kappa) { return barrier_value<double>(d2, dh2, kappa); }
 
