@@ -179,6 +179,19 @@ int pdipc_debug_lbvh(pdipc_handle* h, unsigned long long* keys, int* left, int* 
 int pdipc_debug_sell_layout(pdipc_handle* h, int lanes, long long* sizes, long long* slice_off, int* slice_w,
                             int* col, int* row_flag, short* col16, int* perm);
 
+/* --- no counterpart in the reference: for callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The same minimum time of impact as pdipc_debug_toi over
+ * the current candidate records, on device-resident AoS positions x0, x1 (3 *
+ * total points doubles each), written to the device double *toi.  The work is
+ * only enqueued on `stream` (NULL: the handle's stream); nothing is allocated
+ * and the simulation state is untouched.  Per-workgroup minima go to a
+ * library-wide device array of 512 KiB per GPU, held until the process ends,
+ * so at most 16,777,216 candidate records (more: PDIPC_E_CAPACITY).  Every
+ * call on one GPU uses that GPU's array: calls that overlap on one GPU (other
+ * streams or threads) race. */
+int pdipc_toi_noalloc(pdipc_handle* h, const double* x0, const double* x1, double* toi, void* stream);
+
 /* --- measurement support (no counterpart in the reference): replays one
  * kernel of the step `reps` times on the handle's stream, bracketed by HIP
  * events, on the state left by the last step (work buffers are clobbered, the

@@ -8,7 +8,7 @@ HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libpdipc.so")
 SOURCES = ["host_setup.cpp", "k_step.hip", "k_local.hip", "k_solve.hip", "k_contact.hip", "k_broad.hip",
-           "pdipc_api.hip", "adjoint.hip"]
+           "pdipc_api.hip", "adjoint.hip", "noalloc.hip"]
 HIPCC = os.environ.get("HIPCC", "/opt/rocm/bin/hipcc")
 
 # Packed FP32 instructions are disabled for all device code (hipcc flag, not a

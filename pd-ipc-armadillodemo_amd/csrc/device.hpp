@@ -215,6 +215,10 @@ void launch_add_rows(double* dst, const double* src, int64_t n, hipStream_t s);
 void launch_toi_init(const DeviceMesh& m, hipStream_t s);
 void launch_ccd(const DeviceMesh& m, int nRecs, double filter, int iters, double step, double gap, int touchPolicy,
                 hipStream_t s);
+// CCD over m.recs on AoS positions; the minimum of workgroup b into blockMin[b]
+// for b < ceil(nRecs / kBlock) (every such entry written, no atomics)
+void launch_ccd_blockmin(const DeviceMesh& m, int nRecs, const double* x0Aos, const double* x1Aos, double filter,
+                         int iters, double step, double gap, int touchPolicy, double* blockMin, hipStream_t s);
 // writes colDiag / colRhs of every simulated point (deterministic order)
 int launch_collision(DeviceMesh& m, int nRecs, double dhat, double kappa, double dt, hipStream_t s);
 // k_broad.hip

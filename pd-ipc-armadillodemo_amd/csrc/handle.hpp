@@ -40,6 +40,7 @@ struct pdipc_handle {
   SellLayout sell;
   DeviceMesh m;
   hipStream_t stream = nullptr;
+  int device = 0;  // the HIP device current at pdipc_init
   // local/global loop captured once per (collision flag, precision) and replayed
   hipGraphExec_t lgGraph[2][2] = {{nullptr, nullptr}, {nullptr, nullptr}};
   std::vector<void*> allocs;
@@ -104,4 +105,5 @@ int collision(pdipc_handle* h);
 int require_device(pdipc_handle* h);
 int advance(pdipc_handle* h);
 int adjoint_refresh_values(pdipc_handle* h);  // adjoint.hip: after a stiffness change
+int noalloc_setup(int device);                 // noalloc.hip: with `device` current
 }  // namespace pdipc

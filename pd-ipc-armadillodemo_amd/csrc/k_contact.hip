@@ -222,6 +222,38 @@ __host__ __device__ inline P3 load3(const double* X, int ld, int i) { return {{X
 
 __global__ void k_toi_init(unsigned long long* bits) { *bits = (unsigned long long)__double_as_longlong(1.0); }
 
+// One record's ACCD time of impact between positions X0 and X1 (1 when the
+// record shares a vertex or fails the swept-box filter).  Component c of point
+// i is X[c * cs + i * ps]: cs = ld, ps = 1 for the SoA state vectors, cs = 1,
+// ps = 3 for AoS arrays.
+__device__ __forceinline__ double record_toi(int4 r, int cs, int ps, const double* __restrict__ X0,
+                                             const double* __restrict__ X1, double filter, int iters, double step,
+                                             double gapf, int touchPolicy) {
+  int type = r.x < 0 ? 1 : 0;
+  int id[4] = {type ? ~r.x : r.x, r.y, r.z, r.w};
+  bool skip = type == 0 ? (id[0] == id[1] || id[0] == id[2] || id[0] == id[3])
+                        : (id[0] == id[2] || id[0] == id[3] || id[1] == id[2] || id[1] == id[3]);
+  if (skip) return 1.0;
+  P3 x[4], dx[4];
+#pragma unroll
+  for (int q = 0; q < 4; q++) {
+    const int64_t b = (int64_t)id[q] * ps;
+    x[q] = {{X0[b], X0[b + cs], X0[b + 2 * (int64_t)cs]}};
+    P3 y = {{X1[b], X1[b + cs], X1[b + 2 * (int64_t)cs]}};
+    dx[q] = sub(y, x[q]);
+  }
+  bool pass = type == 0 ? pt_ccd_filter(x, dx, filter) : ee_ccd_filter(x, dx, filter);
+  return pass ? accd(type, x, dx, iters, step, gapf, touchPolicy) : 1.0;
+}
+
+__device__ __forceinline__ double wave_min(double toi) {
+  for (int o = kWave / 2; o > 0; o >>= 1) {
+    double other = __shfl_xor(toi, o, kWave);
+    toi = other < toi ? other : toi;
+  }
+  return toi;
+}
+
 __global__ void __launch_bounds__(kBlock) k_ccd(int nRecs, int ld, const int4* __restrict__ recs,
                                                 const double* __restrict__ X0, const double* __restrict__ X1,
                                                 double filter, int iters, double step, double gapf, int touchPolicy,
@@ -229,31 +261,34 @@ __global__ void __launch_bounds__(kBlock) k_ccd(int nRecs, int ld, const int4* _
   int t = blockIdx.x * blockDim.x + threadIdx.x;
   double toi = 1.0;
   if (t < nRecs) {
-    int4 r = recs[t];
-    int type = r.x < 0 ? 1 : 0;
-    int id[4] = {type ? ~r.x : r.x, r.y, r.z, r.w};
-    bool skip = type == 0 ? (id[0] == id[1] || id[0] == id[2] || id[0] == id[3])
-                          : (id[0] == id[2] || id[0] == id[3] || id[1] == id[2] || id[1] == id[3]);
-    if (!skip) {
-      P3 x[4], dx[4];
-#pragma unroll
-      for (int q = 0; q < 4; q++) {
-        x[q] = load3(X0, ld, id[q]);
-        P3 y = load3(X1, ld, id[q]);
-        dx[q] = sub(y, x[q]);
-      }
-      bool pass = type == 0 ? pt_ccd_filter(x, dx, filter) : ee_ccd_filter(x, dx, filter);
-      if (pass) {
-        double v = accd(type, x, dx, iters, step, gapf, touchPolicy);
-        if (v < toi) toi = v;  // NaN (reference rule) never wins
-      }
-    }
+    double v = record_toi(recs[t], ld, 1, X0, X1, filter, iters, step, gapf, touchPolicy);
+    if (v < toi) toi = v;  // NaN (reference rule) never wins
   }
-  for (int o = kWave / 2; o > 0; o >>= 1) {
-    double other = __shfl_xor(toi, o, kWave);
-    toi = other < toi ? other : toi;
-  }
+  toi = wave_min(toi);
   if ((threadIdx.x % kWave) == 0 && toi < 1.0) atomicMin(toiBits, (unsigned long long)__double_as_longlong(toi));
+}
+
+// The same minimum without atomics on AoS positions: each workgroup writes its
+// own minimum to blockMin[blockIdx.x] (every entry of the grid is written).
+__global__ void __launch_bounds__(kBlock) k_ccd_blockmin(int nRecs, const int4* __restrict__ recs,
+                                                         const double* __restrict__ X0, const double* __restrict__ X1,
+                                                         double filter, int iters, double step, double gapf,
+                                                         int touchPolicy, double* __restrict__ blockMin) {
+  __shared__ double waveMin[kBlock / kWave];
+  int t = blockIdx.x * blockDim.x + threadIdx.x;
+  double toi = 1.0;
+  if (t < nRecs) {
+    double v = record_toi(recs[t], 1, 3, X0, X1, filter, iters, step, gapf, touchPolicy);
+    if (v < toi) toi = v;
+  }
+  toi = wave_min(toi);
+  if ((threadIdx.x % kWave) == 0) waveMin[threadIdx.x / kWave] = toi;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    double b = waveMin[0];
+    for (int w = 1; w < kBlock / kWave; w++) b = waveMin[w] < b ? waveMin[w] : b;
+    blockMin[blockIdx.x] = b;
+  }
 }
 
 // barrier (PdIpcGpuFunc.cu:2745-2754)
@@ -599,6 +634,13 @@ void launch_ccd(const DeviceMesh& m, int nRecs, double filter, int iters, double
   if (nRecs <= 0) return;
   hipLaunchKernelGGL(k_ccd, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, m.ld, m.recs, m.Xold, m.X,
                      filter, iters, step, gap, touchPolicy, m.toiBits);
+}
+
+void launch_ccd_blockmin(const DeviceMesh& m, int nRecs, const double* x0Aos, const double* x1Aos, double filter,
+                         int iters, double step, double gap, int touchPolicy, double* blockMin, hipStream_t s) {
+  if (nRecs <= 0) return;
+  hipLaunchKernelGGL(k_ccd_blockmin, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, m.recs, x0Aos,
+                     x1Aos, filter, iters, step, gap, touchPolicy, blockMin);
 }
 
 void launch_ccd_records(const DeviceMesh& m, const int4* recs, int nRecs, const double* x0, const double* x1,

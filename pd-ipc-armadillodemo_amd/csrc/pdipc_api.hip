@@ -595,6 +595,8 @@ int pdipc_init(pdipc_handle* h) {
   if (h->deviceReady) return PDIPC_OK;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(PDIPC_E_NODEVICE, "no HIP device");
+  HIPCHK(hipGetDevice(&h->device));
+  if ((rc = pdipc::noalloc_setup(h->device))) return rc;
   HIPCHK(hipStreamCreateWithFlags(&h->stream, hipStreamNonBlocking));
   if ((rc = upload_all(h))) return rc;
   h->cnt.sell_entries = h->m.sellEntries;

@@ -75,6 +75,7 @@ SIGNATURES = {
     "pdipc_get_model_dat": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int, _I]),
     "pdipc_debug_broad_phase": (ctypes.c_int, [_H, _D, _D, _I]),
     "pdipc_debug_toi": (ctypes.c_int, [_H, _D, _D, _D]),
+    "pdipc_toi_noalloc": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pdipc_debug_collision": (ctypes.c_int, [_H, _D, _D, _D, _I]),
     "pdipc_debug_local": (ctypes.c_int, [_H, _D, _D, _D]),
     "pdipc_debug_solve": (ctypes.c_int, [_H, _D, _D, _D, _D]),
@@ -334,6 +335,12 @@ class Simulator:
         t = ctypes.c_double()
         _check(lib().pdipc_debug_toi(self.h, _dp(a), _dp(b), ctypes.byref(t)))
         return t.value
+
+    def toi_noalloc(self, x0_dev, x1_dev, toi_dev, stream=0):
+        """pdipc_toi_noalloc on device addresses (ints): AoS positions x0, x1
+        and one double for the result; enqueued on `stream` (0: the handle's)."""
+        _check(lib().pdipc_toi_noalloc(self.h, ctypes.c_void_p(x0_dev), ctypes.c_void_p(x1_dev),
+                                       ctypes.c_void_p(toi_dev), ctypes.c_void_p(stream or None)))
 
     def debug_collision(self, x):
         a = np.ascontiguousarray(np.asarray(x, np.float64).reshape(-1))
