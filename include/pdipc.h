@@ -63,6 +63,8 @@ typedef struct pdipc_params {
   int precision;       /* PDIPC_FP64 or PDIPC_FP32_SPLIT */
   int sell_lanes;      /* lanes per row in the Chebyshev sweep (1, 2, 4, 8); 0 = default */
   int use_graphs;      /* 1: replay the local/global loop from a HIP graph */
+  int sell_stage;      /* 1: the sweeps stage their gathered operand in LDS, per workgroup, when
+                          every workgroup's column union fits (default); 0: gather from L2 */
 } pdipc_params;
 
 typedef struct pdipc_counters {
@@ -178,6 +180,17 @@ int pdipc_debug_lbvh(pdipc_handle* h, unsigned long long* keys, int* left, int* 
  * slot -> row order (only if rows are reordered). */
 int pdipc_debug_sell_layout(pdipc_handle* h, int lanes, long long* sizes, long long* slice_off, int* slice_w,
                             int* col, int* row_flag, short* col16, int* perm);
+/* Validation, no counterpart in the reference: the LDS staging tables of the
+ * sweeps for `lanes` lanes per row (after pdipc_host_setup; no device needed).
+ * A tile is sizes[5] consecutive slices (one workgroup).  sizes[0..5] receive
+ * 1 if staging applies (every tile's column union spans at most 65536 columns
+ * and holds at most 1024), the tile count, the total union length, the
+ * largest union, the SELL entry count and the slices per tile.  When staging
+ * applies the arrays (may be null) receive tile count + 1 union offsets, the
+ * tiles' smallest columns, the union columns minus their tile's smallest, and
+ * per SELL entry its index in its tile's union. */
+int pdipc_debug_stage_layout(pdipc_handle* h, int lanes, long long* sizes, int* tile_off, int* tile_base,
+                             unsigned short* tile_col, unsigned short* loc);
 
 /* --- no counterpart in the reference: for callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
@@ -200,6 +213,21 @@ int pdipc_toi_noalloc(pdipc_handle* h, const double* x0, const double* x1, doubl
  * operator update.  Returns the mean duration per launch in microseconds and
  * the algorithmic HBM bytes one launch moves (matrix entries once, vectors once). */
 int pdipc_time_kernel(pdipc_handle* h, int which, int reps, double* avg_us, double* bytes);
+
+/* --- measurement support (no counterpart in the reference; the reference's
+ * timing members are declared but never written, PdIpcSimulator.h:54-64):
+ * in-step kernel timing.  After pdipc_kernel_timing(h, 1) the following steps
+ * run with direct launches (no graph replay) and every launch of the timed
+ * kernels records its own start and stop with HIP events, the duration a
+ * profiler's kernel trace reports; pdipc_kernel_timing(h, 0) stops and a new
+ * enable starts from zero.  pdipc_get_kernel_times returns, for the first
+ * nclasses classes, the summed microseconds and launch counts since then:
+ * 0 FP64 sweeps, 1 FP32-split sweeps, 2 local-step rhs gather, 3 cloth
+ * curvature and tet projection, 4 first FP64 sweep and FP32-split residual
+ * passes, 5 M^2 operator update, 6 time of impact, 7 barrier terms, 8 broad
+ * phase (one span per broad phase, its host waits included). */
+int pdipc_kernel_timing(pdipc_handle* h, int enable);
+int pdipc_get_kernel_times(pdipc_handle* h, double* us, long long* counts, int nclasses);
 
 /* --- per-triangle strain stiffness of a cloth (an extension: the reference
  * takes one strain weight per cloth model, PdIpcClothModel.h:282,300).  Each

@@ -310,7 +310,7 @@ namespace {
 // replays `g` (capturing `body` on first use) or, without graphs, runs `body` directly
 template <class F>
 int run_captured(pdipc_handle* h, hipGraphExec_t& g, F&& body) {
-  if (!h->prm.use_graphs) {
+  if (!h->prm.use_graphs || h->m.timer) {  // kernel timing times each launch: no graph replay
     body();
     return check_launch(h);
   }

@@ -5,7 +5,10 @@
 #pragma once
 #include <hip/hip_runtime.h>
 
+#include <hip/hip_ext.h>
+
 #include <cstdint>
+#include <vector>
 
 namespace pdipc {
 
@@ -27,7 +30,65 @@ struct __align__(16) BvhNode {
   int skip;
 };
 
+// In-step kernel timing (pdipc_kernel_timing): while a timer is attached to
+// the mesh, every launch of the timed kernels records its own start and stop
+// with HIP events (hipExtLaunchKernel), i.e. the kernel's execution time as a
+// profiler reports it; the broad phase is timed as one span.
+enum TimerClass {
+  TC_SWEEP64 = 0,  // FP64 Chebyshev sweeps
+  TC_SWEEP32,      // FP32-split sweeps
+  TC_RHS,          // local step: the rhs gather (k_rhs)
+  TC_PROJECT,      // local step: cloth curvature and tet projection
+  TC_FIRST,        // first FP64 sweep, FP32-split residual passes
+  TC_OPERATORS,    // M^2 operator update
+  TC_CCD,          // time of impact
+  TC_BARRIER,      // barrier projection terms
+  TC_BROAD,        // broad phase (span, host waits included)
+  TC_COUNT
+};
+
+struct KernelTimer {
+  std::vector<hipEvent_t> pool;  // (start, stop) pairs
+  std::vector<int> cls;          // class of each pair in use
+  size_t used = 0;
+  double us[TC_COUNT] = {};
+  long long count[TC_COUNT] = {};
+  // a (start, stop) pair for one launch of class c (false: no event available)
+  bool take(int c, hipEvent_t* e0, hipEvent_t* e1) {
+    if (2 * used + 2 > pool.size()) {
+      hipEvent_t a = nullptr, b = nullptr;
+      if (hipEventCreate(&a) != hipSuccess) return false;
+      if (hipEventCreate(&b) != hipSuccess) {
+        (void)hipEventDestroy(a);
+        return false;
+      }
+      pool.push_back(a);
+      pool.push_back(b);
+    }
+    *e0 = pool[2 * used];
+    *e1 = pool[2 * used + 1];
+    if (cls.size() <= used) cls.resize(used + 1);
+    cls[used++] = c;
+    return true;
+  }
+  ~KernelTimer() {
+    for (hipEvent_t e : pool) (void)hipEventDestroy(e);
+  }
+};
+
+// a kernel launch, timed when a timer is attached
+template <class... KA, class... A>
+inline void launch_t(KernelTimer* t, int cls, void (*kernel)(KA...), dim3 grid, dim3 block, uint32_t shm,
+                     hipStream_t s, A... args) {
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  if (t && t->take(cls, &e0, &e1))
+    hipExtLaunchKernelGGL(kernel, grid, block, shm, s, e0, e1, 0, args...);
+  else
+    hipLaunchKernelGGL(kernel, grid, block, shm, s, args...);
+}
+
 struct DeviceMesh {
+  KernelTimer* timer = nullptr;  // in-step kernel timing (null: off)
   int nTotal = 0, nSim = 0, nCloth = 0, nTets = 0;
   int ld = 0;  // leading dimension of SoA vectors (>= nTotal, multiple of 64)
   // state and work vectors, SoA [3][ld] doubles
@@ -73,6 +134,12 @@ struct DeviceMesh {
   int64_t* p2Off = nullptr;
   int* p2K = nullptr;
   double* p2A = nullptr;
+  // LDS staging of the sweeps' operand (null: the sweeps gather from L2): per
+  // tile of kTileSlices slices the union of its columns (tileCol + tileBase,
+  // tileOff[t] .. tileOff[t + 1]) and per entry its index in that union
+  int *tileOff = nullptr, *tileBase = nullptr;
+  uint16_t *tileCol = nullptr, *m2Loc = nullptr;
+  int stageMax = 0;
   // collision topology
   int nFaces = 0, nEdges = 0, nPatches = 0;
   int3* bFaces = nullptr;

@@ -74,6 +74,7 @@ struct pdipc_handle {
   }
   pdipc::AdjointState* adj = nullptr;
   ~pdipc_handle() {
+    delete m.timer;
     pdipc::adjoint_free(adj);
     for (void* p : allocs) (void)hipFree(p);
     if (scalHost) (void)hipHostFree(scalHost);

@@ -791,6 +791,50 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
   return std::string();
 }
 
+// The column union of each tile of kTileSlices slices (see SellLayout).
+static void build_stage_tiles(SellLayout& L, int nt) {
+  const int nTiles = (L.nSlices + kTileSlices - 1) / kTileSlices;
+  if (L.sliceOff.back() >= ((int64_t)1 << 29)) return;  // the staged sweeps address entries in 32 bits
+  if (L.lanes < 2) return;  // one lane per row: a tile spans 256 rows, not staged
+  std::vector<std::vector<int>> uni(nTiles);
+  std::vector<uint8_t> ok(nTiles, 1);
+  const int64_t total = L.sliceOff.back();
+  L.loc.assign(total, 0);
+  parallel_ranges(nTiles, nt, [&](int64_t t0, int64_t t1, int) {
+    for (int t = (int)t0; t < (int)t1; t++) {
+      const int64_t e0 = L.sliceOff[(size_t)t * kTileSlices];
+      const int64_t e1 = L.sliceOff[std::min(L.nSlices, (t + 1) * kTileSlices)];
+      std::vector<int>& U = uni[t];
+      U.assign(L.col.begin() + e0, L.col.begin() + e1);
+      std::sort(U.begin(), U.end());
+      U.erase(std::unique(U.begin(), U.end()), U.end());
+      if (U.empty() || U.back() - U.front() > 65535 || (int)U.size() > kStageMaxCols) {
+        ok[t] = 0;
+        continue;
+      }
+      for (int64_t e = e0; e < e1; e++)
+        L.loc[e] = (uint16_t)(std::lower_bound(U.begin(), U.end(), L.col[e]) - U.begin());
+    }
+  });
+  L.stageOk = nTiles > 0 && std::all_of(ok.begin(), ok.end(), [](uint8_t v) { return v != 0; });
+  if (!L.stageOk) {
+    L.loc.clear();
+    return;
+  }
+  L.tileOff.assign(nTiles + 1, 0);
+  L.tileBase.assign(nTiles, 0);
+  for (int t = 0; t < nTiles; t++) {
+    L.tileOff[t + 1] = L.tileOff[t] + (int)uni[t].size();
+    L.tileBase[t] = uni[t].front();
+    L.stageMax = std::max(L.stageMax, (int)uni[t].size());
+  }
+  L.tileCol.resize(L.tileOff.back());
+  parallel_ranges(nTiles, nt, [&](int64_t t0, int64_t t1, int) {
+    for (int t = (int)t0; t < (int)t1; t++)
+      for (size_t i = 0; i < uni[t].size(); i++) L.tileCol[L.tileOff[t] + i] = (uint16_t)(uni[t][i] - L.tileBase[t]);
+  });
+}
+
 void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
   L = SellLayout();
   L.lanes = lanes;
@@ -887,6 +931,7 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
           L.col16[e] = (int16_t)(L.col[e] - (s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) % 64) / lanes));
     });
   }
+  build_stage_tiles(L, nt);
   L.pOff.assign(total + 1, 0);
   for (int64_t e = 0; e < total; e++) {
     int64_t q = src[e];

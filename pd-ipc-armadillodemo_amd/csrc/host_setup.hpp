@@ -115,8 +115,21 @@ struct SellLayout {
   std::vector<double> pA;
   std::vector<int> r1Idx;          // per SELL entry: R1 index of (i,j), -1 if none or padding
   std::vector<int> pIa, pIb;       // per 2-path: R1 indices of its two factors
+  // LDS staging of the sweeps' gathered operand, per tile of kTileSlices
+  // consecutive slices (one workgroup): the sorted union of the columns of the
+  // tile's entries (padding included) as 16-bit offsets from the tile's
+  // smallest column, and per entry its index in that union.  stageOk: every
+  // tile's union spans at most 65536 columns and holds at most kStageMaxCols.
+  bool stageOk = false;
+  int stageMax = 0;                // largest union
+  std::vector<int> tileOff;        // nTiles + 1: start of each tile's union in tileCol
+  std::vector<int> tileBase;       // nTiles: smallest column of the tile
+  std::vector<uint16_t> tileCol;   // union columns - tileBase
+  std::vector<uint16_t> loc;       // per SELL entry: index in its tile's union
 };
 constexpr int kSellSortSlices = 2;  // slices per row-sorting window
+constexpr int kTileSlices = 4;      // slices per staged tile (a 256-lane workgroup)
+constexpr int kStageMaxCols = 1024; // largest staged union (16 KiB of FP32 rows, 32 KiB of FP64 rows)
 void build_sell(const HostSetup& hs, int lanes, SellLayout& out);
 
 }  // namespace pdipc

@@ -632,8 +632,8 @@ void launch_toi_init(const DeviceMesh& m, hipStream_t s) {
 void launch_ccd(const DeviceMesh& m, int nRecs, double filter, int iters, double step, double gap, int touchPolicy,
                 hipStream_t s) {
   if (nRecs <= 0) return;
-  hipLaunchKernelGGL(k_ccd, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, m.ld, m.recs, m.Xold, m.X,
-                     filter, iters, step, gap, touchPolicy, m.toiBits);
+  launch_t(m.timer, TC_CCD, k_ccd, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, m.ld, m.recs,
+           m.Xold, m.X, filter, iters, step, gap, touchPolicy, m.toiBits);
 }
 
 void launch_ccd_blockmin(const DeviceMesh& m, int nRecs, const double* x0Aos, const double* x1Aos, double filter,
@@ -673,16 +673,16 @@ int launch_collision(DeviceMesh& m, int nRecs, double dhat, double kappa, double
   hipMemsetAsync(m.ptCount, 0, ((size_t)m.nSim + 1) * sizeof(int), s);
   hipMemsetAsync(m.ptCursor, 0, (size_t)m.nSim * sizeof(int), s);
   if (nRecs > 0)
-    hipLaunchKernelGGL(k_col_terms, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, m.ld, m.nSim,
-                       m.recs, m.X, m.colMass, dhat, kappa, dt, m.actCap, m.actRec, m.actW, m.actT, m.ptCount,
-                       m.counters);
+    launch_t(m.timer, TC_BARRIER, k_col_terms, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, m.ld,
+             m.nSim, m.recs, m.X, m.colMass, dhat, kappa, dt, m.actCap, m.actRec, m.actW, m.actT, m.ptCount,
+             m.counters);
   launch_exclusive_scan(m.ptCount, m.ptOff, m.nSim + 1, nullptr, m.ptScan, s);
-  hipLaunchKernelGGL(k_col_fill, dim3(512), dim3(kBlock), 0, s, m.ld, m.nSim, m.actCap, m.recs, m.counters, m.actRec, m.ptOff,
-                     m.ptCursor, m.entKey, m.entA, m.entPt);
-  hipLaunchKernelGGL(k_col_rank, dim3(512), dim3(kBlock), 0, s, m.nSim, 4 * m.actCap, m.ptOff, m.entKey, m.entA, m.entPt,
-                     m.entSorted);
-  hipLaunchKernelGGL(k_col_gather, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, m.ptOff,
-                     m.entSorted, m.actW, m.actT, m.colDiag, m.colRhs);
+  launch_t(m.timer, TC_BARRIER, k_col_fill, dim3(512), dim3(kBlock), 0, s, m.ld, m.nSim, m.actCap, m.recs, m.counters,
+           m.actRec, m.ptOff, m.ptCursor, m.entKey, m.entA, m.entPt);
+  launch_t(m.timer, TC_BARRIER, k_col_rank, dim3(512), dim3(kBlock), 0, s, m.nSim, 4 * m.actCap, m.ptOff, m.entKey,
+           m.entA, m.entPt, m.entSorted);
+  launch_t(m.timer, TC_BARRIER, k_col_gather, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld,
+           m.ptOff, m.entSorted, m.actW, m.actT, m.colDiag, m.colRhs);
   return 0;
 }
 

@@ -362,21 +362,16 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, ResidualArgs ra, co
 template <class T>
 void local_impl(const DeviceMesh& m, T lo, T hi, int withCol, double resW, hipStream_t s) {
   if (m.nCloth > 0)
-    hipLaunchKernelGGL(k_cloth_curvature<T>, dim3(grid_for(m.nCloth)), dim3(kBlock), 0, s, m.nCloth, m.ld, m.cCenter,
-                       m.cOff, m.cNbr, m.cLbo, m.cH0, m.X, m.clothA);
+    launch_t(m.timer, TC_PROJECT, k_cloth_curvature<T>, dim3(grid_for(m.nCloth)), dim3(kBlock), 0, s, m.nCloth, m.ld,
+             m.cCenter, m.cOff, m.cNbr, m.cLbo, m.cH0, m.X, m.clothA);
   if (m.nTets > 0)
-    hipLaunchKernelGGL(k_tet_project<T>, dim3(grid_for(m.nTets)), dim3(kBlock), 0, s, m.nTets, m.ld, lo, hi,
-                       m.tetIdx, m.tetDminv, m.tetW, m.X, m.tetOut);
+    launch_t(m.timer, TC_PROJECT, k_tet_project<T>, dim3(grid_for(m.nTets)), dim3(kBlock), 0, s, m.nTets, m.ld, lo, hi,
+             m.tetIdx, m.tetDminv, m.tetW, m.X, m.tetOut);
   const ResidualArgs ra{resW, m.r1Off, m.r1Col, m.r1Val, m.sysDiag, m.xa0, m.cA};
   const dim3 grid(grid_for((int64_t)m.nSim * kRhsLanes));
-  if (resW >= 0.0)
-    hipLaunchKernelGGL((k_rhs<T, true>), grid, dim3(kBlock), 0, s, m.nSim, m.ld, m.nTets, withCol, ra, m.initRhs,
-                       m.vtOff, m.vtList, m.tetOut, m.clothOfPoint, m.cOff, m.cNbr, m.cRest, m.cStrain, m.cBendIn,
-                       m.clothA, m.X, m.colRhs, m.fixK, m.Xrest, m.sysRhs);
-  else
-    hipLaunchKernelGGL((k_rhs<T, false>), grid, dim3(kBlock), 0, s, m.nSim, m.ld, m.nTets, withCol, ra, m.initRhs,
-                       m.vtOff, m.vtList, m.tetOut, m.clothOfPoint, m.cOff, m.cNbr, m.cRest, m.cStrain, m.cBendIn,
-                       m.clothA, m.X, m.colRhs, m.fixK, m.Xrest, m.sysRhs);
+  launch_t(m.timer, TC_RHS, resW >= 0.0 ? k_rhs<T, true> : k_rhs<T, false>, grid, dim3(kBlock), 0, s, m.nSim, m.ld,
+           m.nTets, withCol, ra, m.initRhs, m.vtOff, m.vtList, m.tetOut, m.clothOfPoint, m.cOff, m.cNbr, m.cRest,
+           m.cStrain, m.cBendIn, m.clothA, m.X, m.colRhs, m.fixK, m.Xrest, m.sysRhs);
 }
 
 // ---- adjoint of the local step (FP64), evaluated at the recorded x

@@ -26,6 +26,7 @@
 // which is algebraically the same iterate; FP32 only rounds the (small)
 // correction, never the absolute coordinates.
 #include "device.hpp"
+#include "host_setup.hpp"  // kTileSlices
 
 namespace pdipc {
 namespace {
@@ -345,6 +346,204 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
   }
 }
 
+// LDS-staged sweeps.  One workgroup runs a tile of kTileSlices slices: it
+// first copies the gathered operand at the tile's column union into LDS (the
+// union's columns as 16-bit offsets from the tile's smallest column; SP rows
+// per lane, all loads in flight together), while the lanes' first kChunk
+// matrix entries are already on their way; after one barrier every entry
+// reads its operand from LDS (its union index `loc`) instead of L2.  On a
+// mesh-ordered cloth a 32-row tile touches about 9 runs of ~40 columns (the
+// 4-ring stencil of M^2), so the L2 traffic of the gathers drops from 16 B
+// per entry to about 6 B per row.  Entries are summed in the same order as the
+// L2-gather kernels, so the results are bit-identical to theirs.  Entry
+// offsets are 32-bit (staging requires fewer than 2^29 entries): each load
+// is a 64-bit base in scalar registers plus a 32-bit lane offset, and a
+// lane's loads past its slice's width re-read its last entry (no branches)
+// and are weighted 0.
+constexpr int kChunk32 = 8;  // matrix entries per lane loaded together (FP32 values)
+constexpr int kChunk64 = 4;  // (FP64 values)
+
+template <class T>
+__device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t i) {
+  return *(const T*)((const char*)base + i * (uint32_t)sizeof(T));
+}
+
+template <int CH, class V>
+__device__ __forceinline__ void load_chunk(const V* __restrict__ val, const uint16_t* __restrict__ loc, uint32_t off,
+                                           int c0, int width, V (&v)[CH], int (&li)[CH]) {
+#pragma unroll
+  for (int u = 0; u < CH; u++) {
+    const int c = c0 + u < width ? c0 + u : width - 1;
+    const uint32_t e = off + (uint32_t)(c * kWave);
+    const V x = ld32(val, e);
+    li[u] = ld32(loc, e);
+    v[u] = c0 + u < width ? x : V(0);
+  }
+}
+
+template <class T, int SP>
+__device__ __forceinline__ void stage_tile(T* __restrict__ xs, const T* __restrict__ src, int u0, int nU, int base,
+                                           const uint16_t* __restrict__ tileCol) {
+  int c[SP];
+#pragma unroll
+  for (int q = 0; q < SP; q++) {
+    const int u = threadIdx.x + q * kBlock;
+    c[q] = base + (int)ld32(tileCol, (uint32_t)(u0 + (u < nU ? u : nU - 1)));
+  }
+  T v[SP];
+#pragma unroll
+  for (int q = 0; q < SP; q++) v[q] = ld32(src, (uint32_t)c[q]);
+#pragma unroll
+  for (int q = 0; q < SP; q++)
+    if (threadIdx.x + q * kBlock < nU) xs[threadIdx.x + q * kBlock] = v[q];
+}
+
+template <int G, int LAST, int SP>
+__global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int ld, float W,
+                                                        const int64_t* __restrict__ sliceOff,
+                                                        const int* __restrict__ sliceW, int uw,
+                                                        const int* __restrict__ perm, const uint16_t* __restrict__ loc,
+                                                        const float* __restrict__ val, const int* __restrict__ tileOff,
+                                                        const int* __restrict__ tileBase,
+                                                        const uint16_t* __restrict__ tileCol,
+                                                        const float4* __restrict__ dcur, const float4* __restrict__ dpre,
+                                                        const float4* __restrict__ r0, float4* __restrict__ dnext,
+                                                        double* __restrict__ X) {
+  extern __shared__ float4 xs32[];
+  const int tile = blockIdx.x;
+  const int u0 = tileOff[tile], nU = tileOff[tile + 1] - u0;
+  const int slice = tile * kTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const bool live = slice < nSlices;  // every lane stages and meets the barrier
+  constexpr int S = kWave / G;
+  const int sl = live ? slice : nSlices - 1;
+  const int slot = sl * S + lane / G;
+  const int row = perm ? perm[slot] : slot;
+  const int width = uw > 0 ? uw : sliceW[sl];
+  const uint32_t off = (uint32_t)(uw > 0 ? (int64_t)sl * uw * kWave : sliceOff[sl]) + lane;
+  float v[kChunk32];
+  int li[kChunk32];
+  load_chunk(val, loc, off, 0, width, v, li);
+  const float4 p = ld32(dpre, (uint32_t)row), r = ld32(r0, (uint32_t)row);
+  stage_tile<float4, SP>(xs32, dcur, u0, nU, tileBase[tile], tileCol);
+  __syncthreads();
+  if (!live) return;
+  float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll 1
+  for (int c0 = 0;;) {
+#pragma unroll
+    for (int u = 0; u < kChunk32; u++) {
+      const float4 x = xs32[li[u]];
+      acc[0] += v[u] * x.x;
+      acc[1] += v[u] * x.y;
+      acc[2] += v[u] * x.z;
+    }
+    c0 += kChunk32;
+    if (c0 >= width) break;
+    load_chunk(val, loc, off, c0, width, v, li);
+  }
+  group_reduce<G>(acc);
+  if ((lane % G) != 0 || row >= n) return;
+  float d0 = W * (acc[0] + r.x - p.x) + p.x;
+  float d1 = W * (acc[1] + r.y - p.y) + p.y;
+  float d2 = W * (acc[2] + r.z - p.z) + p.z;
+  if (LAST) {
+    X[row] += (double)d0;
+    X[ld + row] += (double)d1;
+    X[2 * ld + row] += (double)d2;
+  } else {
+    dnext[row] = make_float4(d0, d1, d2, 0.0f);
+  }
+}
+
+template <int G, int SP>
+__global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int ld, double W,
+                                                        const int64_t* __restrict__ sliceOff,
+                                                        const int* __restrict__ sliceW, int uw,
+                                                        const int* __restrict__ perm, const uint16_t* __restrict__ loc,
+                                                        const double* __restrict__ val, const int* __restrict__ tileOff,
+                                                        const int* __restrict__ tileBase,
+                                                        const uint16_t* __restrict__ tileCol,
+                                                        const Row64* __restrict__ cur, const Row64* __restrict__ pre,
+                                                        const Row64* __restrict__ cA, Row64* __restrict__ next,
+                                                        double* __restrict__ out) {
+  extern __shared__ Row64 xs64[];
+  const int tile = blockIdx.x;
+  const int u0 = tileOff[tile], nU = tileOff[tile + 1] - u0;
+  const int slice = tile * kTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const bool live = slice < nSlices;
+  constexpr int S = kWave / G;
+  const int sl = live ? slice : nSlices - 1;
+  const int slot = sl * S + lane / G;
+  const int row = perm ? perm[slot] : slot;
+  const int width = uw > 0 ? uw : sliceW[sl];
+  const uint32_t off = (uint32_t)(uw > 0 ? (int64_t)sl * uw * kWave : sliceOff[sl]) + lane;
+  double v[kChunk64];
+  int li[kChunk64];
+  load_chunk(val, loc, off, 0, width, v, li);
+  stage_tile<Row64, SP>(xs64, cur, u0, nU, tileBase[tile], tileCol);
+  __syncthreads();
+  if (!live) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int c0 = 0;;) {
+#pragma unroll
+    for (int u = 0; u < kChunk64; u++) {
+      const Row64 x = xs64[li[u]];
+      acc[0] += v[u] * x.x;
+      acc[1] += v[u] * x.y;
+      acc[2] += v[u] * x.z;
+    }
+    c0 += kChunk64;
+    if (c0 >= width) break;
+    load_chunk(val, loc, off, c0, width, v, li);
+  }
+  group_reduce<G>(acc);
+  if ((lane % G) != 0 || row >= n) return;
+  const Row64 p = ld32(pre, (uint32_t)row), c = ld32(cA, (uint32_t)row);
+  double x0 = W * (acc[0] + c.x - p.x) + p.x;
+  double x1 = W * (acc[1] + c.y - p.y) + p.y;
+  double x2 = W * (acc[2] + c.z - p.z) + p.z;
+  next[row] = Row64{x0, x1, x2, 0.0};
+  if (out) {
+    out[row] = x0;
+    out[ld + row] = x1;
+    out[2 * ld + row] = x2;
+  }
+}
+
+// rows of the staged union per lane: 2 or 4 (the union holds at most kStageMaxCols = 4 * kBlock)
+static_assert(kStageMaxCols <= 4 * kBlock, "staged unions are loaded at most four rows per lane");
+inline int stage_per_lane(int stageMax) { return stageMax <= 2 * kBlock ? 2 : 4; }
+
+template <int G, int LAST>
+void launch_sweep32s(const DeviceMesh& m, float W, const float4* cur, const float4* pre, float4* next, hipStream_t s) {
+  const int tiles = (m.nSlices + kTileSlices - 1) / kTileSlices;
+  const size_t lds = (size_t)m.stageMax * sizeof(float4);
+#define SW32(SP)                                                                                                  \
+  launch_t(m.timer, TC_SWEEP32, k_sweep32s<G, LAST, SP>, dim3(tiles), dim3(kBlock), (uint32_t)lds, s, m.nSim,      \
+           m.nSlices, m.ld, W,                                                                                       \
+                     m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2ValF, m.tileOff, m.tileBase,      \
+                     m.tileCol, cur, pre, m.r0, next, m.X)
+  if (stage_per_lane(m.stageMax) == 2) SW32(2);
+  else SW32(4);
+#undef SW32
+}
+
+template <int G>
+void launch_sweep64s(const DeviceMesh& m, double W, const Row64* cur, const Row64* pre, Row64* next, double* out,
+                     hipStream_t s) {
+  const int tiles = (m.nSlices + kTileSlices - 1) / kTileSlices;
+  const size_t lds = (size_t)m.stageMax * sizeof(Row64);
+#define SW64(SP)                                                                                                  \
+  launch_t(m.timer, TC_SWEEP64, k_sweep64s<G, SP>, dim3(tiles), dim3(kBlock), (uint32_t)lds, s, m.nSim, m.nSlices, \
+           m.ld, W, m.sliceOff,                                                                                      \
+                     m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.tileOff, m.tileBase, m.tileCol, cur,   \
+                     pre, m.cA, next, out)
+  if (stage_per_lane(m.stageMax) == 2) SW64(2);
+  else SW64(4);
+#undef SW64
+}
+
 __global__ void k_apply_delta(int n, int ld, const float4* __restrict__ d, double* __restrict__ X) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
@@ -357,12 +556,15 @@ __global__ void k_apply_delta(int n, int ld, const float4* __restrict__ d, doubl
 template <int G, int LAST>
 void launch_sweep32(const DeviceMesh& m, int grid, float W, const float4* cur, const float4* pre, float4* next,
                     hipStream_t s) {
-  if (m.m2Col16)
-    hipLaunchKernelGGL((k_sweep32<G, LAST, true>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W,
-                       m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
-  else
-    hipLaunchKernelGGL((k_sweep32<G, LAST, false>), dim3(grid), dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W,
-                       m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
+  if constexpr (G > 1) {  // one lane per row is never staged (its tiles span 256 rows)
+    if (m.m2Loc) {
+      launch_sweep32s<G, LAST>(m, W, cur, pre, next, s);
+      return;
+    }
+  }
+  launch_t(m.timer, TC_SWEEP32, m.m2Col16 ? k_sweep32<G, LAST, true> : k_sweep32<G, LAST, false>, dim3(grid),
+           dim3(kBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col,
+           m.m2Col16, m.m2ValF, cur, pre, m.r0, next, m.X);
 }
 
 // The FP64 solve: x_0 = X, iterates rotate through xa0..xa2, the last one is
@@ -376,9 +578,9 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   double r = rho * rho;
   double W = 2.0 / (2.0 - r);
   const bool c16 = m.m2Col16 != nullptr;
-  hipLaunchKernelGGL((c16 ? k_first64<G, true> : k_first64<G, false>), dim3(grid), dim3(kBlock), 0, s, m.nSim,
-                     m.nSlices, m.ld, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2Val, m.X, m.cA,
-                     m.xa0, m.xa1, outs ? outs[1] : nullptr, ft);
+  launch_t(m.timer, TC_FIRST, c16 ? k_first64<G, true> : k_first64<G, false>, dim3(grid), dim3(kBlock), 0, s, m.nSim,
+           m.nSlices, m.ld, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2Val, m.X, m.cA,
+           m.xa0, m.xa1, outs ? outs[1] : nullptr, ft);
   Row64 *pre = m.xa0, *cur = m.xa1, *next = m.xa2;
   if (iters <= 1) {
     hipLaunchKernelGGL(k_row64_to_soa, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, cur,
@@ -387,9 +589,17 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   }
   for (int it = 1; it < iters; it++) {
     double* out = outs ? outs[it + 1] : (it == iters - 1 ? m.X : nullptr);
-    hipLaunchKernelGGL((c16 ? k_sweep64<G, true> : k_sweep64<G, false>), dim3(grid), dim3(kBlock), 0, s, m.nSim,
-                       m.nSlices, m.ld, W, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2Val, cur,
-                       pre, m.cA, next, out);
+    bool staged = false;
+    if constexpr (G > 1) {
+      if (m.m2Loc) {
+        launch_sweep64s<G>(m, W, cur, pre, next, out, s);
+        staged = true;
+      }
+    }
+    if (!staged)
+      launch_t(m.timer, TC_SWEEP64, c16 ? k_sweep64<G, true> : k_sweep64<G, false>, dim3(grid), dim3(kBlock), 0, s,
+               m.nSim, m.nSlices, m.ld, W, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16,
+               m.m2Val, cur, pre, m.cA, next, out);
     W = 4.0 / (4.0 - r * W);
     Row64* t = pre; pre = cur; cur = next; next = t;
   }
@@ -412,10 +622,10 @@ void solve_impl(const DeviceMesh& m, double w, double rho, int iters, int precis
   constexpr int GR = 4;  // lanes per R1 row
   const int gridR = (m.nSim * GR + kBlock - 1) / kBlock;
   if (!residualReady)  // else the local step's k_rhs wrote v and pt
-    hipLaunchKernelGGL((k_res_a<GR>), dim3(gridR), dim3(kBlock), 0, s, m.nSim, m.ld, w, m.r1Off, m.r1Col, m.r1Val,
-                       m.sysDiag, m.X, m.sysRhs, m.xa0, m.cA);
-  hipLaunchKernelGGL((k_res_b<GR>), dim3(gridR), dim3(kBlock), 0, s, m.nSim, w, m.r1Off, m.r1Col, m.r1Val, m.sysDiag,
-                     m.xa0, m.cA, pre, cur, m.r0);
+    launch_t(m.timer, TC_FIRST, k_res_a<GR>, dim3(gridR), dim3(kBlock), 0, s, m.nSim, m.ld, w, m.r1Off, m.r1Col,
+             m.r1Val, m.sysDiag, m.X, m.sysRhs, m.xa0, m.cA);
+  launch_t(m.timer, TC_FIRST, k_res_b<GR>, dim3(gridR), dim3(kBlock), 0, s, m.nSim, w, m.r1Off, m.r1Col, m.r1Val,
+           m.sysDiag, m.xa0, m.cA, pre, cur, m.r0);
   if (iters <= 1) {
     hipLaunchKernelGGL(k_apply_delta, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, cur,
                        m.X);
@@ -693,6 +903,12 @@ template <int G>
 void one_sweep_impl(const DeviceMesh& m, int precision, hipStream_t s) {
   const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
   const double W = 1.9;
+  if constexpr (G > 1) {
+    if (precision == 0 && m.m2Loc) {
+      launch_sweep64s<G>(m, W, m.xa1, m.xa0, m.xa2, nullptr, s);
+      return;
+    }
+  }
   if (precision == 0)
     hipLaunchKernelGGL((m.m2Col16 ? k_sweep64<G, true> : k_sweep64<G, false>), dim3(grid), dim3(kBlock), 0, s,
                        m.nSim, m.nSlices, m.ld, W, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2Val,
@@ -715,10 +931,10 @@ void launch_one_sweep(const DeviceMesh& m, int precision, hipStream_t s) {
 void launch_operators(const DeviceMesh& m, double omega, int precision, hipStream_t s) {
   (void)precision;  // both the FP64 and the FP32 copies are kept current
   int64_t nE = m.sellEntries;
-  hipLaunchKernelGGL(k_dirty_rows, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.r1Off, m.r1Col,
-                     m.dChanged, m.dDirty);
-  hipLaunchKernelGGL(k_operators, dim3((unsigned)((nE + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nE, omega,
-                     m.m2RowFlag, m.dDirty, m.m2R1, m.p2Off, m.p2K, m.p2A, m.sysDiag, m.m2Val, m.m2ValF);
+  launch_t(m.timer, TC_OPERATORS, k_dirty_rows, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim,
+           m.r1Off, m.r1Col, m.dChanged, m.dDirty);
+  launch_t(m.timer, TC_OPERATORS, k_operators, dim3((unsigned)((nE + kBlock - 1) / kBlock)), dim3(kBlock), 0, s, nE,
+           omega, m.m2RowFlag, m.dDirty, m.m2R1, m.p2Off, m.p2K, m.p2A, m.sysDiag, m.m2Val, m.m2ValF);
 }
 
 void launch_solve_taped(const DeviceMesh& m, double omega, double rho, int iters, double* const* outs, hipStream_t s) {

@@ -133,6 +133,10 @@ int upload_all(pdipc_handle* h) {
   }
   UP(p2Off, L.pOff); UP(p2K, L.pK); UP(p2A, L.pA);
   AL(m2Val, m.sellEntries); AL(m2ValF, m.sellEntries);
+  if (L.stageOk && h->prm.sell_stage) {
+    UP(tileOff, L.tileOff); UP(tileBase, L.tileBase); UP(tileCol, L.tileCol); UP(m2Loc, L.loc);
+    m.stageMax = L.stageMax;
+  }
   // collision
   m.nFaces = (int)hs.bFaces.size() / 3;
   m.nEdges = (int)hs.bEdges.size() / 2;
@@ -343,7 +347,7 @@ void enqueue_local_global(pdipc_handle* h, int withCol) {
 // arguments are identical every outer iteration (the solver's buffer rotation
 // is deterministic), so with use_graphs it is captured once and replayed.
 int local_global(pdipc_handle* h, int withCol) {
-  if (!h->prm.use_graphs) {
+  if (!h->prm.use_graphs || h->m.timer) {  // kernel timing times each launch: no graph replay
     enqueue_local_global(h, withCol);
     return check_launch(h);
   }
@@ -382,7 +386,10 @@ int advance(pdipc_handle* h) {
     h->cnt.lg_iters += p.lg_iters;
     launch_direction(m, s);
     if (outer == 0) {
+      hipEvent_t b0 = nullptr, b1 = nullptr;
+      if (m.timer && m.timer->take(TC_BROAD, &b0, &b1)) HIPCHK(hipEventRecord(b0, s));
       if ((rc = broad_phase(h, m.Xold, m.Dir))) return rc;
+      if (b1) HIPCHK(hipEventRecord(b1, s));
     }
     launch_toi_init(m, s);
     launch_ccd(m, h->lastRecs, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap, p.touch_policy, s);
@@ -471,7 +478,7 @@ void pdipc_default_params(pdipc_params* o) {
   o->omega = 2.0 / 3.0; o->rho = 0.9992; o->sigma_lo = 0.999; o->sigma_hi = 1.001;
   o->ccd_filter = 5e-3; o->toi_scale = 0.98; o->ccd_iters = 10; o->ccd_step = 0.9; o->ccd_gap = 0.1;
   o->static_mass = 1e4; o->inflation = -1.0; o->skip_static_pairs = 0; o->touch_policy = 1;
-  o->precision = PDIPC_FP64; o->sell_lanes = 8; o->use_graphs = 0;
+  o->precision = PDIPC_FP64; o->sell_lanes = 8; o->use_graphs = 0; o->sell_stage = 1;
 }
 
 const char* pdipc_last_error(void) { return pdipc::g_err.c_str(); }
@@ -604,12 +611,51 @@ int pdipc_init(pdipc_handle* h) {
   return PDIPC_OK;
 }
 
+// adds the recorded launches' durations to the timer's sums
+static int fold_timer(pdipc_handle* h) {
+  pdipc::KernelTimer* t = h->m.timer;
+  if (!t || !t->used) return PDIPC_OK;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  for (size_t i = 0; i < t->used; i++) {
+    float ms = 0.0f;
+    HIPCHK(hipEventElapsedTime(&ms, t->pool[2 * i], t->pool[2 * i + 1]));
+    t->us[t->cls[i]] += 1000.0 * ms;
+    t->count[t->cls[i]]++;
+  }
+  t->used = 0;
+  return PDIPC_OK;
+}
+
 int pdipc_step(pdipc_handle* h, int nsteps) {
   int rc = require_device(h);
   if (rc) return rc;
-  for (int i = 0; i < nsteps; i++)
+  for (int i = 0; i < nsteps; i++) {
     if ((rc = advance(h))) return rc;
+    if ((rc = fold_timer(h))) return rc;
+  }
   HIPCHK(hipStreamSynchronize(h->stream));
+  return PDIPC_OK;
+}
+
+int pdipc_kernel_timing(pdipc_handle* h, int enable) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  HIPCHK(hipStreamSynchronize(h->stream));
+  delete h->m.timer;  // a new timer starts from zero
+  h->m.timer = nullptr;
+  if (enable) h->m.timer = new pdipc::KernelTimer();
+  return PDIPC_OK;
+}
+
+int pdipc_get_kernel_times(pdipc_handle* h, double* us, long long* counts, int nclasses) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (!us || !counts || nclasses < 0) return fail(PDIPC_E_ARG, "null output");
+  if ((rc = fold_timer(h))) return rc;
+  for (int c = 0; c < nclasses; c++) {
+    us[c] = (h->m.timer && c < pdipc::TC_COUNT) ? h->m.timer->us[c] : 0.0;
+    counts[c] = (h->m.timer && c < pdipc::TC_COUNT) ? h->m.timer->count[c] : 0;
+  }
   return PDIPC_OK;
 }
 
@@ -789,6 +835,29 @@ int pdipc_debug_sell_layout(pdipc_handle* h, int lanes, long long* sizes, long l
   if (row_flag) std::copy(L.rowFlag.begin(), L.rowFlag.begin() + nE, row_flag);
   if (col16 && L.col16ok) std::copy(L.col16.begin(), L.col16.begin() + nE, col16);
   if (perm) std::copy(L.perm.begin(), L.perm.end(), perm);
+  return PDIPC_OK;
+}
+
+int pdipc_debug_stage_layout(pdipc_handle* h, int lanes, long long* sizes, int* tile_off, int* tile_base,
+                             unsigned short* tile_col, unsigned short* loc) {
+  if (!h) return fail(PDIPC_E_ARG, "null handle");
+  if (!h->hostReady) return fail(PDIPC_E_ARG, "pdipc_host_setup has not run");
+  if (lanes != 1 && lanes != 2 && lanes != 4 && lanes != 8) return fail(PDIPC_E_ARG, "lanes must be 1, 2, 4 or 8");
+  if (!sizes) return fail(PDIPC_E_ARG, "null sizes");
+  SellLayout L;
+  build_sell(h->hs, lanes, L);
+  const int nTiles = L.stageOk ? (int)L.tileBase.size() : 0;
+  sizes[0] = L.stageOk ? 1 : 0;
+  sizes[1] = nTiles;
+  sizes[2] = L.stageOk ? (long long)L.tileCol.size() : 0;
+  sizes[3] = L.stageMax;
+  sizes[4] = L.sliceOff.back();
+  sizes[5] = kTileSlices;
+  if (!L.stageOk) return PDIPC_OK;
+  if (tile_off) std::copy(L.tileOff.begin(), L.tileOff.end(), tile_off);
+  if (tile_base) std::copy(L.tileBase.begin(), L.tileBase.end(), tile_base);
+  if (tile_col) std::copy(L.tileCol.begin(), L.tileCol.end(), tile_col);
+  if (loc) std::copy(L.loc.begin(), L.loc.end(), loc);
   return PDIPC_OK;
 }
 

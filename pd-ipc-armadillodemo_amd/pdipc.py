@@ -28,7 +28,8 @@ class Params(ctypes.Structure):
                 ("ccd_filter", ctypes.c_double), ("toi_scale", ctypes.c_double), ("ccd_iters", ctypes.c_int),
                 ("ccd_step", ctypes.c_double), ("ccd_gap", ctypes.c_double), ("static_mass", ctypes.c_double),
                 ("inflation", ctypes.c_double), ("skip_static_pairs", ctypes.c_int), ("touch_policy", ctypes.c_int),
-                ("precision", ctypes.c_int), ("sell_lanes", ctypes.c_int), ("use_graphs", ctypes.c_int)]
+                ("precision", ctypes.c_int), ("sell_lanes", ctypes.c_int), ("use_graphs", ctypes.c_int),
+                ("sell_stage", ctypes.c_int)]
 
 
 class Counters(ctypes.Structure):
@@ -75,11 +76,15 @@ SIGNATURES = {
     "pdipc_get_model_dat": (ctypes.c_int, [_H, ctypes.c_int, ctypes.c_int, _I, ctypes.c_int, _I]),
     "pdipc_debug_broad_phase": (ctypes.c_int, [_H, _D, _D, _I]),
     "pdipc_debug_toi": (ctypes.c_int, [_H, _D, _D, _D]),
+    "pdipc_kernel_timing": (ctypes.c_int, [_H, ctypes.c_int]),
+    "pdipc_get_kernel_times": (ctypes.c_int, [_H, _D, ctypes.POINTER(ctypes.c_longlong), ctypes.c_int]),
     "pdipc_toi_noalloc": (ctypes.c_int, [_H, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]),
     "pdipc_debug_collision": (ctypes.c_int, [_H, _D, _D, _D, _I]),
     "pdipc_debug_local": (ctypes.c_int, [_H, _D, _D, _D]),
     "pdipc_debug_solve": (ctypes.c_int, [_H, _D, _D, _D, _D]),
     "pdipc_debug_lbvh": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_ulonglong), _I, _I, _I, _I]),
+    "pdipc_debug_stage_layout": (ctypes.c_int, [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _I, _I,
+                                                ctypes.POINTER(ctypes.c_ushort), ctypes.POINTER(ctypes.c_ushort)]),
     "pdipc_debug_sell_layout": (ctypes.c_int, [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
                                                ctypes.POINTER(ctypes.c_longlong), _I, _I, _I,
                                                ctypes.POINTER(ctypes.c_short), _I]),
@@ -95,6 +100,9 @@ SIGNATURES = {
 }
 
 KERNEL_IDS = {"sweep_fp64": 0, "sweep_fp32": 1, "local": 2, "ccd": 3, "operators": 4}
+# classes of pdipc_get_kernel_times, in order
+TIMER_CLASSES = ["sweep_fp64", "sweep_fp32", "rhs", "project", "first_residual", "operators", "ccd", "barrier",
+                 "broad_phase"]
 
 _lib = None
 
@@ -314,6 +322,25 @@ class Simulator:
                     col16=col16[:ne] if c16 else None, perm=perm[:nperm] if nperm else None,
                     uniform_w=uw, rows_per_slice=rows)
 
+    def stage_layout(self, lanes=8):
+        """The sweeps' LDS staging tables for `lanes` lanes per row (host
+        precompute only), or None when staging does not apply."""
+        sz = np.zeros(6, np.int64)
+        szp = sz.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+        _check(lib().pdipc_debug_stage_layout(self.h, lanes, szp, None, None, None, None))
+        ok, nt, nu, umax, ne, ts = (int(v) for v in sz)
+        if not ok:
+            return None
+        off = np.zeros(nt + 1, np.int32)
+        base = np.zeros(nt, np.int32)
+        tcol = np.zeros(max(nu, 1), np.uint16)
+        loc = np.zeros(max(ne, 1), np.uint16)
+        us = ctypes.POINTER(ctypes.c_ushort)
+        _check(lib().pdipc_debug_stage_layout(self.h, lanes, szp, _ip(off), _ip(base), tcol.ctypes.data_as(us),
+                                              loc.ctypes.data_as(us)))
+        return dict(tile_off=off, tile_base=base, tile_col=tcol[:nu], loc=loc[:ne], stage_max=umax,
+                    tile_slices=ts)
+
     def model_dat(self, model, which):
         n = ctypes.c_int()
         _check(lib().pdipc_get_model_dat(self.h, model, which, None, 0, ctypes.byref(n)))
@@ -364,6 +391,19 @@ class Simulator:
         out = np.zeros((self.sim, 3))
         _check(lib().pdipc_debug_solve(self.h, _dp(d), _dp(r), _dp(a), _dp(out)))
         return out
+
+    def kernel_timing(self, enable=True):
+        """In-step kernel timing on (direct launches, each timed kernel's own
+        start/stop recorded) or off; turning it on again starts from zero."""
+        _check(lib().pdipc_kernel_timing(self.h, 1 if enable else 0))
+
+    def kernel_times(self):
+        """{class: (summed microseconds, launches)} since timing was enabled."""
+        n = len(TIMER_CLASSES)
+        us = np.zeros(n)
+        cnt = np.zeros(n, np.int64)
+        _check(lib().pdipc_get_kernel_times(self.h, _dp(us), cnt.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), n))
+        return {name: (float(us[i]), int(cnt[i])) for i, name in enumerate(TIMER_CLASSES)}
 
     def time_kernel(self, name, reps=50):
         """(mean microseconds per launch, algorithmic bytes per launch) of one kernel."""

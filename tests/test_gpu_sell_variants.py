@@ -91,3 +91,40 @@ def test_reordered_rows_tet_block(lanes):
         print("worst step error", worst, flush=True)
         if prec == pdipc.FP32_SPLIT:
             assert worst < 1e-5
+
+
+@pytest.mark.parametrize("which,lanes", [("cloth", 8), ("cloth", 4), ("cloth", 2), ("tets", 8), ("tets", 4)])
+def test_staged_sweeps_are_bit_identical(which, lanes):
+    """The sweeps that stage their operand in LDS (the default whenever the
+    tiles fit) sum every row's entries in the same order as the L2-gather
+    sweeps: whole steps agree bit for bit in both precisions, and the staged
+    FP64 step meets the oracle's bar."""
+    if which == "cloth":
+        sc = scenes.cloth_over_sphere(24, 2, ground=True, seed=2, patch_dim=24, sphere_patch_dim=12, height=0.16)
+    else:
+        sc = scenes.config2(cells=4, cloth_n=8)
+    ref = oracle.Oracle(sc)
+    for _ in range(6):  # into contact
+        ref.step(1)
+    probe = pdipc.Simulator.from_scene(sc, init=False, sell_lanes=lanes)
+    probe.host_setup()
+    assert probe.stage_layout(lanes) is not None
+    for prec in (pdipc.FP64, pdipc.FP32_SPLIT):
+        print(f"case: staged vs L2-gather sweeps, {which}, {lanes} lanes, precision {prec}", flush=True)
+        sims = [pdipc.Simulator.from_scene(sc, sell_lanes=lanes, precision=prec, sell_stage=st) for st in (1, 0)]
+        r = oracle.Oracle(sc)
+        r.set_x(ref.x())
+        r.set_v(ref.v())
+        for k in range(4):
+            x, v = r.x(), r.v()
+            for sim in sims:
+                sim.set_x(x)
+                sim.set_v(v)
+                sim.step(1)
+            r.step(1)
+            assert np.array_equal(sims[0].x(), sims[1].x()), (k, rel(sims[0].x(), sims[1].x()))
+            assert np.array_equal(sims[0].v(), sims[1].v())
+            if prec == pdipc.FP64:
+                check_step(sc, x, v, sims[0].x(), r.x(), r.sim, seed=k)
+            else:
+                assert rel(sims[0].x()[: r.sim], r.x()[: r.sim]) < 1e-5

@@ -26,9 +26,10 @@ MUTANTS = {
 
 
 @pytest.fixture(scope="module")
-def mutant_libs():
-    subprocess.run(["make", "-s", "-j8", "-C", ORACLE, "mutants"], check=True)
-    return {k: os.path.join(ORACLE, "_build", f"liboracle_mut{k}.so") for k in MUTANTS}
+def mutant_libs(tmp_path_factory):
+    out = str(tmp_path_factory.mktemp("oracle_mutants"))  # outside the repository
+    subprocess.run(["make", "-s", "-j8", "-C", ORACLE, "mutants", f"MUTDIR={out}"], check=True)
+    return {k: os.path.join(out, f"liboracle_mut{k}.so") for k in MUTANTS}
 
 
 def _run_closed_forms(lib, kexpr):

@@ -94,3 +94,35 @@ def test_sell_layout_holds_every_entry_once(which, lanes):
 
     if L["col16"] is not None:
         assert np.array_equal(L["col16"].astype(np.int64), col - slot)
+
+
+@pytest.mark.parametrize("which,lanes", [("cloth", 8), ("cloth", 4), ("cloth96", 8), ("cloth96", 2), ("tets", 8),
+                                         ("tets", 4)])
+def test_stage_tables_map_every_entry_to_its_column(which, lanes):
+    """LDS staging: each tile's union is sorted, holds every column of the
+    tile's entries (padding included) and nothing else, fits its limits, and
+    every entry's union index leads back to its column."""
+    sim = pdipc.Simulator.from_scene(_scene(which), init=False)
+    sim.host_setup()
+    L = sim.sell_layout(lanes)
+    T = sim.stage_layout(lanes)
+    assert T is not None
+    ts = T["tile_slices"]
+    off, col = L["slice_off"], L["col"].astype(np.int64)
+    ns = len(L["slice_w"])
+    nt = -(-ns // ts)
+    toff, base, tcol, loc = T["tile_off"], T["tile_base"].astype(np.int64), T["tile_col"].astype(np.int64), T["loc"]
+    assert len(toff) == nt + 1 and toff[0] == 0 and len(loc) == len(col)
+    assert np.diff(toff).max() == T["stage_max"] <= 1024
+    for t in range(nt):
+        e0, e1 = off[t * ts], off[min(ns, (t + 1) * ts)]
+        u = base[t] + tcol[toff[t]:toff[t + 1]]
+        assert np.array_equal(u, np.unique(col[e0:e1]))  # sorted, exactly the tile's columns
+        assert tcol[toff[t]] == 0 and u[-1] - u[0] <= 65535
+        assert np.array_equal(u[loc[e0:e1].astype(np.int64)], col[e0:e1])
+
+
+def test_one_lane_per_row_is_not_staged():
+    sim = pdipc.Simulator.from_scene(_scene("cloth96"), init=False)
+    sim.host_setup()
+    assert sim.stage_layout(1) is None
