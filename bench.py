@@ -3,12 +3,20 @@
 Workload: a 256x256-vertex cloth (130,050 triangles) dropped onto a static
 noise-displaced icosphere (subdivision 6, 81,920 triangles) above a ground
 plane; dt = 1/100, 10 local/global passes x 15 Chebyshev sweeps per outer
-iteration, dHat = 1e-3, kappa = 1e3.  A "step" is one full PD-IPC time step
-(prediction, outer CCD iterations, velocity update) with all data resident in
-HBM.  By default 20 warm-up steps and 180 timed steps: steps 21-200 of the
-200-step run from rest.  The FP32-split step is the headline value; the plain
-FP64 step is timed from the same state, and the per-step error of FP32-split
-against FP64 is measured from identical states.
+iteration, dHat = 1e-3, kappa = 1e3, a 200-step run from rest.  A "step" is
+one full PD-IPC time step (prediction, outer CCD iterations, velocity update)
+with all data resident in HBM.
+
+Measured, each precision from the same state:
+  * the timed window: steps W+1 .. W+K of the run (W warm-up steps, K timed;
+    default W = 20, K = 180, i.e. steps 21-200), graphs on -- `value` is the
+    FP32-split steps/s there;
+  * the settled window: 10 steps from step 150, once the cloth rests on the
+    sphere (the outer iterations per step, hence the cost, roughly double);
+  * the per-step error of FP32-split against FP64 from identical states;
+  * in-step kernel durations (HIP events on every launch, over the timed
+    window re-run with direct launches): the roofline of the FP32 sweep uses
+    the survey's algorithmic bytes (8 B per M^2 entry) over that mean.
 
 Prints one JSON line.  --gpus above 1 exits: this build runs on one GPU.
 """
@@ -44,12 +52,57 @@ def log(msg):
 
 
 def timed_steps(sim, steps):
+    """Wall time of `steps` steps (the counter reads between steps are outside
+    the step but inside the window); outer iterations and the candidate
+    records of each step's broad phase."""
     t0 = time.perf_counter()
-    outer = 0
+    outer, recs = 0, []
     for _ in range(steps):
         sim.step(1)
-        outer += sim.counters()["outer_iters"]
-    return time.perf_counter() - t0, outer
+        c = sim.counters()
+        outer += c["outer_iters"]
+        recs.append(c["candidates"])
+    return time.perf_counter() - t0, outer, recs
+
+
+def cloth_ring_slots(scene):
+    """N + 2E of the simulated cloths: the local step's ring slots."""
+    n = 0
+    for c in scene["cloths"]:
+        f = np.asarray(c["f"], np.int64)
+        e = np.sort(np.concatenate([f[:, [0, 1]], f[:, [1, 2]], f[:, [2, 0]]]), axis=1)
+        n += len(c["v"]) + 2 * len(np.unique(e, axis=0))
+    return n
+
+
+def survey_step_bytes(c, ring_slots, outer_per_step, recs_per_step, p_vec, lg=10, cheb=15):
+    """Algorithmic HBM bytes of one step by the survey's formula (FP32 values,
+    int32 indices, p_vec bytes per vector element in the solve and local step,
+    FP64 CCD and barrier): per outer iteration the operator update, lg x
+    (local step + cheb sweeps), CCD and barrier over the step's candidate
+    records K, plus prediction and velocity (4 and 3 vector passes; the broad
+    phase is left out)."""
+    N, nnz1, nnz2, n2p = c["sim_points"], c["nnz_r1"], c["nnz_r2"], c["n_2paths"]
+    D, tot, K = 3 * N, c["total_points"], recs_per_step
+    b_spmv = 8 * nnz2 + 4 * D * p_vec
+    b_local = ring_slots * 20 + N * 4 + 2 * D * p_vec
+    b_ops = nnz1 * 12 + n2p * 16 + nnz2 * 16 + N * p_vec
+    b_ccd = K * (20 + 24 * 8)
+    b_bar = K * (20 + 12 * 8) + 2 * (tot * 8 + 3 * tot * 8)
+    per_outer = b_ops + lg * (b_local + cheb * b_spmv) + b_ccd + b_bar
+    return outer_per_step * per_outer + 7 * D * p_vec, b_spmv
+
+
+def kernel_means(sim, x, v, steps):
+    """Mean in-step duration (us) and launches per timed class over `steps`
+    steps from (x, v), every launch timed with HIP events (direct launches)."""
+    sim.set_x(x)
+    sim.set_v(v)
+    sim.kernel_timing(True)
+    sim.step(steps)
+    t = sim.kernel_times()
+    sim.kernel_timing(False)
+    return {k: {"mean_us": us / n, "launches": n} for k, (us, n) in t.items() if n}
 
 
 def stacked_bench(args):
@@ -123,6 +176,17 @@ def cpu_baseline(scene, x, v, seconds_budget=20.0, label="256x256 cloth"):
     a bounded sample: the full-size scene from the same warmed-up state the GPU
     timing starts from, as many steps as fit the budget, on all the host
     threads this process may use (`value`) and on one thread."""
+    # the oracle built for this host, -O3 -march=native (into a temporary
+    # folder); the prebuilt -O2 portable build if that fails
+    build_note = "-O3 -march=native, built on this host"
+    try:
+        import tempfile
+        ndir = tempfile.mkdtemp(prefix="pdipc_oracle_native_")
+        subprocess.run(["make", "-s", "-C", os.path.join(ROOT, "oracle"), "native", f"NATIVEDIR={ndir}"], check=True,
+                       stdout=subprocess.PIPE, stderr=subprocess.STDOUT, timeout=300)
+        os.environ["ORACLE_LIB"] = os.path.join(ndir, "liboracle.so")
+    except Exception as e:  # noqa: BLE001
+        build_note = f"-O2 portable build (native build failed: {type(e).__name__})"
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # the checker, timed here as the CPU baseline
 
@@ -146,7 +210,7 @@ def cpu_baseline(scene, x, v, seconds_budget=20.0, label="256x256 cloth"):
     n, t, init_s, outer = sample(threads, 0.5 * seconds_budget)
     n1, t1, _, _ = sample(1, seconds_budget)
     oracle.set_threads(0)
-    return dict(value=n / t, unit="steps/s", cores=threads, kind="port",
+    return dict(value=n / t, unit="steps/s", cores=threads, kind="port", build=build_note,
                 single_thread_steps_per_s=n1 / t1, outer_iters_per_step=outer / n,
                 sample=f"FP64 oracle steps of the same scene ({label}) from the GPU timing's warmed-up state: "
                        f"{n} steps on {threads} threads ({t:.1f} s, {outer / n:.1f} outer iterations per step), "
@@ -199,20 +263,56 @@ def adjoint_bench(graphs, fd_dirs, log_every=8):
         log(f"adjoint ({mode}): forward {tf:.3f} s, backward {tb:.3f} s (ratio {tb / tf:.2f}); {stats}")
     (dk, dv0), (dkc, dvc) = grads["recorded"], grads["checkpointed"]
     same = max(float(np.abs(dk - dkc).max() / np.abs(dkc).max()), float(np.abs(dv0 - dvc).max() / np.abs(dvc).max()))
+
+    # plain steps of the same run (bit for bit the rollout's) on a handle that
+    # also reports the record attaining the minimum TOI at every outer iteration
+    sim_fd = pdipc.Simulator.from_scene(sc, use_graphs=graphs, track_toi_argmin=1)
+
+    def plain_forward(k, v):
+        """The loss, and the choices the adjoint freezes, read after every step."""
+        sim_fd.set_cloth_stiffness(0, k)
+        sim_fd.set_x(x0)
+        sim_fd.set_v(v)
+        sig = []
+        for _ in range(steps):
+            sim_fd.step(1)
+            cn = sim_fd.counters()
+            sig.append((cn["outer_iters"], cn["candidates"], cn["active_pairs"], cn["toi_argmins"]))
+        return float(np.sum((sim_fd.x()[: sim_fd.sim] - target) ** 2)), sig
+
+    def switch_of(s0, sp, sm):
+        """Why the +/- eps runs leave the adjoint's frozen choices, or None: a
+        different outer-iteration, candidate-record or active-pair count, or a
+        different record attaining the minimum TOI, at some step."""
+        names = ("outer iterations", "candidate records", "active barrier pairs", "minimum-TOI records")
+        for st, (a, b, c0) in enumerate(zip(sp, sm, s0)):
+            for idx, name in enumerate(names):
+                if not (a[idx] == b[idx] == c0[idx]):
+                    if idx == 3:
+                        return f"step {st + 1}: {name} differ"
+                    return f"step {st + 1}: {name} {b[idx]} / {c0[idx]} / {a[idx]} (-eps / 0 / +eps)"
+        return None
+
+    l0, sig0 = plain_forward(k0, v0)
     rng = np.random.default_rng(12345)
-    errs = []
+    errs, switches, outliers = [], [], []
     eps = 1e-6
     for q in range(fd_dirs):
         dkq = rng.normal(size=ntri) * k0
         dvq = rng.normal(size=v0.shape) * 0.05
-        lp, _, _ = forward(k0 + eps * dkq, v0 + eps * dvq, every)
-        lm, _, _ = forward(k0 - eps * dkq, v0 - eps * dvq, every)
+        lp, sp = plain_forward(k0 + eps * dkq, v0 + eps * dvq)
+        lm, sm = plain_forward(k0 - eps * dkq, v0 - eps * dvq)
         fd = (lp - lm) / (2 * eps)
         ad = float(dk @ dkq + np.sum(dv0 * dvq))
         errs.append(abs(fd - ad) / max(abs(ad), 1e-300))
-        if q % log_every == 0:
-            log(f"  direction {q}: fd {fd:.6e} adjoint {ad:.6e} rel {errs[-1]:.2e}")
+        why = switch_of(sig0, sp, sm)
+        switches.append(why)
+        if errs[-1] > 1e-4:
+            outliers.append({"direction": q, "rel_err": errs[-1], "switch": why})
+        if q % log_every == 0 or errs[-1] > 1e-4:
+            log(f"  direction {q}: fd {fd:.6e} adjoint {ad:.6e} rel {errs[-1]:.2e} switch: {why}")
     errs = np.array(errs)
+    smooth = np.array([w is None for w in switches], bool)
     r, c = res["recorded"], res["checkpointed"]
     return {
         "workload": "cloth 128x128 (32,258 triangles) over icosphere subdiv 4 (config-0 obstacle), 50 FP64 steps, "
@@ -225,6 +325,15 @@ def adjoint_bench(graphs, fd_dirs, log_every=8):
         "fd_rel_err_median": float(np.median(errs)) if fd_dirs else None,
         "fd_rel_err_max": float(errs.max()) if fd_dirs else None,
         "fd_dirs_within_1e-3": int(np.sum(errs < 1e-3)) if fd_dirs else None,
+        "fd_non_switching": {"directions": int(smooth.sum()),
+                             "rel_err_max": float(errs[smooth].max()) if smooth.any() else None,
+                             "rel_err_median": float(np.median(errs[smooth])) if smooth.any() else None},
+        "fd_switching": {"directions": int((~smooth).sum()),
+                         "rel_err_max": float(errs[~smooth].max()) if (~smooth).any() else None},
+        "fd_outliers_above_1e-4": outliers,
+        "fd_switch_rule": "a direction switches when its -eps, 0 and +eps runs differ at some step in outer "
+                          "iterations, candidate records, active barrier pairs or the record attaining the minimum "
+                          "TOI of an outer iteration (track_toi_argmin)",
     }
 
 
@@ -234,6 +343,12 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (default: the rest of the scene's run after the warm-up: 200 steps in all "
                          "for the 256^2 cloth, 100 for config0 and tet_block, 50 for stacked)")
+    ap.add_argument("--settle-step", type=int, default=None,
+                    help="step from which the settled window is timed (default: 150 of the 200-step cloth run, "
+                         "75 of the 100-step runs; 0: no settled window)")
+    ap.add_argument("--settle-steps", type=int, default=10, help="steps of the settled window")
+    ap.add_argument("--replay", action="store_true",
+                    help="also time 200 back-to-back replays of each sweep (pdipc_time_kernel)")
     ap.add_argument("--warmup", type=int, default=20)
     ap.add_argument("--scene", choices=("cloth", "config0", "tet_block", "stacked"), default="cloth",
                     help="cloth: the n x n cloth over the sphere (default); config0: the 64x64 cloth over the "
@@ -315,29 +430,71 @@ def main():
         sim = sims[name]
         sim.set_x(x_w)
         sim.set_v(v_w)
-        secs, outer = timed_steps(sim, args.steps)
-        res[name] = dict(seconds=secs, outer=outer)
-        log(f"{name}: {args.steps} steps in {secs:.3f} s ({args.steps / secs:.2f} steps/s, {outer} outer iterations)")
+        secs, outer, recs = timed_steps(sim, args.steps)
+        res[name] = dict(seconds=secs, outer=outer, recs=recs)
+        log(f"{name}: steps {args.warmup + 1}-{args.warmup + args.steps} in {secs:.3f} s "
+            f"({args.steps / secs:.2f} steps/s, {outer} outer iterations)")
 
     # per-step error of FP32-split vs FP64 from identical states (teacher forced)
-    errs = []
-    a, b = sims["fp64"], sims["fp32_split"]
-    for k in range(5):
-        x, v = a.x(), a.v()
-        b.set_x(x)
-        b.set_v(v)
-        a.step(1)
-        b.step(1)
-        xa, xb = a.x()[: a.sim], b.x()[: b.sim]
-        errs.append(float(np.abs(xa - xb).max() / np.abs(xa).max()))
+    def step_errors(n):
+        errs = []
+        a, b = sims["fp64"], sims["fp32_split"]
+        for _ in range(n):
+            x, v = a.x(), a.v()
+            b.set_x(x)
+            b.set_v(v)
+            a.step(1)
+            b.step(1)
+            xa, xb = a.x()[: a.sim], b.x()[: b.sim]
+            errs.append(float(np.abs(xa - xb).max() / np.abs(xa).max()))
+        return errs
+
+    errs = step_errors(5)
     log(f"FP32-split per-step error vs FP64: {errs}")
 
-    # dominant kernel: the Chebyshev sweep (150 launches per outer iteration)
-    us64, bytes64 = a.time_kernel("sweep_fp64", 200)
-    us32, bytes32 = b.time_kernel("sweep_fp32", 200)
-    c = a.counters()
-    ach = bytes32 / (us32 * 1e-6) / 1e9
-    log(f"sweep fp64 {us64:.2f} us ({bytes64 / (us64 * 1e-6) / 1e9:.0f} GB/s), fp32 {us32:.2f} us ({ach:.0f} GB/s)")
+    # in-step kernel durations: the timed window again, every launch timed
+    kt = {name: kernel_means(sims[name], x_w, v_w, args.steps) for name in ("fp64", "fp32_split")}
+    log(f"in-step kernel means: {json.dumps(kt)}")
+
+    # settled window: the FP64 run continued to the settle step, both precisions from there
+    settle_step = args.settle_step if args.settle_step is not None else (150 if run_steps >= 200 else 75)
+    settled = None
+    if settle_step > 0:
+        a = sims["fp64"]
+        a.set_x(x_w)
+        a.set_v(v_w)
+        a.step(max(0, settle_step - args.warmup))  # the run from the warm-up state to the settle step
+        x_s, v_s = a.x(), a.v()
+        st = {}
+        for name in ("fp64", "fp32_split"):
+            sim = sims[name]
+            sim.set_x(x_s)
+            sim.set_v(v_s)
+            secs, outer, recs = timed_steps(sim, args.settle_steps)
+            st[name] = dict(seconds=secs, outer=outer, recs=recs)
+            log(f"settled {name}: steps {settle_step + 1}-{settle_step + args.settle_steps} in {secs:.3f} s "
+                f"({args.settle_steps / secs:.2f} steps/s, {outer} outer iterations)")
+        a.set_x(x_s)
+        a.set_v(v_s)
+        serr = step_errors(3)
+        settled = {
+            "timed_steps": f"steps {settle_step + 1}-{settle_step + args.settle_steps} of the same run from rest "
+                           f"(the FP64 run continued to step {settle_step}), each precision from that state, graphs "
+                           f"{'on' if args.graphs else 'off'}",
+            "fp32_split_steps_per_s": args.settle_steps / st["fp32_split"]["seconds"],
+            "fp64_steps_per_s": args.settle_steps / st["fp64"]["seconds"],
+            "fp32_split_speedup_vs_fp64": st["fp64"]["seconds"] / st["fp32_split"]["seconds"],
+            "outer_iters_per_step": st["fp32_split"]["outer"] / args.settle_steps,
+            "fp32_split_step_error_vs_fp64": max(serr),
+        }
+
+    c = sims["fp64"].counters()
+    # optional: back-to-back replays of one sweep (caches warm on one input)
+    replay = None
+    if args.replay:
+        us64, bytes64 = sims["fp64"].time_kernel("sweep_fp64", 200)
+        us32, bytes32 = sims["fp32_split"].time_kernel("sweep_fp32", 200)
+        replay = {"sweep_fp64_us": us64, "sweep_fp32_us": us32, "note": "200 back-to-back launches, HIP events"}
 
     adj = None
     if not args.no_adjoint:
@@ -352,6 +509,15 @@ def main():
 
     steps = args.steps
     t32 = res["fp32_split"]["seconds"]
+    outer32 = res["fp32_split"]["outer"] / steps
+    recs32 = float(np.mean(res["fp32_split"]["recs"]))
+    slots = cloth_ring_slots(scene)
+    step_bytes, spmv_bytes = survey_step_bytes(c, slots, outer32, recs32, 4, scene["params"]["lg_iters"],
+                                               scene["params"]["cheb_iters"])
+    sweep = kt["fp32_split"]["sweep_fp32"]
+    ach = spmv_bytes / (sweep["mean_us"] * 1e-6) / 1e9
+    stored = c["nnz_r2"] * 6.0 + 3 * c["sim_points"] * 4.0 * 4
+    matrix_bytes = c["sell_entries"] * 6.0
     out = {
         "metric": METRIC,
         "value": steps / t32,
@@ -369,31 +535,50 @@ def main():
             "workload": workload or f"cloth {args.n}x{args.n} ({c['cloth_points']} vertices) over icosphere subdiv "
                                     f"{args.subdiv} + ground, dt=1/100, 10 LG x 15 Chebyshev, dHat=1e-3, kappa=1e3",
             "precision": "FP32-split (FP32 local step + sweeps on an FP64 residual; FP64 CCD/barrier)",
-            "timed_steps": f"steps {args.warmup + 1}-{args.warmup + steps} of a run from rest "
-                           f"({RUN_STEPS[args.scene]} steps configured), each precision from the same state",
+            "timed_steps": f"steps {args.warmup + 1}-{args.warmup + steps} of a {run_steps}-step run from rest "
+                           f"({args.warmup} warm-up steps, FP64), each precision from the same state; the steps "
+                           f"after them were not run in this window (see settled)",
             "fp64_steps_per_s": steps / res["fp64"]["seconds"],
             "fp32_split_speedup_vs_fp64": res["fp64"]["seconds"] / t32,
             "fp32_split_step_error_vs_fp64": max(errs),
-            "outer_iters_per_step": res["fp32_split"]["outer"] / steps,
-            "nnz_r2": c["nnz_r2"], "candidates": c["candidates"], "patch_pairs": c["patch_pairs"],
+            "outer_iters_per_step": outer32,
+            "settled": settled,
+            "algorithmic_bytes_per_step": step_bytes,
+            "algorithmic_GBs": step_bytes * steps / t32 / 1e9,
+            "nnz_r2": c["nnz_r2"], "candidates_per_step": recs32, "patch_pairs": c["patch_pairs"],
             "patches": c["patches"],
             "static_static_pairs": "kept" if args.keep_static_pairs else
                                    "skipped (inert: they only write rows the step never reads)",
             "graphs": bool(args.graphs),
             "parallelism": "single GPU",
+            "kernel_means_us": kt,
             "adjoint": adj,
         },
         "roofline": {
-            "kernel": "k_sweep32 (FP32 Chebyshev sweep, SELL M^2)",
+            "kernel": "k_sweep32s (FP32 Chebyshev sweep, SELL M^2, operand staged in LDS)",
             "bound": "hbm",
             "achieved": ach,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
             "traffic": None if workload else SWEEP32_TRAFFIC_BYTES.get((args.n, args.subdiv, args.sell_lanes)),
-            "avg_us": us32,
-            "bytes_per_launch": bytes32,
-            "fp64_sweep": {"avg_us": us64, "achieved": bytes64 / (us64 * 1e-6) / 1e9},
+            "traffic_source": "profiles/k_sweep_config1_hbm_traffic.md: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
+                              "per launch, scaled by a 1 GiB copy; a recorded constant, not measured in this run",
+            "avg_us": sweep["mean_us"],
+            "avg_us_source": f"mean of {sweep['launches']} in-step launches, HIP events on each launch "
+                             "(pdipc_kernel_timing, the timed window with direct launches)",
+            "bytes_per_launch": spmv_bytes,
+            "bytes_formula": "8 B per M^2 entry (FP32 value + int32 column) + 4 vectors x 3N x 4 B",
+            "bytes_stored_format": stored,
+            "frac_stored_format": stored / (sweep["mean_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS,
+            "mall_resident": matrix_bytes < 256e6,
+            "mall_note": "the M^2 matrix (FP32 values + 16-bit union indices, "
+                         f"{matrix_bytes / 1e6:.1f} MB) fits the 256 MB Infinity Cache: after the first sweep it is "
+                         "served from MALL, so this fraction of HBM peak is not evidence of HBM-bound execution",
+            "fp64_sweep": {"avg_us": kt["fp64"]["sweep_fp64"]["mean_us"],
+                           "achieved": (8 * c["nnz_r2"] + 4 * 3 * c["sim_points"] * 8)
+                           / (kt["fp64"]["sweep_fp64"]["mean_us"] * 1e-6) / 1e9},
+            "replay": replay,
         },
         "cpu_baseline": cpu,
     }

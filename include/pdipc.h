@@ -65,6 +65,11 @@ typedef struct pdipc_params {
   int use_graphs;      /* 1: replay the local/global loop from a HIP graph */
   int sell_stage;      /* 1: the sweeps stage their gathered operand in LDS, per workgroup, when
                           every workgroup's column union fits (default); 0: gather from L2 */
+  int capacity_hint;   /* 0: default initial capacities of the growable broad-phase and contact
+                          buffers (sized from the patch count); > 0: start each at this many
+                          entries (they grow on demand either way) */
+  int track_toi_argmin; /* 1: also find the record attaining the minimum TOI at every outer
+                           iteration (pdipc_counters.toi_argmins; diagnostics, one more pass) */
 } pdipc_params;
 
 typedef struct pdipc_counters {
@@ -80,6 +85,8 @@ typedef struct pdipc_counters {
   long long n_2paths;   /* 2-path terms feeding the R2 operator update */
   long long sell_entries; /* stored M^2 entries including SELL padding */
   int sim_points, total_points, cloth_points, tets, patches, boundary_faces, boundary_edges;
+  long long toi_argmins;  /* with track_toi_argmin: a hash of the record attaining the minimum TOI
+                             at each outer iteration of the last step (0 where none is below 1) */
 } pdipc_counters;
 
 /* --- lifecycle: PdIpcSimulator ctor + readExtraAttributeFromConfigFile (PdIpcSimulator.h:441-450,602-642) */

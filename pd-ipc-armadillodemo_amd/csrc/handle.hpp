@@ -50,6 +50,8 @@ struct pdipc_handle {
   std::vector<double> stage;
   pdipc_counters cnt;
   int lastRecs = 0, lastPairs = 0;
+  double* trackToi = nullptr;  // per-record TOIs when track_toi_argmin is on
+  int trackCap = 0;
 
   template <class T>
   int alloc(T** p, size_t n) {

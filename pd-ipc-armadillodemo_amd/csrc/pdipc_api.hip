@@ -163,23 +163,24 @@ int upload_all(pdipc_handle* h) {
     while ((1 << levels) <= P) levels++;
     AL(nodeTable, (size_t)levels * 6 * P);
   }
+  const int hint = h->prm.capacity_hint;  // > 0: every growable buffer starts at this size
   {
     int nw = traverse_top_waves(P);
-    m.segCapI = 256;
-    m.segCapP = 256;
+    m.segCapI = hint > 0 ? hint : 256;
+    m.segCapP = m.segCapI;
     AL(segI, (size_t)nw * m.segCapI); AL(segP, (size_t)nw * m.segCapP);
     AL(cntI, nw); AL(offI, nw); AL(cntP, nw); AL(offP, nw);
-    m.itemCap = std::max(1024, 2 * P);
+    m.itemCap = hint > 0 ? hint : std::max(1024, 2 * P);
     AL(items, m.itemCap);
     AL(segS, (size_t)m.itemCap * traverse_sub_cap()); AL(cntS, m.itemCap); AL(offS, m.itemCap);
   }
-  m.pairCap = std::max(1024, 8 * P);
+  m.pairCap = hint > 0 ? hint : std::max(1024, 8 * P);
   AL(pairs, m.pairCap); AL(recCount, 2 * (size_t)m.pairCap); AL(recOff, 2 * (size_t)m.pairCap);
-  m.recCap = std::max(4096, 32 * P);
+  m.recCap = hint > 0 ? hint : std::max(4096, 32 * P);
   AL(recs, m.recCap);
   m.scanBlockCap = (std::max(std::max(P, m.pairCap), m.itemCap) + 1023) / 1024 + 2;
   AL(scanBlock, m.scanBlockCap);
-  m.actCap = 4096;
+  m.actCap = hint > 0 ? hint : 4096;
   AL(actRec, m.actCap); AL(actW, m.actCap); AL(actT, 12 * (size_t)m.actCap);
   AL(entKey, 4 * (size_t)m.actCap); AL(entA, 4 * (size_t)m.actCap); AL(entPt, 4 * (size_t)m.actCap);
   AL(entSorted, 4 * (size_t)m.actCap);
@@ -380,6 +381,7 @@ int advance(pdipc_handle* h) {
   bool activeCol = false;
   int outer = 0;
   double toiMin = 1.0;
+  long long argminHash = 0;
   h->cnt.lg_iters = 0;
   for (; outer < p.max_outer; outer++) {
     if ((rc = local_global(h, activeCol ? 1 : 0))) return rc;
@@ -392,9 +394,27 @@ int advance(pdipc_handle* h) {
       if (b1) HIPCHK(hipEventRecord(b1, s));
     }
     launch_toi_init(m, s);
-    launch_ccd(m, h->lastRecs, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap, p.touch_policy, s);
+    const bool track = p.track_toi_argmin && h->lastRecs > 0;
+    if (track) {  // per-record TOIs, then the first record attaining the minimum
+      if (h->trackCap < h->lastRecs) {
+        h->release(h->trackToi);
+        h->trackToi = nullptr;
+        h->trackCap = 0;
+        if ((rc = h->alloc(&h->trackToi, (size_t)h->lastRecs))) return rc;
+        h->trackCap = h->lastRecs;
+      }
+      HIPCHK(hipMemsetAsync(m.counters + 4, 0x7f, sizeof(int), s));
+      launch_ccd_records(m, m.recs, h->lastRecs, m.Xold, m.X, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap,
+                         p.touch_policy, h->trackToi, m.counters + 4, s);
+    } else {
+      launch_ccd(m, h->lastRecs, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap, p.touch_policy, s);
+    }
     launch_clamp_residual(m, p.toi_scale, s);
     if ((rc = collision(h))) return rc;  // also reads the scalars back
+    if (p.track_toi_argmin) {
+      const int am = track ? h->hCounters()[4] : 0x7f7f7f7f;
+      argminHash = argminHash * 1000003LL + (am >= 0 && am < h->lastRecs ? am + 1 : 0);
+    }
     activeCol = true;
     double res = *h->hResidual();
     double tmin;
@@ -410,6 +430,7 @@ int advance(pdipc_handle* h) {
   }
   h->cnt.outer_iters = outer < p.max_outer ? outer + 1 : p.max_outer;
   h->cnt.toi_min = toiMin;
+  h->cnt.toi_argmins = argminHash;
   h->cnt.patch_pairs = h->lastPairs;
   h->cnt.candidates = h->lastRecs;
   launch_velocity(m, p.dt, s);

@@ -29,7 +29,7 @@ class Params(ctypes.Structure):
                 ("ccd_step", ctypes.c_double), ("ccd_gap", ctypes.c_double), ("static_mass", ctypes.c_double),
                 ("inflation", ctypes.c_double), ("skip_static_pairs", ctypes.c_int), ("touch_policy", ctypes.c_int),
                 ("precision", ctypes.c_int), ("sell_lanes", ctypes.c_int), ("use_graphs", ctypes.c_int),
-                ("sell_stage", ctypes.c_int)]
+                ("sell_stage", ctypes.c_int), ("capacity_hint", ctypes.c_int), ("track_toi_argmin", ctypes.c_int)]
 
 
 class Counters(ctypes.Structure):
@@ -38,7 +38,8 @@ class Counters(ctypes.Structure):
                 ("residual", ctypes.c_double), ("nnz_r1", ctypes.c_longlong), ("nnz_r2", ctypes.c_longlong),
                 ("n_2paths", ctypes.c_longlong), ("sell_entries", ctypes.c_longlong), ("sim_points", ctypes.c_int),
                 ("total_points", ctypes.c_int), ("cloth_points", ctypes.c_int), ("tets", ctypes.c_int),
-                ("patches", ctypes.c_int), ("boundary_faces", ctypes.c_int), ("boundary_edges", ctypes.c_int)]
+                ("patches", ctypes.c_int), ("boundary_faces", ctypes.c_int), ("boundary_edges", ctypes.c_int),
+                ("toi_argmins", ctypes.c_longlong)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

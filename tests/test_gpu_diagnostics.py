@@ -1,7 +1,9 @@
-"""In-step kernel timing through the C ABI (pdipc_kernel_timing /
-pdipc_get_kernel_times): timed steps are the plain steps bit for bit, and the
-launch counts follow the step's structure (outer iterations x local/global
-passes x sweeps)."""
+"""Diagnostics through the C ABI.  In-step kernel timing
+(pdipc_kernel_timing / pdipc_get_kernel_times): timed steps are the plain
+steps bit for bit, and the launch counts follow the step's structure (outer
+iterations x local/global passes x sweeps).  Minimum-TOI record tracking
+(track_toi_argmin): the same steps bit for bit, with the record of every
+clamped outer iteration reported."""
 
 import numpy as np
 import pytest
@@ -44,3 +46,21 @@ def test_timed_steps_match_plain_steps_and_count_launches(prec):
             assert 0.0 < us / n < 1e5, (name, us, n)
     b.kernel_timing(False)
     assert b.kernel_times()[sweeps] == (0.0, 0)
+
+
+def test_toi_argmin_tracking_changes_nothing():
+    sc = scenes.cloth_over_sphere(24, 2, ground=True, seed=4, patch_dim=24, sphere_patch_dim=12, height=0.16)
+    print("case: TOI argmin tracking", flush=True)
+    a = pdipc.Simulator.from_scene(sc)
+    b = pdipc.Simulator.from_scene(sc, track_toi_argmin=1)
+    clamped = 0
+    for k in range(12):
+        a.step(1)
+        b.step(1)
+        assert np.array_equal(a.x(), b.x()), k
+        ca, cb = a.counters(), b.counters()
+        assert ca["toi_argmins"] == 0
+        if cb["toi_min"] < 1.0:
+            clamped += 1
+            assert cb["toi_argmins"] != 0, k
+    assert clamped > 0

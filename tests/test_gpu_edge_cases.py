@@ -56,6 +56,7 @@ def _teacher_forced_fp64(sc, steps, label):
         assert c["outer_iters"] == o["outer"], (k, c["outer_iters"], o["outer"])
         assert c["candidates"] == o["candidates"], (k, c["candidates"], o["candidates"])
         cand = max(cand, c["candidates"])
+        assert sim.count_intersections() == 0, k  # no two triangles intersect after the step
     print("worst FP64 step error", worst, "most candidates", cand, flush=True)
     return cand
 

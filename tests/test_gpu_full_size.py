@@ -55,6 +55,8 @@ def test_config1_step_in_contact(config1_contact):
     print("config-1 FP64 step error", err, "oracle sensitivity", sens)
     print("case: config-1 patch-pair sets", flush=True)
     assert np.array_equal(_rows(sim.patch_pairs()), _rows(ref.patch_pairs()))
+    print("case: config-1 zero-intersection check after the contact step", flush=True)
+    assert sim.count_intersections() == 0
 
 
 @pytest.mark.timeout(600)
