@@ -109,6 +109,7 @@ struct DeviceMesh {
   uint8_t *dStale = nullptr, *dChanged = nullptr, *dDirty = nullptr;
   // cloth
   int *cCenter = nullptr, *cOff = nullptr, *cNbr = nullptr, *clothOfPoint = nullptr;
+  int2* ringRange = nullptr;  // per simulated point: its cloth ring slots [x, y) (empty for tet points)
   double *cRest = nullptr, *cLbo = nullptr, *cBendIn = nullptr, *cStrain = nullptr, *cH0 = nullptr;
   // tets
   int4* tetIdx = nullptr;

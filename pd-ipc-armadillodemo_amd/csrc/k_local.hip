@@ -238,10 +238,14 @@ __device__ inline V rhs_lanes_sum(V v) {
   return v;
 }
 
+// ring slots of each lane fetched together (all loads first, then the sums
+// in slot order, so the sums are those of the one-at-a-time loop)
+constexpr int kRingBatch = 2;
+
 template <class T, bool RES>
 __global__ void k_rhs(int nSim, int ld, int nT, int withCol, ResidualArgs ra, const double* __restrict__ initRhs,
                       const int* __restrict__ vtOff, const int* __restrict__ vtList, const double* __restrict__ tetOut,
-                      const int* __restrict__ clothOfPoint, const int* __restrict__ cOff, const int* __restrict__ cNbr,
+                      const int2* __restrict__ ringRange, const int* __restrict__ cNbr,
                       const double* __restrict__ cRest, const double* __restrict__ cStrain,
                       const double* __restrict__ cBendIn, const double* __restrict__ A, const double* __restrict__ X,
                       const double* __restrict__ colRhs, const double* __restrict__ fixK,
@@ -249,8 +253,23 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, ResidualArgs ra, co
   int gt = blockIdx.x * blockDim.x + threadIdx.x;
   int i = gt / kRhsLanes, sub = gt % kRhsLanes;
   bool live = i < nSim;
+  const int ii = live ? i : nSim - 1;
+  // every operand that depends on the point alone, up front
+  const int2 ring = ringRange[ii];  // the point's cloth ring slots (empty for tet points)
+  const int q0 = live ? vtOff[ii] : 0, q1 = live ? vtOff[ii + 1] : 0;
+  const int r0 = RES ? ra.off[ii] : 0, r1 = RES && live ? ra.off[ii + 1] : 0;
+  const double xi[3] = {X[ii], X[ld + ii], X[2 * ld + ii]};
+  double init[3] = {0, 0, 0}, col[3] = {0, 0, 0}, kf = 0.0, di = 1.0;
+  if (sub == 0) {
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      init[k] = initRhs[k * ld + ii];
+      if (withCol) col[k] = colRhs[k * ld + ii];
+    }
+    kf = fixK[ii];
+    if (RES) di = ra.D[ii];
+  }
   double tr[3] = {0, 0, 0};  // tet outputs
-  int q0 = live ? vtOff[i] : 0, q1 = live ? vtOff[i + 1] : 0;
   for (int q = q0 + sub; q < q1; q += kRhsBatch * kRhsLanes) {
     int tc[kRhsBatch];
 #pragma unroll
@@ -267,44 +286,56 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, ResidualArgs ra, co
 #pragma unroll
         for (int k = 0; k < 3; k++) tr[k] += a[b][k];
   }
-  double xi[3] = {0, 0, 0};
-  if (live) {
-    xi[0] = X[i];
-    xi[1] = X[ld + i];
-    xi[2] = X[2 * ld + i];
-  }
-  int c = live ? clothOfPoint[i] : -1;
+  // cloth ring: springs over the neighbour slots, the bending gather over all
+  // slots (the first slot is the point itself), in one pass
   T sp[3] = {0, 0, 0};
   double bd[3] = {0, 0, 0};
-  if (c >= 0) {
-    int o = cOff[c], e = cOff[c + 1];
-    for (int s = o + 1 + sub; s < e; s += kRhsLanes) {
-      int j = cNbr[s];
-      T d[3];
+  const int o = live ? ring.x : 0, e = live ? ring.y : 0;
+  for (int s0 = o + sub; s0 < e; s0 += kRingBatch * kRhsLanes) {
+    int j[kRingBatch];
+    double xj[kRingBatch][3], aj[kRingBatch][3], bw[kRingBatch], rest[kRingBatch], stiff[kRingBatch];
 #pragma unroll
-      for (int k = 0; k < 3; k++) d[k] = (T)(xi[k] - X[k * ld + j]);
-      T len = dsqrt<T>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
-      T rc = (T)cStrain[s] * ((T)cRest[s] / len);
-#pragma unroll
-      for (int k = 0; k < 3; k++) sp[k] += rc * d[k];
+    for (int b = 0; b < kRingBatch; b++) {
+      const int sl = s0 + b * kRhsLanes;
+      const bool in = sl < e;
+      j[b] = in ? cNbr[sl] : ii;
+      bw[b] = in ? cBendIn[sl] : 0.0;
+      rest[b] = in && sl > o ? cRest[sl] : 0.0;
+      stiff[b] = in && sl > o ? cStrain[sl] : 0.0;
     }
-    for (int s = o + sub; s < e; s += kRhsLanes) {
-      int j = cNbr[s];
-      double b = cBendIn[s];
 #pragma unroll
-      for (int k = 0; k < 3; k++) bd[k] += b * A[k * ld + j];
+    for (int b = 0; b < kRingBatch; b++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        xj[b][k] = X[k * ld + j[b]];
+        aj[b][k] = A[k * ld + j[b]];
+      }
+#pragma unroll
+    for (int b = 0; b < kRingBatch; b++) {
+      const int sl = s0 + b * kRhsLanes;
+      if (sl >= e) continue;
+      if (sl > o) {
+        T d[3];
+#pragma unroll
+        for (int k = 0; k < 3; k++) d[k] = (T)(xi[k] - xj[b][k]);
+        T len = dsqrt<T>(d[0] * d[0] + d[1] * d[1] + d[2] * d[2]);
+        T rc = (T)stiff[b] * ((T)rest[b] / len);
+#pragma unroll
+        for (int k = 0; k < 3; k++) sp[k] += rc * d[k];
+      }
+#pragma unroll
+      for (int k = 0; k < 3; k++) bd[k] += bw[b] * aj[b][k];
     }
   }
   double sx[3] = {0, 0, 0};
-  if (RES && live) {
-    const int r1 = ra.off[i + 1];
-    for (int q = ra.off[i] + sub; q < r1; q += kRhsBatch * kRhsLanes) {
+  if (RES) {
+    for (int q = r0 + sub; q < r1; q += kRhsBatch * kRhsLanes) {
       int j[kRhsBatch];
       double a[kRhsBatch], xv[kRhsBatch][3];
 #pragma unroll
       for (int b = 0; b < kRhsBatch; b++) {
         const bool in = q + b * kRhsLanes < r1;
-        j[b] = in ? ra.col[q + b * kRhsLanes] : i;
+        j[b] = in ? ra.col[q + b * kRhsLanes] : ii;
         a[b] = in ? ra.val[q + b * kRhsLanes] : 0.0;
       }
 #pragma unroll
@@ -326,18 +357,13 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, ResidualArgs ra, co
     if (RES) sx[k] = rhs_lanes_sum(sx[k]);
   }
   if (!live || sub != 0) return;
-  double r[3] = {initRhs[i], initRhs[ld + i], initRhs[2 * ld + i]};
+  double r[3];
 #pragma unroll
-  for (int k = 0; k < 3; k++) r[k] += tr[k];
-  if (c >= 0) {
-#pragma unroll
-    for (int k = 0; k < 3; k++) r[k] += (double)sp[k] + bd[k];
+  for (int k = 0; k < 3; k++) {
+    r[k] = init[k] + tr[k];  // the reference's order: rhs0, tets, cloth, collision, positions
+    if (e > o) r[k] += (double)sp[k] + bd[k];
+    if (withCol) r[k] += col[k];
   }
-  if (withCol) {
-#pragma unroll
-    for (int k = 0; k < 3; k++) r[k] += colRhs[k * ld + i];
-  }
-  double kf = fixK[i];
   if (kf != 0.0) {
 #pragma unroll
     for (int k = 0; k < 3; k++) r[k] += kf * Xrest[k * ld + i];
@@ -345,7 +371,7 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, ResidualArgs ra, co
 #pragma unroll
   for (int k = 0; k < 3; k++) rhs[k * ld + i] = r[k];
   if (RES) {
-    double di = ra.D[i], w = ra.w;
+    const double w = ra.w;
     double vv[3], pp[3];
 #pragma unroll
     for (int k = 0; k < 3; k++) {
@@ -370,8 +396,8 @@ void local_impl(const DeviceMesh& m, T lo, T hi, int withCol, double resW, hipSt
   const ResidualArgs ra{resW, m.r1Off, m.r1Col, m.r1Val, m.sysDiag, m.xa0, m.cA};
   const dim3 grid(grid_for((int64_t)m.nSim * kRhsLanes));
   launch_t(m.timer, TC_RHS, resW >= 0.0 ? k_rhs<T, true> : k_rhs<T, false>, grid, dim3(kBlock), 0, s, m.nSim, m.ld,
-           m.nTets, withCol, ra, m.initRhs, m.vtOff, m.vtList, m.tetOut, m.clothOfPoint, m.cOff, m.cNbr, m.cRest,
-           m.cStrain, m.cBendIn, m.clothA, m.X, m.colRhs, m.fixK, m.Xrest, m.sysRhs);
+           m.nTets, withCol, ra, m.initRhs, m.vtOff, m.vtList, m.tetOut, m.ringRange, m.cNbr, m.cRest, m.cStrain,
+           m.cBendIn, m.clothA, m.X, m.colRhs, m.fixK, m.Xrest, m.sysRhs);
 }
 
 // ---- adjoint of the local step (FP64), evaluated at the recorded x

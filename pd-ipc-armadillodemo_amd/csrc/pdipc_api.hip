@@ -106,6 +106,14 @@ int upload_all(pdipc_handle* h) {
   HIPCHK(hipMemset(m.colDiag, 0, std::max(1, hs.nTotal) * sizeof(double)));
   UP(cCenter, hs.cCenter); UP(cOff, hs.cOff.empty() ? std::vector<int>{0} : hs.cOff); UP(cNbr, hs.cNbr);
   UP(clothOfPoint, hs.clothOfPoint); UP(cRest, hs.cRest); UP(cLbo, hs.cLbo); UP(cBendIn, hs.cBendIn);
+  {
+    std::vector<int2> rr(std::max(1, hs.nSim), make_int2(0, 0));
+    for (int i = 0; i < hs.nSim; i++) {
+      const int c = hs.clothOfPoint[i];
+      if (c >= 0) rr[i] = make_int2(hs.cOff[c], hs.cOff[c + 1]);
+    }
+    UP(ringRange, rr);
+  }
   UP(cStrain, hs.cStrain); UP(cH0, hs.cH0);
   {
     std::vector<int4> t(hs.nTets);

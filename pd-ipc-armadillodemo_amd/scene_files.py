@@ -318,25 +318,56 @@ def surface_positions(scene, surface_index, tet_positions):
 
 
 def _interpolation(points, nodes, tets):
+    """BaseModel::computeTetMeshInterpolation (Model/BaseModel.cpp:249-350):
+    for each surface point the first tet (in tet order) whose barycentric
+    coordinates are all >= 0 once those below 1e-6 in magnitude are set to 0
+    (BaseTetMesh::isInsideTetElement, VolumetricMesh/BaseTetMesh.cpp:241-259,
+    with the float build's MIN_VALUE), normalised to sum 1; otherwise the
+    nearest tet node and the first tet that holds it, with that corner's weight
+    1.  Coordinates come from an FP64 affine inverse per tet.  A uniform grid
+    of tet boxes (padded by 1e-5 of the tet's extent, beyond what the 1e-6
+    tolerance admits) lists each point's candidate tets in tet order."""
     pts = np.asarray(points, np.float64)
     nd = np.asarray(nodes, np.float64)
+    tets = np.asarray(tets)
     X = nd[tets]  # (T, 4, 3)
     A = np.concatenate([X, np.ones((len(tets), 4, 1))], 2).transpose(0, 2, 1)  # columns: (x, 1) per corner
     inv = np.linalg.inv(A)
-    eid = np.zeros(len(pts), np.int32)
-    w = np.zeros((len(pts), 4))
+    ext = (X.max(1) - X.min(1)).max(1)
+    lo = X.min(1) - (1e-5 * ext + 1e-12)[:, None]
+    hi = X.max(1) + (1e-5 * ext + 1e-12)[:, None]
+    g0 = np.minimum(lo.min(0), pts.min(0))
+    h = max(float(np.median(ext)), 1e-12)
+    c_lo = np.floor((lo - g0) / h).astype(np.int64)
+    c_hi = np.floor((hi - g0) / h).astype(np.int64)
+    cells = {}
+    for t in range(len(tets)):
+        for cx in range(c_lo[t, 0], c_hi[t, 0] + 1):
+            for cy in range(c_lo[t, 1], c_hi[t, 1] + 1):
+                for cz in range(c_lo[t, 2], c_hi[t, 2] + 1):
+                    cells.setdefault((cx, cy, cz), []).append(t)
+    pc = np.floor((pts - g0) / h).astype(np.int64)
     first_tet = {}
     for t, row in enumerate(tets):
         for v in row:
             first_tet.setdefault(int(v), t)
+    eid = np.zeros(len(pts), np.int32)
+    w = np.zeros((len(pts), 4))
     for i, p in enumerate(pts):
-        b = inv @ np.append(p, 1.0)
-        b[np.abs(b) < 1e-6] = 0.0
-        inside = np.nonzero((b >= 0).all(1))[0]
-        if len(inside):
-            j = inside[0]
-            eid[i], w[i] = j, b[j] / b[j].sum()
-        else:
+        cand = cells.get(tuple(pc[i]))
+        j = -1
+        if cand:
+            cand = np.asarray(cand)
+            inb = ((p >= lo[cand]) & (p <= hi[cand])).all(1)
+            cand = cand[inb]
+            if len(cand):
+                b = inv[cand] @ np.append(p, 1.0)
+                b[np.abs(b) < 1e-6] = 0.0
+                ins = np.nonzero((b >= 0).all(1))[0]
+                if len(ins):
+                    j = int(cand[ins[0]])
+                    eid[i], w[i] = j, b[ins[0]] / b[ins[0]].sum()
+        if j < 0:
             nid = int(np.argmin(((nd - p) ** 2).sum(1)))
             j = first_tet[nid]
             eid[i] = j

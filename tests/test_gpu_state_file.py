@@ -52,3 +52,28 @@ def test_rejects_other_scenes_and_damaged_files(tmp_path):
     x = a.x().copy()
     a.load_state(path)  # the rejected loads left the state alone; this one restores it
     assert np.array_equal(a.x(), x)
+
+
+@pytest.mark.parametrize("as_double", [0, 1])
+def test_frame_dump_has_the_eigen_write_binary_layout(tmp_path, as_double):
+    """pdipc_save_frame writes what saveSimulator writes (PdIpcSimulator.h:879-895
+    through EigenMatrixIO::write_binary, Commom/EigenMatrixIO.h:22-27): the
+    Eigen index type (int64) rows = 3 and cols = points, then the 3 x n points
+    matrix column by column (x, y, z of point 0, then point 1, ...), in the
+    reference's float (or in double)."""
+    sc = scenes.cloth_over_sphere(12, 2, ground=True, seed=2, patch_dim=12, sphere_patch_dim=12)
+    sim = pdipc.Simulator.from_scene(sc)
+    sim.step(3)
+    path = str(tmp_path / "frame_0003.dat")
+    print(f"case: frame dump, as_double={as_double}", flush=True)
+    sim.save_simulator(path, as_double=bool(as_double))
+    raw = open(path, "rb").read()
+    x = sim.x()
+    n = len(x)
+    item = 8 if as_double else 4
+    assert len(raw) == 16 + 3 * n * item
+    rows, cols = np.frombuffer(raw[:16], "<i8")
+    assert (rows, cols) == (3, n)
+    data = np.frombuffer(raw[16:], "<f8" if as_double else "<f4")
+    cm = data.reshape(cols, rows)  # column-major 3 x n: one column (point) after another
+    assert np.array_equal(cm, x.astype(np.float64 if as_double else np.float32))

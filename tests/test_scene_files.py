@@ -273,3 +273,38 @@ def test_shipped_demo_steps(tmp_path):
         c, o = sim.counters(), ref.counters()
         assert c["candidates"] == o["candidates"] and c["outer_iters"] == o["outer"]
         print("demo step", k, "error", err, "sensitivity", sens, "active", o["active_pairs"])
+
+
+def test_tet_interpolation_matches_shipped_file():
+    """BaseModel::computeTetMeshInterpolation restated (scene_files._interpolation)
+    against the armadillo's shipped tetMeshInterpolation.txt: the same tet and
+    weights for every surface point but a handful at tet boundaries, where the
+    reference's float inside test and the FP64 one decide differently (one of
+    the two then takes the nearest-node fallback); fallback entries exact;
+    inside weights within the float rounding of the reference's 4x4 inverse.
+    The shipped weights put each inside point back at its rest position."""
+    r = np.load(os.path.join(HERE, "golden", "cloth_armadillo_demo_rest.npz"))
+    v, nodes, tets = r["armadillo_v"], r["armadillo_nodes"], r["armadillo_tets"]
+    fe, fw = r["armadillo_interp_eid"], r["armadillo_interp_w"]
+    eid, w = scene_files._interpolation(v, nodes, tets)
+
+    def onehot(W):
+        return (W == 1.0).any(1) & ((W == 0.0).sum(1) == 3)
+
+    differ = (eid != fe) | (np.abs(w - fw).max(1) > 1e-2)
+    assert differ.sum() <= 5, differ.sum()  # of 25,002 points
+    assert (onehot(w[differ]) != onehot(fw[differ])).all()
+    same_fallback = ~differ & onehot(fw)
+    assert same_fallback.sum() > 15000
+    assert np.array_equal(w[same_fallback], fw[same_fallback])
+    inside = ~differ & ~onehot(fw)
+    dw = np.abs(w - fw)[inside].max(1)
+    assert dw.max() < 2e-3 and np.median(dw) < 1e-4
+    # surface_positions with the shipped weights at the rest nodes
+    sc = {"tets": [{"t": tets, "v": nodes}], "surfaces": [{"tet_model": 0, "v": v, "interpolation": (fe, fw)}]}
+    pos = scene_files.surface_positions(sc, 0, nodes)
+    err = np.abs(pos - v).max(1)
+    assert err[~onehot(fw)].max() < 5e-3 and np.median(err[~onehot(fw)]) < 1e-4  # model extent ~150
+    fb = np.nonzero(onehot(fw))[0]
+    corner = np.argmax(fw[fb], 1)
+    assert np.array_equal(pos[fb], nodes[tets[fe[fb], corner]].astype(F32))  # snapped to the nearest node
