@@ -755,15 +755,104 @@ __global__ void __launch_bounds__(kGbarThreads) k_adj_gbar(int ld, const int64_t
   }
 }
 
+// The reverse sweep with its operand staged in LDS (as k_sweep64s; the
+// reverse iterates are SoA, so a staged row is three loads)
+template <int G, int SP>
+__global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, int ld, double a, double b,
+                                                          const int64_t* __restrict__ sliceOff,
+                                                          const int* __restrict__ sliceW, int uw,
+                                                          const int* __restrict__ perm,
+                                                          const uint16_t* __restrict__ loc,
+                                                          const double* __restrict__ val,
+                                                          const int* __restrict__ tileOff,
+                                                          const int* __restrict__ tileBase,
+                                                          const uint16_t* __restrict__ tileCol,
+                                                          const double* __restrict__ uin,
+                                                          const double* __restrict__ uoth, double* __restrict__ uout,
+                                                          double* __restrict__ uc) {
+  extern __shared__ Row64 us64[];
+  const int tile = blockIdx.x;
+  const int u0 = tileOff[tile], nU = tileOff[tile + 1] - u0;
+  const int slice = tile * kTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const bool live = slice < nSlices;
+  constexpr int S = kWave / G;
+  const int sl = live ? slice : nSlices - 1;
+  const int slot = sl * S + lane / G;
+  const int row = perm ? perm[slot] : slot;
+  const int width = uw > 0 ? uw : sliceW[sl];
+  const uint32_t off = (uint32_t)(uw > 0 ? (int64_t)sl * uw * kWave : sliceOff[sl]) + lane;
+  double v[kChunk64];
+  int li[kChunk64];
+  load_chunk(val, loc, off, 0, width, v, li);
+  {
+    const int base = tileBase[tile];
+    int c[SP];
+#pragma unroll
+    for (int q = 0; q < SP; q++) {
+      const int u = threadIdx.x + q * kBlock;
+      c[q] = base + (int)ld32(tileCol, (uint32_t)(u0 + (u < nU ? u : nU - 1)));
+    }
+    double g[SP][3];
+#pragma unroll
+    for (int q = 0; q < SP; q++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) g[q][k] = ld32(uin, (uint32_t)(k * ld + c[q]));
+#pragma unroll
+    for (int q = 0; q < SP; q++)
+      if (threadIdx.x + q * kBlock < nU) us64[threadIdx.x + q * kBlock] = Row64{g[q][0], g[q][1], g[q][2], 0.0};
+  }
+  __syncthreads();
+  if (!live) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int c0 = 0;;) {
+#pragma unroll
+    for (int u = 0; u < kChunk64; u++) {
+      const Row64 x = us64[li[u]];
+      acc[0] += v[u] * x.x;
+      acc[1] += v[u] * x.y;
+      acc[2] += v[u] * x.z;
+    }
+    c0 += kChunk64;
+    if (c0 >= width) break;
+    load_chunk(val, loc, off, c0, width, v, li);
+  }
+  group_reduce<G>(acc);
+  if ((lane % G) != 0 || row >= n) return;
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    int idx = k * ld + row;
+    uout[idx] = a * acc[k] + (b != 0.0 ? b * uoth[idx] : 0.0);
+    uc[idx] += a * uin[idx];
+  }
+}
+
 template <int G>
 void adj_pass_impl(const DeviceMesh& m, const AdjMesh& a, const double* W, int K, double* const* u, const double* x0,
                    const double* x1, int64_t xStride, double* Gbar, double* uc, hipStream_t s) {
   const int grid = (m.nSlices * kWave + kBlock - 1) / kBlock;
   for (int k = K - 1; k >= 0; k--) {
     double b = k + 1 <= K - 1 ? 1.0 - W[k + 1] : 0.0;
-    hipLaunchKernelGGL((m.m2Col16 ? k_adj_spmv<G, true> : k_adj_spmv<G, false>), dim3(grid), dim3(kBlock), 0, s,
-                       m.nSim, m.nSlices, m.ld, W[k], b, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16,
-                       m.m2Val, u[k + 1], u[k + 2], u[k], uc);
+    bool staged = false;
+    if constexpr (G > 1) {
+      if (m.m2Loc) {
+        const int tiles = (m.nSlices + kTileSlices - 1) / kTileSlices;
+        const size_t lds = (size_t)m.stageMax * sizeof(Row64);
+        if (stage_per_lane(m.stageMax) == 2)
+          hipLaunchKernelGGL((k_adj_spmv_s<G, 2>), dim3(tiles), dim3(kBlock), lds, s, m.nSim, m.nSlices, m.ld, W[k], b,
+                             m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.tileOff, m.tileBase,
+                             m.tileCol, u[k + 1], u[k + 2], u[k], uc);
+        else
+          hipLaunchKernelGGL((k_adj_spmv_s<G, 4>), dim3(tiles), dim3(kBlock), lds, s, m.nSim, m.nSlices, m.ld, W[k], b,
+                             m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.tileOff, m.tileBase,
+                             m.tileCol, u[k + 1], u[k + 2], u[k], uc);
+        staged = true;
+      }
+    }
+    if (!staged)
+      hipLaunchKernelGGL((m.m2Col16 ? k_adj_spmv<G, true> : k_adj_spmv<G, false>), dim3(grid), dim3(kBlock), 0, s,
+                         m.nSim, m.nSlices, m.ld, W[k], b, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col,
+                         m.m2Col16, m.m2Val, u[k + 1], u[k + 2], u[k], uc);
   }
   GbarArgs ga;
   ga.K = K;

@@ -327,3 +327,23 @@ def test_adjoint_after_stiffness_change_matches_dual_oracle():
     got = np.array([dkg @ dk[q] + np.sum(dv0 * dv[q]) for q in range(len(dk))])
     print("adjoint", got, "\ndual   ", want, flush=True)
     assert np.abs(got - want).max() <= 1e-6 * np.abs(want).max()
+
+
+def test_staged_reverse_sweeps_match_l2_gather():
+    """The reverse sweeps staged in LDS (the default) against the L2-gather
+    ones: the same trajectory bit for bit and the same gradient (to the
+    rounding of the barrier adjoint's FP64 atomics)."""
+    steps = 10
+    sc, v0, target = _cloth_case(6, steps=steps)
+    ntri = len(sc["cloths"][0]["f"])
+    out = {}
+    for st in (1, 0):
+        print(f"case: adjoint with sell_stage={st}", flush=True)
+        sim = pdipc.Simulator.from_scene(sc, sell_stage=st)
+        sim.set_v(v0)
+        sim.rollout(steps, 5)
+        xT = sim.x()[: sim.sim].copy()
+        out[st] = (xT,) + sim.backward(dx=2.0 * (xT - target), n_tris=ntri)
+    assert np.array_equal(out[0][0], out[1][0])
+    for a, b in zip(out[1][1:], out[0][1:]):
+        assert np.abs(a - b).max() <= 1e-12 * np.abs(b).max()
