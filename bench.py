@@ -135,6 +135,14 @@ def stacked_bench(args):
                 f"K {c['candidates']}, Q {c['patch_pairs']}, active {c['active_pairs']}")
     n_end = sim.count_intersections()
     log(f"intersections at end: {n_end}")
+    # in-step kernel durations over two more steps (direct launches, every launch timed)
+    sim.kernel_timing(True)
+    sim.step(2)
+    kt = {k: {"mean_us": us / n, "launches": n} for k, (us, n) in sim.kernel_times().items() if n}
+    sim.kernel_timing(False)
+    log(f"in-step kernel means: {json.dumps(kt)}")
+    sweep_bytes = 8 * c0["nnz_r2"] + 4 * 3 * c0["sim_points"] * 4
+    sweep_us = kt["sweep_fp32"]["mean_us"]
     return {
         "metric": "PD-IPC steps/s, scale run (3 stacked 1024^2 cloths)",
         "value": args.steps / secs,
@@ -159,6 +167,20 @@ def stacked_bench(args):
             "patch_pairs_Q": {"first": qs[0], "last": qs[-1], "max": max(qs)},
             "intersections_fp64": {"start": n_start, "end": n_end},
             "graphs": bool(args.graphs),
+            "kernel_means_us": kt,
+        },
+        "roofline": {
+            "kernel": "k_sweep32s (FP32 Chebyshev sweep, SELL M^2, operand staged in LDS)",
+            "bound": "hbm",
+            "achieved": sweep_bytes / (sweep_us * 1e-6) / 1e9,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": sweep_bytes / (sweep_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_us": sweep_us,
+            "bytes_per_launch": sweep_bytes,
+            "bytes_formula": "8 B per M^2 entry (FP32 value + int32 column) + 4 vectors x 3N x 4 B",
+            "mall_resident": c0["sell_entries"] * 6.0 < 256e6,
         },
     }
 

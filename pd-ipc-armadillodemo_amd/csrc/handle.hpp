@@ -76,11 +76,12 @@ struct pdipc_handle {
   }
   pdipc::AdjointState* adj = nullptr;
   ~pdipc_handle() {
+    if (stream) (void)hipStreamSynchronize(stream);
+    drop_graphs();  // the executable graphs go before the buffers their nodes point to
     delete m.timer;
     pdipc::adjoint_free(adj);
     for (void* p : allocs) (void)hipFree(p);
     if (scalHost) (void)hipHostFree(scalHost);
-    drop_graphs();
     if (stream) (void)hipStreamDestroy(stream);
   }
   void drop_graphs() {
