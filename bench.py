@@ -41,7 +41,10 @@ HBM_PEAK_GBS = 8000.0
 # Bytes one FP32 sweep launch moves past L2 at the default workload, from two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
 # a plain 1 GiB copy gives): profiles/k_sweep_config1_hbm_traffic.md.
-SWEEP32_TRAFFIC_BYTES = {(256, 6, 8): 29.86e6}
+SWEEP32_TRAFFIC_BYTES = {(256, 6, 8): 31.74e6}
+# the same for the three stacked 1024^2 cloths at 8 lanes per row:
+# profiles/config4_stacked_kernel_stats.md
+STACKED_SWEEP32_TRAFFIC_BYTES = {8: 1524.5e6}
 # steps of each scene's run from rest; by default the warm-up steps and the
 # timed steps together make up the whole run
 RUN_STEPS = {"cloth": 200, "config0": 100, "tet_block": 100, "stacked": 50}
@@ -176,7 +179,7 @@ def stacked_bench(args):
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": sweep_bytes / (sweep_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": STACKED_SWEEP32_TRAFFIC_BYTES.get(args.sell_lanes),
             "avg_us": sweep_us,
             "bytes_per_launch": sweep_bytes,
             "bytes_formula": "8 B per M^2 entry (FP32 value + int32 column) + 4 vectors x 3N x 4 B",
@@ -496,6 +499,8 @@ def main():
             st[name] = dict(seconds=secs, outer=outer, recs=recs)
             log(f"settled {name}: steps {settle_step + 1}-{settle_step + args.settle_steps} in {secs:.3f} s "
                 f"({args.settle_steps / secs:.2f} steps/s, {outer} outer iterations)")
+        kt_settled = {"fp32_split": kernel_means(sims["fp32_split"], x_s, v_s, args.settle_steps)}
+        log(f"settled in-step kernel means: {json.dumps(kt_settled)}")
         a.set_x(x_s)
         a.set_v(v_s)
         serr = step_errors(3)
@@ -508,6 +513,7 @@ def main():
             "fp32_split_speedup_vs_fp64": st["fp64"]["seconds"] / st["fp32_split"]["seconds"],
             "outer_iters_per_step": st["fp32_split"]["outer"] / args.settle_steps,
             "fp32_split_step_error_vs_fp64": max(serr),
+            "kernel_means_us": kt_settled,
         }
 
     c = sims["fp64"].counters()
