@@ -302,18 +302,21 @@ def adjoint_bench(graphs, fd_dirs, log_every=8):
         for _ in range(steps):
             sim_fd.step(1)
             cn = sim_fd.counters()
-            sig.append((cn["outer_iters"], cn["candidates"], cn["active_pairs"], cn["toi_argmins"]))
+            sig.append((cn["outer_iters"], cn["candidates"], cn["active_pairs"], cn["toi_argmins"],
+                        cn["active_sets"]))
         return float(np.sum((sim_fd.x()[: sim_fd.sim] - target) ** 2)), sig
 
     def switch_of(s0, sp, sm):
         """Why the +/- eps runs leave the adjoint's frozen choices, or None: a
-        different outer-iteration, candidate-record or active-pair count, or a
-        different record attaining the minimum TOI, at some step."""
-        names = ("outer iterations", "candidate records", "active barrier pairs", "minimum-TOI records")
+        different outer-iteration, candidate-record or active-pair count, a
+        different record attaining the minimum TOI or ACCD exit path of that
+        record, or a different set of active barrier records, at some step."""
+        names = ("outer iterations", "candidate records", "active barrier pairs",
+                 "minimum-TOI records or their ACCD exit paths", "active barrier record sets")
         for st, (a, b, c0) in enumerate(zip(sp, sm, s0)):
             for idx, name in enumerate(names):
                 if not (a[idx] == b[idx] == c0[idx]):
-                    if idx == 3:
+                    if idx >= 3:
                         return f"step {st + 1}: {name} differ"
                     return f"step {st + 1}: {name} {b[idx]} / {c0[idx]} / {a[idx]} (-eps / 0 / +eps)"
         return None
@@ -357,8 +360,9 @@ def adjoint_bench(graphs, fd_dirs, log_every=8):
                          "rel_err_max": float(errs[~smooth].max()) if (~smooth).any() else None},
         "fd_outliers_above_1e-4": outliers,
         "fd_switch_rule": "a direction switches when its -eps, 0 and +eps runs differ at some step in outer "
-                          "iterations, candidate records, active barrier pairs or the record attaining the minimum "
-                          "TOI of an outer iteration (track_toi_argmin)",
+                          "iterations, candidate records, active barrier pairs, the record attaining the minimum "
+                          "TOI of an outer iteration or the iteration and test at which its ACCD loop ended, or "
+                          "the set of active barrier records (track_toi_argmin)",
     }
 
 
@@ -389,6 +393,11 @@ def main():
     ap.add_argument("--keep-static-pairs", action="store_true",
                     help="keep static-static patch pairs (the reference's full candidate set)")
     ap.add_argument("--adjoint-fd", type=int, default=32, help="finite-difference directions of the adjoint check")
+    ap.add_argument("--state-out", default=None,
+                    help="write the state after the warm-up steps to this file (pdipc_save_state)")
+    ap.add_argument("--state-in", default=None,
+                    help="start from this pdipc_save_state file instead of running the warm-up steps (written "
+                         "by --state-out with the same --warmup: the timed steps keep their step numbers)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain 1 GiB device copy (3 times): the reference for the HBM byte counters")
     ap.add_argument("--no-adjoint", action="store_true")
@@ -440,15 +449,20 @@ def main():
 
     # warm-up on the FP64 handle, then both precisions time K steps from that state
     base = sims["fp64"]
-    for k in range(args.warmup):
-        base.step(1)
+    if args.state_in:
+        base.load_state(args.state_in)
+    else:
+        for k in range(args.warmup):
+            base.step(1)
+    if args.state_out:
+        base.save_state(args.state_out)
     x_w, v_w = base.x(), base.v()
     sims["fp32_split"].set_x(x_w)
     sims["fp32_split"].set_v(v_w)
     sims["fp32_split"].step(1)  # warm its kernels, then reset
     sims["fp32_split"].set_x(x_w)
     sims["fp32_split"].set_v(v_w)
-    log(f"warm-up done ({args.warmup} steps)")
+    log(f"warm-up done ({args.warmup} steps{', restored from ' + args.state_in if args.state_in else ''})")
 
     res = {}
     for name in ("fp64", "fp32_split"):

@@ -68,8 +68,11 @@ typedef struct pdipc_params {
   int capacity_hint;   /* 0: default initial capacities of the growable broad-phase and contact
                           buffers (sized from the patch count); > 0: start each at this many
                           entries (they grow on demand either way) */
-  int track_toi_argmin; /* 1: also find the record attaining the minimum TOI at every outer
-                           iteration (pdipc_counters.toi_argmins; diagnostics, one more pass) */
+  int track_toi_argmin; /* 1: also report the choices a derivative of the step holds fixed, at
+                           every outer iteration: the record attaining the minimum TOI and its
+                           ACCD exit path (pdipc_counters.toi_argmins) and the set of active
+                           barrier records (pdipc_counters.active_sets); diagnostics, two more
+                           passes and one more host wait per outer iteration */
 } pdipc_params;
 
 typedef struct pdipc_counters {
@@ -86,7 +89,10 @@ typedef struct pdipc_counters {
   long long sell_entries; /* stored M^2 entries including SELL padding */
   int sim_points, total_points, cloth_points, tets, patches, boundary_faces, boundary_edges;
   long long toi_argmins;  /* with track_toi_argmin: a hash of the record attaining the minimum TOI
-                             at each outer iteration of the last step (0 where none is below 1) */
+                             at each outer iteration of the last step and of the iteration and
+                             test at which its ACCD loop ended (0 where none is below 1) */
+  long long active_sets;  /* with track_toi_argmin: a hash of the active barrier records of each
+                             outer iteration of the last step */
 } pdipc_counters;
 
 /* --- lifecycle: PdIpcSimulator ctor + readExtraAttributeFromConfigFile (PdIpcSimulator.h:441-450,602-642) */

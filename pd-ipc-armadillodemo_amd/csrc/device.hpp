@@ -263,6 +263,11 @@ void launch_adj_local(const DeviceMesh& m, const AdjMesh& a, double sigLo, doubl
 void launch_ccd_records(const DeviceMesh& m, const int4* recs, int nRecs, const double* x0, const double* x1,
                         double filter, int iters, double step, double gap, int touchPolicy, double* recToi,
                         int* argmin, hipStream_t s);
+// diagnostics of the frozen choices: the ACCD exit path of the record in
+// argmin[0] into argmin[5]; the active barrier records' hash added to *hash
+void launch_argmin_path(const DeviceMesh& m, int nRecs, const double* x0, const double* x1, double filter, int iters,
+                        double step, double gap, int touchPolicy, int* argmin, hipStream_t s);
+void launch_active_hash(const DeviceMesh& m, unsigned long long* hash, hipStream_t s);
 // over the active records `act` (indices into recs) of that collision pass
 void launch_adj_collision(const DeviceMesh& m, const int4* recs, const int* act, int nAct, const double* x, double dhat,
                           double kappa, double dt, const double* colDbar, const double* colRbar, double* ax,

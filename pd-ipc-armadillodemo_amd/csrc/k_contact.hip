@@ -167,9 +167,13 @@ __host__ __device__ inline T ccd_dfun(int type, const P3T<T>* x, T* d2out) {
   return d2;
 }
 
-// ACCD with the reference's constants as parameters
+// ACCD with the reference's constants as parameters.  With `exitPath`, also
+// the discrete path the loop took: 8 * (loop iteration) + how it ended (0 all
+// iterations, 1 the gap test, 2 the clamp at 1, 3 a touch inside the loop,
+// 4 no relative motion, 5 touching at the start, 6 the first-step return).
 template <class T>
-__host__ __device__ inline T accd(int type, P3T<T>* x, P3T<T>* dx, int iters, double step, double gapf, int touchPolicy) {
+__host__ __device__ inline T accd(int type, P3T<T>* x, P3T<T>* dx, int iters, double step, double gapf, int touchPolicy,
+                                  int* exitPath = nullptr) {
   P3T<T> mov;
 #pragma unroll
   for (int k = 0; k < 3; k++) mov.v[k] = (dx[0].v[k] + dx[1].v[k] + dx[2].v[k] + dx[3].v[k]) / T(4.0);
@@ -187,13 +191,20 @@ __host__ __device__ inline T accd(int type, P3T<T>* x, P3T<T>* dx, int iters, do
     T m1 = mx2(dot(dx[2], dx[2]), dot(dx[3], dx[3]));
     maxdisp = sqrt(m0) + sqrt(m1);
   }
-  if (maxdisp == T(0.0)) return T(1.0);
+  if (maxdisp == T(0.0)) {
+    if (exitPath) *exitPath = 4;
+    return T(1.0);
+  }
   T d2;
   T dF = ccd_dfun(type, x, &d2);
   T dcur = sqrt(d2);
-  if (touchPolicy == 1 && !(dcur > T(0.0))) return T(0.0);
+  if (touchPolicy == 1 && !(dcur > T(0.0))) {
+    if (exitPath) *exitPath = 5;
+    return T(0.0);
+  }
   T gap = T(gapf) * dF / dcur;
   T toc = T(0.0);
+  if (exitPath) *exitPath = 8 * iters;
   for (int i = 0; i < iters; i++) {
     T tlb = T(step) * dF / (dcur * maxdisp);
     // First iteration with tlb > 1: the loop returns 1.  The move shifts the
@@ -203,17 +214,30 @@ __host__ __device__ inline T accd(int type, P3T<T>* x, P3T<T>* dx, int iters, do
     // rounding can turn that distance into 0.  The second distance
     // evaluation is skipped (most records are far apart relative to their
     // motion).
-    if (i == 0 && tlb > T(1.0) && dcur > T(1e-9)) return T(1.0);
+    if (i == 0 && tlb > T(1.0) && dcur > T(1e-9)) {
+      if (exitPath) *exitPath = 6;
+      return T(1.0);
+    }
 #pragma unroll
     for (int j = 0; j < 4; j++)
 #pragma unroll
       for (int k = 0; k < 3; k++) x[j].v[k] += tlb * dx[j].v[k];
     dF = ccd_dfun(type, x, &d2);
     dcur = sqrt(d2);
-    if (touchPolicy == 1 && !(dcur > T(0.0))) return toc;
-    if (toc != T(0.0) && (dF / dcur) < gap) break;
+    if (touchPolicy == 1 && !(dcur > T(0.0))) {
+      if (exitPath) *exitPath = 8 * i + 3;
+      return toc;
+    }
+    if (toc != T(0.0) && (dF / dcur) < gap) {
+      if (exitPath) *exitPath = 8 * i + 1;
+      break;
+    }
     toc += tlb;
-    if (toc > T(1.0)) { toc = T(1.0); break; }
+    if (toc > T(1.0)) {
+      if (exitPath) *exitPath = 8 * i + 2;
+      toc = T(1.0);
+      break;
+    }
   }
   return toc;
 }
@@ -589,6 +613,44 @@ __global__ void k_toi_argmin(int nRecs, const double* __restrict__ recToi, const
   if (tmin < 1.0 && recToi[t] == tmin) atomicMin(argmin, t);
 }
 
+// ACCD exit path of the record argmin[0] (written to argmin[5]; -1 when there
+// is none): one thread re-runs that record's ACCD
+__global__ void k_argmin_path(int nRecs, int ld, const int4* __restrict__ recs, const double* __restrict__ X0,
+                              const double* __restrict__ X1, double filter, int iters, double step, double gapf,
+                              int touchPolicy, int* __restrict__ argmin) {
+  if (blockIdx.x != 0 || threadIdx.x != 0) return;
+  const int t = argmin[0];
+  int path = -1;
+  int id[4];
+  if (t >= 0 && t < nRecs && record_ids(recs[t], id)) {
+    const int type = recs[t].x < 0 ? 1 : 0;
+    P3 x[4], dx[4];
+    for (int q = 0; q < 4; q++) {
+      x[q] = load3(X0, ld, id[q]);
+      dx[q] = sub(load3(X1, ld, id[q]), x[q]);
+    }
+    if (type == 0 ? pt_ccd_filter(x, dx, filter) : ee_ccd_filter(x, dx, filter))
+      accd(type, x, dx, iters, step, gapf, touchPolicy, &path);
+  }
+  argmin[5] = path;
+}
+
+// order-independent hash of the active barrier records (the sum of a 64-bit
+// mix of each record index), added to *hash
+__global__ void k_active_hash(const int* __restrict__ counters, int actCap, const int* __restrict__ actRec,
+                              unsigned long long* __restrict__ hash) {
+  const int A = min(counters[2], actCap);
+  unsigned long long sum = 0;
+  for (int a = blockIdx.x * blockDim.x + threadIdx.x; a < A; a += gridDim.x * blockDim.x) {
+    unsigned long long z = (unsigned long long)actRec[a] + 0x9e3779b97f4a7c15ull;  // splitmix64 finaliser
+    z = (z ^ (z >> 30)) * 0xbf58476d1ce4e5b9ull;
+    z = (z ^ (z >> 27)) * 0x94d049bb133111ebull;
+    sum += z ^ (z >> 31);
+  }
+  for (int o = kWave / 2; o > 0; o >>= 1) sum += __shfl_xor(sum, o, kWave);
+  if ((threadIdx.x % kWave) == 0 && sum) atomicAdd(hash, sum);
+}
+
 // d toi / d (x0, x1) of one record, scaled by g, added to ax0 / ax1 (simulated
 // points): thread c of 24 differentiates along input coordinate c
 __global__ void k_adj_ccd(int ld, int nSim, const int4* __restrict__ recs, int argmin, const double* __restrict__ X0,
@@ -651,6 +713,16 @@ void launch_ccd_records(const DeviceMesh& m, const int4* recs, int nRecs, const 
                      filter, iters, step, gap, touchPolicy, recToi, m.toiBits);
   hipLaunchKernelGGL(k_toi_argmin, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, recToi, m.toiBits,
                      argmin);
+}
+
+void launch_argmin_path(const DeviceMesh& m, int nRecs, const double* x0, const double* x1, double filter, int iters,
+                        double step, double gap, int touchPolicy, int* argmin, hipStream_t s) {
+  hipLaunchKernelGGL(k_argmin_path, dim3(1), dim3(64), 0, s, nRecs, m.ld, m.recs, x0, x1, filter, iters, step, gap,
+                     touchPolicy, argmin);
+}
+
+void launch_active_hash(const DeviceMesh& m, unsigned long long* hash, hipStream_t s) {
+  hipLaunchKernelGGL(k_active_hash, dim3(64), dim3(kBlock), 0, s, m.counters, m.actCap, m.actRec, hash);
 }
 
 void launch_adj_collision(const DeviceMesh& m, const int4* recs, const int* act, int nAct, const double* x, double dhat,

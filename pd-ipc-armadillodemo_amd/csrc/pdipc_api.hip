@@ -389,7 +389,7 @@ int advance(pdipc_handle* h) {
   bool activeCol = false;
   int outer = 0;
   double toiMin = 1.0;
-  long long argminHash = 0;
+  uint64_t argminHash = 0, activeHash = 0;  // diagnostics (track_toi_argmin), wrapping arithmetic
   h->cnt.lg_iters = 0;
   for (; outer < p.max_outer; outer++) {
     if ((rc = local_global(h, activeCol ? 1 : 0))) return rc;
@@ -414,14 +414,25 @@ int advance(pdipc_handle* h) {
       HIPCHK(hipMemsetAsync(m.counters + 4, 0x7f, sizeof(int), s));
       launch_ccd_records(m, m.recs, h->lastRecs, m.Xold, m.X, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap,
                          p.touch_policy, h->trackToi, m.counters + 4, s);
+      launch_argmin_path(m, h->lastRecs, m.Xold, m.X, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap,
+                         p.touch_policy, m.counters + 4, s);
     } else {
       launch_ccd(m, h->lastRecs, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap, p.touch_policy, s);
     }
     launch_clamp_residual(m, p.toi_scale, s);
     if ((rc = collision(h))) return rc;  // also reads the scalars back
-    if (p.track_toi_argmin) {
+    if (p.track_toi_argmin) {  // the record attaining the minimum TOI, its ACCD exit path, the active set
       const int am = track ? h->hCounters()[4] : 0x7f7f7f7f;
-      argminHash = argminHash * 1000003LL + (am >= 0 && am < h->lastRecs ? am + 1 : 0);
+      const bool hit = am >= 0 && am < h->lastRecs;
+      argminHash = argminHash * 1000003ull + (hit ? (uint64_t)am + 1 : 0);
+      argminHash = argminHash * 1000003ull + (hit ? (uint64_t)(h->hCounters()[9] + 2) : 0);
+      unsigned long long* hashDev = (unsigned long long*)(m.counters + 10);
+      HIPCHK(hipMemsetAsync(hashDev, 0, sizeof(unsigned long long), s));
+      launch_active_hash(m, hashDev, s);
+      if ((rc = read_scalars(h))) return rc;
+      uint64_t act;
+      memcpy(&act, h->hCounters() + 10, sizeof act);
+      activeHash = activeHash * 1000003ull + act;
     }
     activeCol = true;
     double res = *h->hResidual();
@@ -438,7 +449,8 @@ int advance(pdipc_handle* h) {
   }
   h->cnt.outer_iters = outer < p.max_outer ? outer + 1 : p.max_outer;
   h->cnt.toi_min = toiMin;
-  h->cnt.toi_argmins = argminHash;
+  h->cnt.toi_argmins = (long long)argminHash;
+  h->cnt.active_sets = (long long)activeHash;
   h->cnt.patch_pairs = h->lastPairs;
   h->cnt.candidates = h->lastRecs;
   launch_velocity(m, p.dt, s);

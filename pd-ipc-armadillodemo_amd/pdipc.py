@@ -39,7 +39,7 @@ class Counters(ctypes.Structure):
                 ("n_2paths", ctypes.c_longlong), ("sell_entries", ctypes.c_longlong), ("sim_points", ctypes.c_int),
                 ("total_points", ctypes.c_int), ("cloth_points", ctypes.c_int), ("tets", ctypes.c_int),
                 ("patches", ctypes.c_int), ("boundary_faces", ctypes.c_int), ("boundary_edges", ctypes.c_int),
-                ("toi_argmins", ctypes.c_longlong)]
+                ("toi_argmins", ctypes.c_longlong), ("active_sets", ctypes.c_longlong)]
 
     def as_dict(self):
         return {name: getattr(self, name) for name, _ in self._fields_}

@@ -49,18 +49,27 @@ def test_timed_steps_match_plain_steps_and_count_launches(prec):
 
 
 def test_toi_argmin_tracking_changes_nothing():
+    """Tracking the frozen choices (minimum-TOI record and its ACCD exit path,
+    active barrier record set) leaves the step bit for bit as it is, and the
+    hashes are reproducible run to run."""
     sc = scenes.cloth_over_sphere(24, 2, ground=True, seed=4, patch_dim=24, sphere_patch_dim=12, height=0.16)
     print("case: TOI argmin tracking", flush=True)
     a = pdipc.Simulator.from_scene(sc)
     b = pdipc.Simulator.from_scene(sc, track_toi_argmin=1)
-    clamped = 0
+    c = pdipc.Simulator.from_scene(sc, track_toi_argmin=1)
+    clamped = active = 0
     for k in range(12):
         a.step(1)
         b.step(1)
+        c.step(1)
         assert np.array_equal(a.x(), b.x()), k
-        ca, cb = a.counters(), b.counters()
-        assert ca["toi_argmins"] == 0
+        ca, cb, cc = a.counters(), b.counters(), c.counters()
+        assert ca["toi_argmins"] == 0 and ca["active_sets"] == 0
+        assert cb["toi_argmins"] == cc["toi_argmins"] and cb["active_sets"] == cc["active_sets"], k
         if cb["toi_min"] < 1.0:
             clamped += 1
             assert cb["toi_argmins"] != 0, k
-    assert clamped > 0
+        if cb["active_pairs"] > 0:
+            active += 1
+            assert cb["active_sets"] != 0, k
+    assert clamped > 0 and active > 0
