@@ -136,11 +136,12 @@ struct DeviceMesh {
   int* p2K = nullptr;
   double* p2A = nullptr;
   // LDS staging of the sweeps' operand (null: the sweeps gather from L2): per
-  // tile of kTileSlices slices the union of its columns (tileCol + tileBase,
-  // tileOff[t] .. tileOff[t + 1]) and per entry its index in that union
-  int *tileOff = nullptr, *tileBase = nullptr;
+  // tile of kTileSlices slices the union of its columns (tileBase[t] +
+  // tileCol[t * stageStride + u], each list padded to stageStride entries with
+  // its last column) and per entry its index in that union
+  int* tileBase = nullptr;
   uint16_t *tileCol = nullptr, *m2Loc = nullptr;
-  int stageMax = 0;
+  int stageStride = 0;
   // collision topology
   int nFaces = 0, nEdges = 0, nPatches = 0;
   int3* bFaces = nullptr;

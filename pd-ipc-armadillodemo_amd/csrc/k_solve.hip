@@ -25,6 +25,8 @@
 //   delta_1 = r0,  delta_{k+1} = W_k (M^2 delta_k + r0 - delta_{k-1}) + delta_{k-1}
 // which is algebraically the same iterate; FP32 only rounds the (small)
 // correction, never the absolute coordinates.
+#include <type_traits>
+
 #include "device.hpp"
 #include "host_setup.hpp"  // kTileSlices
 
@@ -348,20 +350,29 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
 
 // LDS-staged sweeps.  One workgroup runs a tile of kTileSlices slices: it
 // first copies the gathered operand at the tile's column union into LDS (the
-// union's columns as 16-bit offsets from the tile's smallest column; SP rows
-// per lane, all loads in flight together), while the lanes' first kChunk
-// matrix entries are already on their way; after one barrier every entry
-// reads its operand from LDS (its union index `loc`) instead of L2.  On a
+// union's columns as 16-bit offsets from the tile's smallest column, listed at
+// a fixed stride per tile and padded with its last column, so the list loads
+// go out with the kernel's first instructions and every lane stores its rows
+// unconditionally), while the lanes' first kChunk matrix entries are already
+// on their way; after one barrier every entry reads its operand from LDS (its
+// union index `loc`) instead of L2.  LDS holds the staged rows as (x, y) pairs
+// and z values (8 + 4 bytes in FP32): reads of whole 16-byte rows at the
+// entries' scattered union indices conflicted in the LDS banks more (one
+// sweep at config 1: 5.8 -> 4.9 us with the fixed-stride lists).  On a
 // mesh-ordered cloth a 32-row tile touches about 9 runs of ~40 columns (the
 // 4-ring stencil of M^2), so the L2 traffic of the gathers drops from 16 B
-// per entry to about 6 B per row.  Entries are summed in the same order as the
-// L2-gather kernels, so the results are bit-identical to theirs.  Entry
-// offsets are 32-bit (staging requires fewer than 2^29 entries): each load
-// is a 64-bit base in scalar registers plus a 32-bit lane offset, and a
+// per entry to about 6 B per row.  Tiles are dealt to workgroups so that each
+// XCD (workgroups go to the 8 XCDs round-robin) runs a contiguous range of
+// tiles, whose unions overlap, sharing one L2.  Entries are summed in the same
+// order as the L2-gather kernels, so the results are bit-identical to theirs.
+// Entry offsets are 32-bit (staging requires fewer than 2^29 entries): each
+// load is a 64-bit base in scalar registers plus a 32-bit lane offset, and a
 // lane's loads past its slice's width re-read its last entry (no branches)
-// and are weighted 0.
+// and are weighted 0.  UW > 0: every slice has width UW (known at compile
+// time, so the chunk loop and its guards fold away).
 constexpr int kChunk32 = 8;  // matrix entries per lane loaded together (FP32 values)
 constexpr int kChunk64 = 4;  // (FP64 values)
+constexpr int kXcds = 8;
 
 template <class T>
 __device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t i) {
@@ -381,50 +392,85 @@ __device__ __forceinline__ void load_chunk(const V* __restrict__ val, const uint
   }
 }
 
-template <class T, int SP>
-__device__ __forceinline__ void stage_tile(T* __restrict__ xs, const T* __restrict__ src, int u0, int nU, int base,
-                                           const uint16_t* __restrict__ tileCol) {
-  int c[SP];
+// The tile this workgroup runs (-1: a padding workgroup of the last XCDs):
+// XCD x = blockIdx % 8 takes the x-th of 8 contiguous, near-equal tile ranges.
+__device__ __forceinline__ int xcd_tile(int nTiles) {
+  const int x = (int)blockIdx.x % kXcds, k = (int)blockIdx.x / kXcds;
+  const int q = nTiles / kXcds, r = nTiles % kXcds;
+  return k < q + (x < r ? 1 : 0) ? x * q + (x < r ? x : r) + k : -1;
+}
+inline int xcd_grid(int nTiles) { return (nTiles + kXcds - 1) / kXcds * kXcds; }
+
+// LDS rows of a staged tile: (x, y) pairs then z values, SP * kBlock of each
+template <class S>
+struct StagedRows {
+  using S2 = typename std::conditional<sizeof(S) == 4, float2, double2>::type;
+  S2* xy;
+  S* z;
+  __device__ StagedRows(void* lds, int cap) : xy((S2*)lds), z((S*)((S2*)lds + cap)) {}
+  __device__ void put(int u, S x, S y, S w) const {
+    xy[u] = S2{x, y};
+    z[u] = w;
+  }
+};
+template <class S>
+constexpr size_t staged_lds_bytes(int sp) { return (size_t)sp * kBlock * 3 * sizeof(S); }
+
+// the tile's union columns, SP rows per lane (lanes past the stride re-read
+// the list's last entry)
+template <int SP>
+__device__ __forceinline__ void tile_columns(int tile, int stride, const int* __restrict__ tileBase,
+                                             const uint16_t* __restrict__ tileCol, int (&c)[SP]) {
+  const int base = tileBase[tile];
+  const uint32_t t0 = (uint32_t)tile * (uint32_t)stride;
 #pragma unroll
   for (int q = 0; q < SP; q++) {
     const int u = threadIdx.x + q * kBlock;
-    c[q] = base + (int)ld32(tileCol, (uint32_t)(u0 + (u < nU ? u : nU - 1)));
+    c[q] = base + (int)ld32(tileCol, t0 + (uint32_t)(u < stride ? u : stride - 1));
   }
+}
+
+template <int SP, class T>
+__device__ __forceinline__ void stage_tile(StagedRows<typename std::conditional<sizeof(T) == 16, float, double>::type> rows,
+                                           const T* __restrict__ src, int tile, int stride,
+                                           const int* __restrict__ tileBase, const uint16_t* __restrict__ tileCol) {
+  int c[SP];
+  tile_columns<SP>(tile, stride, tileBase, tileCol, c);
   T v[SP];
 #pragma unroll
   for (int q = 0; q < SP; q++) v[q] = ld32(src, (uint32_t)c[q]);
 #pragma unroll
-  for (int q = 0; q < SP; q++)
-    if (threadIdx.x + q * kBlock < nU) xs[threadIdx.x + q * kBlock] = v[q];
+  for (int q = 0; q < SP; q++) rows.put(threadIdx.x + q * kBlock, v[q].x, v[q].y, v[q].z);
 }
 
-template <int G, int LAST, int SP>
-__global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int ld, float W,
+template <int G, int LAST, int SP, int UW>
+__global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int nTiles, int ld, float W,
                                                         const int64_t* __restrict__ sliceOff,
                                                         const int* __restrict__ sliceW, int uw,
                                                         const int* __restrict__ perm, const uint16_t* __restrict__ loc,
-                                                        const float* __restrict__ val, const int* __restrict__ tileOff,
+                                                        const float* __restrict__ val, int stride,
                                                         const int* __restrict__ tileBase,
                                                         const uint16_t* __restrict__ tileCol,
                                                         const float4* __restrict__ dcur, const float4* __restrict__ dpre,
                                                         const float4* __restrict__ r0, float4* __restrict__ dnext,
                                                         double* __restrict__ X) {
-  extern __shared__ float4 xs32[];
-  const int tile = blockIdx.x;
-  const int u0 = tileOff[tile], nU = tileOff[tile + 1] - u0;
+  extern __shared__ float2 lds32[];
+  const StagedRows<float> xs(lds32, SP * kBlock);
+  const int tile = xcd_tile(nTiles);
+  if (tile < 0) return;  // the whole workgroup
   const int slice = tile * kTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const bool live = slice < nSlices;  // every lane stages and meets the barrier
   constexpr int S = kWave / G;
   const int sl = live ? slice : nSlices - 1;
   const int slot = sl * S + lane / G;
   const int row = perm ? perm[slot] : slot;
-  const int width = uw > 0 ? uw : sliceW[sl];
-  const uint32_t off = (uint32_t)(uw > 0 ? (int64_t)sl * uw * kWave : sliceOff[sl]) + lane;
+  const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
+  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + lane;
   float v[kChunk32];
   int li[kChunk32];
   load_chunk(val, loc, off, 0, width, v, li);
   const float4 p = ld32(dpre, (uint32_t)row), r = ld32(r0, (uint32_t)row);
-  stage_tile<float4, SP>(xs32, dcur, u0, nU, tileBase[tile], tileCol);
+  stage_tile<SP>(xs, dcur, tile, stride, tileBase, tileCol);
   __syncthreads();
   if (!live) return;
   float acc[3] = {0.f, 0.f, 0.f};
@@ -432,10 +478,11 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int 
   for (int c0 = 0;;) {
 #pragma unroll
     for (int u = 0; u < kChunk32; u++) {
-      const float4 x = xs32[li[u]];
-      acc[0] += v[u] * x.x;
-      acc[1] += v[u] * x.y;
-      acc[2] += v[u] * x.z;
+      const float2 a = xs.xy[li[u]];
+      const float z = xs.z[li[u]];
+      acc[0] += v[u] * a.x;
+      acc[1] += v[u] * a.y;
+      acc[2] += v[u] * z;
     }
     c0 += kChunk32;
     if (c0 >= width) break;
@@ -455,32 +502,33 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int 
   }
 }
 
-template <int G, int SP>
-__global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int ld, double W,
+template <int G, int SP, int UW>
+__global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int nTiles, int ld, double W,
                                                         const int64_t* __restrict__ sliceOff,
                                                         const int* __restrict__ sliceW, int uw,
                                                         const int* __restrict__ perm, const uint16_t* __restrict__ loc,
-                                                        const double* __restrict__ val, const int* __restrict__ tileOff,
+                                                        const double* __restrict__ val, int stride,
                                                         const int* __restrict__ tileBase,
                                                         const uint16_t* __restrict__ tileCol,
                                                         const Row64* __restrict__ cur, const Row64* __restrict__ pre,
                                                         const Row64* __restrict__ cA, Row64* __restrict__ next,
                                                         double* __restrict__ out) {
-  extern __shared__ Row64 xs64[];
-  const int tile = blockIdx.x;
-  const int u0 = tileOff[tile], nU = tileOff[tile + 1] - u0;
+  extern __shared__ double2 lds64[];
+  const StagedRows<double> xs(lds64, SP * kBlock);
+  const int tile = xcd_tile(nTiles);
+  if (tile < 0) return;
   const int slice = tile * kTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const bool live = slice < nSlices;
   constexpr int S = kWave / G;
   const int sl = live ? slice : nSlices - 1;
   const int slot = sl * S + lane / G;
   const int row = perm ? perm[slot] : slot;
-  const int width = uw > 0 ? uw : sliceW[sl];
-  const uint32_t off = (uint32_t)(uw > 0 ? (int64_t)sl * uw * kWave : sliceOff[sl]) + lane;
+  const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
+  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + lane;
   double v[kChunk64];
   int li[kChunk64];
   load_chunk(val, loc, off, 0, width, v, li);
-  stage_tile<Row64, SP>(xs64, cur, u0, nU, tileBase[tile], tileCol);
+  stage_tile<SP>(xs, cur, tile, stride, tileBase, tileCol);
   __syncthreads();
   if (!live) return;
   double acc[3] = {0.0, 0.0, 0.0};
@@ -488,10 +536,11 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int 
   for (int c0 = 0;;) {
 #pragma unroll
     for (int u = 0; u < kChunk64; u++) {
-      const Row64 x = xs64[li[u]];
-      acc[0] += v[u] * x.x;
-      acc[1] += v[u] * x.y;
-      acc[2] += v[u] * x.z;
+      const double2 a = xs.xy[li[u]];
+      const double z = xs.z[li[u]];
+      acc[0] += v[u] * a.x;
+      acc[1] += v[u] * a.y;
+      acc[2] += v[u] * z;
     }
     c0 += kChunk64;
     if (c0 >= width) break;
@@ -513,34 +562,45 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int 
 
 // rows of the staged union per lane: 2 or 4 (the union holds at most kStageMaxCols = 4 * kBlock)
 static_assert(kStageMaxCols <= 4 * kBlock, "staged unions are loaded at most four rows per lane");
-inline int stage_per_lane(int stageMax) { return stageMax <= 2 * kBlock ? 2 : 4; }
+inline int stage_per_lane(int stride) { return stride <= 2 * kBlock ? 2 : 4; }
+inline int tiles_of(const DeviceMesh& m) { return (m.nSlices + kTileSlices - 1) / kTileSlices; }
 
 template <int G, int LAST>
 void launch_sweep32s(const DeviceMesh& m, float W, const float4* cur, const float4* pre, float4* next, hipStream_t s) {
-  const int tiles = (m.nSlices + kTileSlices - 1) / kTileSlices;
-  const size_t lds = (size_t)m.stageMax * sizeof(float4);
-#define SW32(SP)                                                                                                  \
-  launch_t(m.timer, TC_SWEEP32, k_sweep32s<G, LAST, SP>, dim3(tiles), dim3(kBlock), (uint32_t)lds, s, m.nSim,      \
-           m.nSlices, m.ld, W,                                                                                       \
-                     m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2ValF, m.tileOff, m.tileBase,      \
-                     m.tileCol, cur, pre, m.r0, next, m.X)
-  if (stage_per_lane(m.stageMax) == 2) SW32(2);
-  else SW32(4);
+  const int tiles = tiles_of(m);
+  const int sp = stage_per_lane(m.stageStride);
+#define SW32(SP, UW)                                                                                               \
+  launch_t(m.timer, TC_SWEEP32, k_sweep32s<G, LAST, SP, UW>, dim3(xcd_grid(tiles)), dim3(kBlock),                  \
+           (uint32_t)staged_lds_bytes<float>(SP), s, m.nSim, m.nSlices, tiles, m.ld, W, m.sliceOff, m.sliceW,       \
+           m.sellUniformW, m.sellPerm, m.m2Loc, m.m2ValF, m.stageStride, m.tileBase, m.tileCol, cur, pre, m.r0, next, \
+           m.X)
+  if (m.sellUniformW == kChunk32) {
+    if (sp == 2) SW32(2, kChunk32);
+    else SW32(4, kChunk32);
+  } else {
+    if (sp == 2) SW32(2, 0);
+    else SW32(4, 0);
+  }
 #undef SW32
 }
 
 template <int G>
 void launch_sweep64s(const DeviceMesh& m, double W, const Row64* cur, const Row64* pre, Row64* next, double* out,
                      hipStream_t s) {
-  const int tiles = (m.nSlices + kTileSlices - 1) / kTileSlices;
-  const size_t lds = (size_t)m.stageMax * sizeof(Row64);
-#define SW64(SP)                                                                                                  \
-  launch_t(m.timer, TC_SWEEP64, k_sweep64s<G, SP>, dim3(tiles), dim3(kBlock), (uint32_t)lds, s, m.nSim, m.nSlices, \
-           m.ld, W, m.sliceOff,                                                                                      \
-                     m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.tileOff, m.tileBase, m.tileCol, cur,   \
-                     pre, m.cA, next, out)
-  if (stage_per_lane(m.stageMax) == 2) SW64(2);
-  else SW64(4);
+  const int tiles = tiles_of(m);
+  const int sp = stage_per_lane(m.stageStride);
+#define SW64(SP, UW)                                                                                               \
+  launch_t(m.timer, TC_SWEEP64, k_sweep64s<G, SP, UW>, dim3(xcd_grid(tiles)), dim3(kBlock),                        \
+           (uint32_t)staged_lds_bytes<double>(SP), s, m.nSim, m.nSlices, tiles, m.ld, W, m.sliceOff, m.sliceW,      \
+           m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.stageStride, m.tileBase, m.tileCol, cur, pre, m.cA, next, \
+           out)
+  if (m.sellUniformW == kChunk32) {
+    if (sp == 2) SW64(2, kChunk32);
+    else SW64(4, kChunk32);
+  } else {
+    if (sp == 2) SW64(2, 0);
+    else SW64(4, 0);
+  }
 #undef SW64
 }
 
@@ -757,49 +817,43 @@ __global__ void __launch_bounds__(kGbarThreads) k_adj_gbar(int ld, const int64_t
 
 // The reverse sweep with its operand staged in LDS (as k_sweep64s; the
 // reverse iterates are SoA, so a staged row is three loads)
-template <int G, int SP>
-__global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, int ld, double a, double b,
+template <int G, int SP, int UW>
+__global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, int nTiles, int ld, double a, double b,
                                                           const int64_t* __restrict__ sliceOff,
                                                           const int* __restrict__ sliceW, int uw,
                                                           const int* __restrict__ perm,
                                                           const uint16_t* __restrict__ loc,
-                                                          const double* __restrict__ val,
-                                                          const int* __restrict__ tileOff,
+                                                          const double* __restrict__ val, int stride,
                                                           const int* __restrict__ tileBase,
                                                           const uint16_t* __restrict__ tileCol,
                                                           const double* __restrict__ uin,
                                                           const double* __restrict__ uoth, double* __restrict__ uout,
                                                           double* __restrict__ uc) {
-  extern __shared__ Row64 us64[];
-  const int tile = blockIdx.x;
-  const int u0 = tileOff[tile], nU = tileOff[tile + 1] - u0;
+  extern __shared__ double2 ldsAdj[];
+  const StagedRows<double> us(ldsAdj, SP * kBlock);
+  const int tile = xcd_tile(nTiles);
+  if (tile < 0) return;
   const int slice = tile * kTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const bool live = slice < nSlices;
   constexpr int S = kWave / G;
   const int sl = live ? slice : nSlices - 1;
   const int slot = sl * S + lane / G;
   const int row = perm ? perm[slot] : slot;
-  const int width = uw > 0 ? uw : sliceW[sl];
-  const uint32_t off = (uint32_t)(uw > 0 ? (int64_t)sl * uw * kWave : sliceOff[sl]) + lane;
+  const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
+  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + lane;
   double v[kChunk64];
   int li[kChunk64];
   load_chunk(val, loc, off, 0, width, v, li);
   {
-    const int base = tileBase[tile];
     int c[SP];
-#pragma unroll
-    for (int q = 0; q < SP; q++) {
-      const int u = threadIdx.x + q * kBlock;
-      c[q] = base + (int)ld32(tileCol, (uint32_t)(u0 + (u < nU ? u : nU - 1)));
-    }
+    tile_columns<SP>(tile, stride, tileBase, tileCol, c);
     double g[SP][3];
 #pragma unroll
     for (int q = 0; q < SP; q++)
 #pragma unroll
       for (int k = 0; k < 3; k++) g[q][k] = ld32(uin, (uint32_t)(k * ld + c[q]));
 #pragma unroll
-    for (int q = 0; q < SP; q++)
-      if (threadIdx.x + q * kBlock < nU) us64[threadIdx.x + q * kBlock] = Row64{g[q][0], g[q][1], g[q][2], 0.0};
+    for (int q = 0; q < SP; q++) us.put(threadIdx.x + q * kBlock, g[q][0], g[q][1], g[q][2]);
   }
   __syncthreads();
   if (!live) return;
@@ -808,10 +862,11 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, in
   for (int c0 = 0;;) {
 #pragma unroll
     for (int u = 0; u < kChunk64; u++) {
-      const Row64 x = us64[li[u]];
-      acc[0] += v[u] * x.x;
-      acc[1] += v[u] * x.y;
-      acc[2] += v[u] * x.z;
+      const double2 xy = us.xy[li[u]];
+      const double z = us.z[li[u]];
+      acc[0] += v[u] * xy.x;
+      acc[1] += v[u] * xy.y;
+      acc[2] += v[u] * z;
     }
     c0 += kChunk64;
     if (c0 >= width) break;
@@ -836,16 +891,21 @@ void adj_pass_impl(const DeviceMesh& m, const AdjMesh& a, const double* W, int K
     bool staged = false;
     if constexpr (G > 1) {
       if (m.m2Loc) {
-        const int tiles = (m.nSlices + kTileSlices - 1) / kTileSlices;
-        const size_t lds = (size_t)m.stageMax * sizeof(Row64);
-        if (stage_per_lane(m.stageMax) == 2)
-          hipLaunchKernelGGL((k_adj_spmv_s<G, 2>), dim3(tiles), dim3(kBlock), lds, s, m.nSim, m.nSlices, m.ld, W[k], b,
-                             m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.tileOff, m.tileBase,
-                             m.tileCol, u[k + 1], u[k + 2], u[k], uc);
-        else
-          hipLaunchKernelGGL((k_adj_spmv_s<G, 4>), dim3(tiles), dim3(kBlock), lds, s, m.nSim, m.nSlices, m.ld, W[k], b,
-                             m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.tileOff, m.tileBase,
-                             m.tileCol, u[k + 1], u[k + 2], u[k], uc);
+        const int tiles = tiles_of(m);
+        const int sp = stage_per_lane(m.stageStride);
+#define ADJ(SP, UW)                                                                                             \
+  hipLaunchKernelGGL((k_adj_spmv_s<G, SP, UW>), dim3(xcd_grid(tiles)), dim3(kBlock),                            \
+                     staged_lds_bytes<double>(SP), s, m.nSim, m.nSlices, tiles, m.ld, W[k], b, m.sliceOff,     \
+                     m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.stageStride, m.tileBase,         \
+                     m.tileCol, u[k + 1], u[k + 2], u[k], uc)
+        if (m.sellUniformW == kChunk32) {
+          if (sp == 2) ADJ(2, kChunk32);
+          else ADJ(4, kChunk32);
+        } else {
+          if (sp == 2) ADJ(2, 0);
+          else ADJ(4, 0);
+        }
+#undef ADJ
         staged = true;
       }
     }

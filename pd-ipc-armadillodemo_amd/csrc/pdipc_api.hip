@@ -142,8 +142,17 @@ int upload_all(pdipc_handle* h) {
   UP(p2Off, L.pOff); UP(p2K, L.pK); UP(p2A, L.pA);
   AL(m2Val, m.sellEntries); AL(m2ValF, m.sellEntries);
   if (L.stageOk && h->prm.sell_stage) {
-    UP(tileOff, L.tileOff); UP(tileBase, L.tileBase); UP(tileCol, L.tileCol); UP(m2Loc, L.loc);
-    m.stageMax = L.stageMax;
+    // the tiles' column lists at one stride (the largest union rounded up to
+    // 8 entries), each padded with its last column
+    const int nTiles = (int)L.tileBase.size();
+    const int stride = (L.stageMax + 7) / 8 * 8;
+    std::vector<uint16_t> padded((size_t)nTiles * stride);
+    for (int t = 0; t < nTiles; t++) {
+      const int u0 = L.tileOff[t], nU = L.tileOff[t + 1] - u0;
+      for (int u = 0; u < stride; u++) padded[(size_t)t * stride + u] = L.tileCol[u0 + std::min(u, nU - 1)];
+    }
+    UP(tileBase, L.tileBase); UP(tileCol, padded); UP(m2Loc, L.loc);
+    m.stageStride = stride;
   }
   // collision
   m.nFaces = (int)hs.bFaces.size() / 3;
