@@ -250,6 +250,52 @@ __device__ inline int block_exclusive_scan(int v, int* sh, int* total) {
   return r;
 }
 
+// Exclusive scan of at most kScanOneMax counts in one workgroup (one launch
+// instead of three): the workgroup walks the array in coalesced tiles of
+// kScanOneBlock * kScanItems counts, carrying the running total.  In-place use
+// (in == out) is allowed: every count is read before its slot is written.
+constexpr int kScanOneBlock = 1024;
+constexpr int kScanOneMax = 1 << 15;
+__global__ void __launch_bounds__(kScanOneBlock) k_scan_one(int n, const int* in, int* out, int* __restrict__ total) {
+  __shared__ int sh[kScanOneBlock / kWave];
+  __shared__ int tileTotal;
+  const int lane = threadIdx.x % kWave, w = threadIdx.x / kWave;
+  int carry = 0;
+  for (int base = 0; base < n; base += kScanOneBlock * kScanItems) {
+    const int i0 = base + threadIdx.x * kScanItems;
+    int v[kScanItems], s = 0;
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+      v[k] = i0 + k < n ? in[i0 + k] : 0;
+      s += v[k];
+    }
+    int x = s;
+#pragma unroll
+    for (int o = 1; o < kWave; o <<= 1) {
+      int y = __shfl_up(x, o, kWave);
+      if (lane >= o) x += y;
+    }
+    if (lane == kWave - 1) sh[w] = x;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+      int c = 0;
+      for (int q = 0; q < kScanOneBlock / kWave; q++) { int t = sh[q]; sh[q] = c; c += t; }
+    }
+    __syncthreads();
+    int pre = carry + x - s + sh[w];
+#pragma unroll
+    for (int k = 0; k < kScanItems; k++) {
+      if (i0 + k < n) out[i0 + k] = pre;
+      pre += v[k];
+    }
+    if (threadIdx.x == kScanOneBlock - 1) tileTotal = pre - carry;  // the last thread's inclusive prefix
+    __syncthreads();
+    carry += tileTotal;
+    __syncthreads();
+  }
+  if (threadIdx.x == 0 && total) *total = carry;
+}
+
 __global__ void k_scan_tiles(int n, const int* __restrict__ in, int* __restrict__ out, int* __restrict__ blockSums) {
   __shared__ int sh[kBlock / kWave + 1];
   int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
@@ -388,13 +434,39 @@ __global__ void k_table_level0(int P, const int* __restrict__ sortedIdx, const d
   for (int k = 0; k < 6; k++) tab[(int64_t)k * P + i] = pbox[(int64_t)k * P + obj];
 }
 
-__global__ void k_table_level(int P, int half, const double* __restrict__ prev, double* __restrict__ cur) {
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
+// Levels j0 .. j0 + count - 1 of the sparse table in one launch, each from
+// level j0 - 1: level L at i is the union of the leaf boxes [i, i + 2^L), i.e.
+// of 2^(L - j0 + 1) consecutive entries of level j0 - 1 at stride 2^(j0 - 1),
+// folded in order (min / max are exact, so every level equals the pairwise
+// construction's).
+constexpr int kTableLevelsPerLaunch = 4;
+__global__ void k_table_levels(int P, int j0, int count, double* __restrict__ tab) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int half = 1 << (j0 - 1);
   if (i + 2 * half > P) return;
+  const double* prev = tab + (int64_t)(j0 - 1) * 6 * P;
+  double lo[3], hi[3];
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    cur[(int64_t)k * P + i] = dmin(prev[(int64_t)k * P + i], prev[(int64_t)k * P + i + half]);
-    cur[(int64_t)(3 + k) * P + i] = dmax(prev[(int64_t)(3 + k) * P + i], prev[(int64_t)(3 + k) * P + i + half]);
+    lo[k] = prev[(int64_t)k * P + i];
+    hi[k] = prev[(int64_t)(3 + k) * P + i];
+  }
+  int t = 1;  // entries of level j0 - 1 folded so far
+  for (int m = 0; m < count; m++) {
+    const int need = 2 << m;
+    if (i + (int64_t)need * half > P) break;
+    for (; t < need; t++)
+#pragma unroll
+      for (int k = 0; k < 3; k++) {
+        lo[k] = dmin(lo[k], prev[(int64_t)k * P + i + t * half]);
+        hi[k] = dmax(hi[k], prev[(int64_t)(3 + k) * P + i + t * half]);
+      }
+    double* cur = tab + (int64_t)(j0 + m) * 6 * P;
+#pragma unroll
+    for (int k = 0; k < 3; k++) {
+      cur[(int64_t)k * P + i] = lo[k];
+      cur[(int64_t)(3 + k) * P + i] = hi[k];
+    }
   }
 }
 
@@ -778,6 +850,10 @@ __global__ void __launch_bounds__(kWave) k_count_crossings(int Q, int P, int ld,
 
 void launch_exclusive_scan(int* in, int* out, int n, int* total_out, int* blockScratch, hipStream_t s) {
   if (n <= 0) return;
+  if (n <= kScanOneMax) {
+    hipLaunchKernelGGL(k_scan_one, dim3(1), dim3(kScanOneBlock), 0, s, n, in, out, total_out);
+    return;
+  }
   int nb = (n + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(kBlock), 0, s, n, in, out, blockScratch);
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, nb, blockScratch);
@@ -816,9 +892,11 @@ void launch_lbvh(DeviceMesh& m, hipStream_t s) {
                        m.nodeRight, m.nodeParent, m.nodeRange);
   hipLaunchKernelGGL(k_table_level0, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, iin, m.patchBox,
                      m.nodeTable);
-  for (int j = 1; (1 << j) <= P; j++)
-    hipLaunchKernelGGL(k_table_level, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, 1 << (j - 1),
-                       m.nodeTable + (int64_t)(j - 1) * 6 * P, m.nodeTable + (int64_t)j * 6 * P);
+  int top = 0;  // the highest level: 2^top <= P
+  while ((2 << top) <= P) top++;
+  for (int j = 1; j <= top; j += kTableLevelsPerLaunch)
+    hipLaunchKernelGGL(k_table_levels, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, j,
+                       std::min(kTableLevelsPerLaunch, top - j + 1), m.nodeTable);
   hipLaunchKernelGGL(k_node_boxes, dim3((2 * P - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, m.nodeRange,
                      m.nodeTable, m.nodeLeft, m.nodeRight, m.nodeParent, iin, m.nodes);
 }
