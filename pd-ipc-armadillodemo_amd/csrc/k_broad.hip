@@ -255,7 +255,7 @@ __device__ inline int block_exclusive_scan(int v, int* sh, int* total) {
 // kScanOneBlock * kScanItems counts, carrying the running total.  In-place use
 // (in == out) is allowed: every count is read before its slot is written.
 constexpr int kScanOneBlock = 1024;
-constexpr int kScanOneMax = 1 << 15;
+constexpr int kScanOneMax = 1 << 14;
 __global__ void __launch_bounds__(kScanOneBlock) k_scan_one(int n, const int* in, int* out, int* __restrict__ total) {
   __shared__ int sh[kScanOneBlock / kWave];
   __shared__ int tileTotal;
@@ -439,7 +439,10 @@ __global__ void k_table_level0(int P, const int* __restrict__ sortedIdx, const d
 // of 2^(L - j0 + 1) consecutive entries of level j0 - 1 at stride 2^(j0 - 1),
 // folded in order (min / max are exact, so every level equals the pairwise
 // construction's).
-constexpr int kTableLevelsPerLaunch = 4;
+// levels per launch: 4 while the table is small (the launches, not the
+// reads, set its cost), 2 beyond (the level reads then cost as in the
+// pairwise construction: 4 entries per output position for 2 levels)
+inline int table_levels_per_launch(int P) { return P < (1 << 17) ? 4 : 2; }
 __global__ void k_table_levels(int P, int j0, int count, double* __restrict__ tab) {
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int half = 1 << (j0 - 1);
@@ -894,9 +897,10 @@ void launch_lbvh(DeviceMesh& m, hipStream_t s) {
                      m.nodeTable);
   int top = 0;  // the highest level: 2^top <= P
   while ((2 << top) <= P) top++;
-  for (int j = 1; j <= top; j += kTableLevelsPerLaunch)
+  const int per = table_levels_per_launch(P);
+  for (int j = 1; j <= top; j += per)
     hipLaunchKernelGGL(k_table_levels, dim3((P + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, j,
-                       std::min(kTableLevelsPerLaunch, top - j + 1), m.nodeTable);
+                       std::min(per, top - j + 1), m.nodeTable);
   hipLaunchKernelGGL(k_node_boxes, dim3((2 * P - 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, P, m.nodeRange,
                      m.nodeTable, m.nodeLeft, m.nodeRight, m.nodeParent, iin, m.nodes);
 }

@@ -361,9 +361,11 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
 // sweep at config 1: 5.8 -> 4.9 us with the fixed-stride lists).  On a
 // mesh-ordered cloth a 32-row tile touches about 9 runs of ~40 columns (the
 // 4-ring stencil of M^2), so the L2 traffic of the gathers drops from 16 B
-// per entry to about 6 B per row.  Tiles are dealt to workgroups so that each
-// XCD (workgroups go to the 8 XCDs round-robin) runs a contiguous range of
-// tiles, whose unions overlap, sharing one L2.  Entries are summed in the same
+// per entry to about 6 B per row.  Workgroup b runs tile b (the hardware deals
+// consecutive workgroups to the 8 XCDs round-robin; giving each XCD a
+// contiguous range of tiles, whose unions overlap, measured slower in the
+// step: 5.2-5.5 -> 5.6-5.9 us at config 1, 261 -> 279 us at the 3 x 1024^2
+// scale).  Entries are summed in the same
 // order as the L2-gather kernels, so the results are bit-identical to theirs.
 // Entry offsets are 32-bit (staging requires fewer than 2^29 entries): each
 // load is a 64-bit base in scalar registers plus a 32-bit lane offset, and a
@@ -372,7 +374,6 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
 // time, so the chunk loop and its guards fold away).
 constexpr int kChunk32 = 8;  // matrix entries per lane loaded together (FP32 values)
 constexpr int kChunk64 = 4;  // (FP64 values)
-constexpr int kXcds = 8;
 
 template <class T>
 __device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t i) {
@@ -391,15 +392,6 @@ __device__ __forceinline__ void load_chunk(const V* __restrict__ val, const uint
     v[u] = c0 + u < width ? x : V(0);
   }
 }
-
-// The tile this workgroup runs (-1: a padding workgroup of the last XCDs):
-// XCD x = blockIdx % 8 takes the x-th of 8 contiguous, near-equal tile ranges.
-__device__ __forceinline__ int xcd_tile(int nTiles) {
-  const int x = (int)blockIdx.x % kXcds, k = (int)blockIdx.x / kXcds;
-  const int q = nTiles / kXcds, r = nTiles % kXcds;
-  return k < q + (x < r ? 1 : 0) ? x * q + (x < r ? x : r) + k : -1;
-}
-inline int xcd_grid(int nTiles) { return (nTiles + kXcds - 1) / kXcds * kXcds; }
 
 // LDS rows of a staged tile: (x, y) pairs then z values, SP * kBlock of each
 template <class S>
@@ -444,7 +436,7 @@ __device__ __forceinline__ void stage_tile(StagedRows<typename std::conditional<
 }
 
 template <int G, int LAST, int SP, int UW>
-__global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int nTiles, int ld, float W,
+__global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int ld, float W,
                                                         const int64_t* __restrict__ sliceOff,
                                                         const int* __restrict__ sliceW, int uw,
                                                         const int* __restrict__ perm, const uint16_t* __restrict__ loc,
@@ -456,8 +448,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int 
                                                         double* __restrict__ X) {
   extern __shared__ float2 lds32[];
   const StagedRows<float> xs(lds32, SP * kBlock);
-  const int tile = xcd_tile(nTiles);
-  if (tile < 0) return;  // the whole workgroup
+  const int tile = blockIdx.x;
   const int slice = tile * kTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const bool live = slice < nSlices;  // every lane stages and meets the barrier
   constexpr int S = kWave / G;
@@ -503,7 +494,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int 
 }
 
 template <int G, int SP, int UW>
-__global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int nTiles, int ld, double W,
+__global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int ld, double W,
                                                         const int64_t* __restrict__ sliceOff,
                                                         const int* __restrict__ sliceW, int uw,
                                                         const int* __restrict__ perm, const uint16_t* __restrict__ loc,
@@ -515,8 +506,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int 
                                                         double* __restrict__ out) {
   extern __shared__ double2 lds64[];
   const StagedRows<double> xs(lds64, SP * kBlock);
-  const int tile = xcd_tile(nTiles);
-  if (tile < 0) return;
+  const int tile = blockIdx.x;
   const int slice = tile * kTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const bool live = slice < nSlices;
   constexpr int S = kWave / G;
@@ -570,8 +560,8 @@ void launch_sweep32s(const DeviceMesh& m, float W, const float4* cur, const floa
   const int tiles = tiles_of(m);
   const int sp = stage_per_lane(m.stageStride);
 #define SW32(SP, UW)                                                                                               \
-  launch_t(m.timer, TC_SWEEP32, k_sweep32s<G, LAST, SP, UW>, dim3(xcd_grid(tiles)), dim3(kBlock),                  \
-           (uint32_t)staged_lds_bytes<float>(SP), s, m.nSim, m.nSlices, tiles, m.ld, W, m.sliceOff, m.sliceW,       \
+  launch_t(m.timer, TC_SWEEP32, k_sweep32s<G, LAST, SP, UW>, dim3(tiles), dim3(kBlock),                  \
+           (uint32_t)staged_lds_bytes<float>(SP), s, m.nSim, m.nSlices, m.ld, W, m.sliceOff, m.sliceW,       \
            m.sellUniformW, m.sellPerm, m.m2Loc, m.m2ValF, m.stageStride, m.tileBase, m.tileCol, cur, pre, m.r0, next, \
            m.X)
   if (m.sellUniformW == kChunk32) {
@@ -590,8 +580,8 @@ void launch_sweep64s(const DeviceMesh& m, double W, const Row64* cur, const Row6
   const int tiles = tiles_of(m);
   const int sp = stage_per_lane(m.stageStride);
 #define SW64(SP, UW)                                                                                               \
-  launch_t(m.timer, TC_SWEEP64, k_sweep64s<G, SP, UW>, dim3(xcd_grid(tiles)), dim3(kBlock),                        \
-           (uint32_t)staged_lds_bytes<double>(SP), s, m.nSim, m.nSlices, tiles, m.ld, W, m.sliceOff, m.sliceW,      \
+  launch_t(m.timer, TC_SWEEP64, k_sweep64s<G, SP, UW>, dim3(tiles), dim3(kBlock),                        \
+           (uint32_t)staged_lds_bytes<double>(SP), s, m.nSim, m.nSlices, m.ld, W, m.sliceOff, m.sliceW,      \
            m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.stageStride, m.tileBase, m.tileCol, cur, pre, m.cA, next, \
            out)
   if (m.sellUniformW == kChunk32) {
@@ -818,7 +808,7 @@ __global__ void __launch_bounds__(kGbarThreads) k_adj_gbar(int ld, const int64_t
 // The reverse sweep with its operand staged in LDS (as k_sweep64s; the
 // reverse iterates are SoA, so a staged row is three loads)
 template <int G, int SP, int UW>
-__global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, int nTiles, int ld, double a, double b,
+__global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, int ld, double a, double b,
                                                           const int64_t* __restrict__ sliceOff,
                                                           const int* __restrict__ sliceW, int uw,
                                                           const int* __restrict__ perm,
@@ -831,8 +821,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, in
                                                           double* __restrict__ uc) {
   extern __shared__ double2 ldsAdj[];
   const StagedRows<double> us(ldsAdj, SP * kBlock);
-  const int tile = xcd_tile(nTiles);
-  if (tile < 0) return;
+  const int tile = blockIdx.x;
   const int slice = tile * kTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
   const bool live = slice < nSlices;
   constexpr int S = kWave / G;
@@ -894,8 +883,8 @@ void adj_pass_impl(const DeviceMesh& m, const AdjMesh& a, const double* W, int K
         const int tiles = tiles_of(m);
         const int sp = stage_per_lane(m.stageStride);
 #define ADJ(SP, UW)                                                                                             \
-  hipLaunchKernelGGL((k_adj_spmv_s<G, SP, UW>), dim3(xcd_grid(tiles)), dim3(kBlock),                            \
-                     staged_lds_bytes<double>(SP), s, m.nSim, m.nSlices, tiles, m.ld, W[k], b, m.sliceOff,     \
+  hipLaunchKernelGGL((k_adj_spmv_s<G, SP, UW>), dim3(tiles), dim3(kBlock),                            \
+                     staged_lds_bytes<double>(SP), s, m.nSim, m.nSlices, m.ld, W[k], b, m.sliceOff,     \
                      m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.stageStride, m.tileBase,         \
                      m.tileCol, u[k + 1], u[k + 2], u[k], uc)
         if (m.sellUniformW == kChunk32) {
