@@ -41,10 +41,10 @@ HBM_PEAK_GBS = 8000.0
 # Bytes one FP32 sweep launch moves past L2 at the default workload, from two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
 # a plain 1 GiB copy gives): profiles/k_sweep_config1_hbm_traffic.md.
-SWEEP32_TRAFFIC_BYTES = {(256, 6, 8): 31.06e6}
+SWEEP32_TRAFFIC_BYTES = {(256, 6, 8): 31.63e6}
 # the same for the three stacked 1024^2 cloths at 8 lanes per row:
 # profiles/config4_stacked_kernel_stats.md
-STACKED_SWEEP32_TRAFFIC_BYTES = {8: 1524.5e6}
+STACKED_SWEEP32_TRAFFIC_BYTES = {8: 1518.6e6}
 # steps of each scene's run from rest; by default the warm-up steps and the
 # timed steps together make up the whole run
 RUN_STEPS = {"cloth": 200, "config0": 100, "tet_block": 100, "stacked": 50}
