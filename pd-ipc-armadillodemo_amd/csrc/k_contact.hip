@@ -108,20 +108,26 @@ __host__ __device__ inline T pt_d2(const P3T<T>& xi, const P3T<T>& xa, const P3T
 
 template <class T> __host__ __device__ inline T mn2(T a, T b) { return a < b ? a : b; }
 template <class T> __host__ __device__ inline T mx2(T a, T b) { return a > b ? a : b; }
-__host__ __device__ inline double mn3(double a, double b, double c) { return a < b ? (a < c ? a : c) : (b < c ? b : c); }
-__host__ __device__ inline double mx3(double a, double b, double c) { return a > b ? (a > c ? a : c) : (b > c ? b : c); }
+// Bounds of the swept-box and position filters as fmin / fmax (one FP64
+// instruction each instead of a compare and two selects): for finite
+// coordinates they pick the same values as the reference's ternary min2 /
+// max2 (DataCore.h:32), so every filter decision is the same.
+__host__ __device__ inline double bmn(double a, double b) { return fmin(a, b); }
+__host__ __device__ inline double bmx(double a, double b) { return fmax(a, b); }
+__host__ __device__ inline double bmn3(double a, double b, double c) { return fmin(fmin(a, b), c); }
+__host__ __device__ inline double bmx3(double a, double b, double c) { return fmax(fmax(a, b), c); }
 
 __host__ __device__ inline bool pt_ccd_filter(const P3* x, const P3* dx, double dist) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    double tmin = mn3(x[1].v[k], x[2].v[k], x[3].v[k]);
-    tmin = mn3(tmin, x[1].v[k] + dx[1].v[k], x[2].v[k] + dx[2].v[k]);
-    tmin = mn2(tmin, x[3].v[k] + dx[3].v[k]);
-    double tmax = mx3(x[1].v[k], x[2].v[k], x[3].v[k]);
-    tmax = mx3(tmax, x[1].v[k] + dx[1].v[k], x[2].v[k] + dx[2].v[k]);
-    tmax = mx2(tmax, x[3].v[k] + dx[3].v[k]);
+    double tmin = bmn3(x[1].v[k], x[2].v[k], x[3].v[k]);
+    tmin = bmn3(tmin, x[1].v[k] + dx[1].v[k], x[2].v[k] + dx[2].v[k]);
+    tmin = bmn(tmin, x[3].v[k] + dx[3].v[k]);
+    double tmax = bmx3(x[1].v[k], x[2].v[k], x[3].v[k]);
+    tmax = bmx3(tmax, x[1].v[k] + dx[1].v[k], x[2].v[k] + dx[2].v[k]);
+    tmax = bmx(tmax, x[3].v[k] + dx[3].v[k]);
     double p0 = x[0].v[k], p1 = x[0].v[k] + dx[0].v[k];
-    double pmin = mn2(p0, p1), pmax = mx2(p0, p1);
+    double pmin = bmn(p0, p1), pmax = bmx(p0, p1);
     if ((pmin - tmax) > dist || (tmin - pmax) > dist) return false;
   }
   return true;
@@ -129,18 +135,18 @@ __host__ __device__ inline bool pt_ccd_filter(const P3* x, const P3* dx, double 
 __host__ __device__ inline bool ee_ccd_filter(const P3* x, const P3* dx, double dist) {
 #pragma unroll
   for (int k = 0; k < 3; k++) {
-    double mina = mn2(x[0].v[k], x[0].v[k] + dx[0].v[k]);
-    mina = mn2(mina, x[1].v[k]);
-    mina = mn2(mina, x[1].v[k] + dx[1].v[k]);
-    double maxa = mx2(x[0].v[k], x[0].v[k] + dx[0].v[k]);
-    maxa = mx2(maxa, x[1].v[k]);
-    maxa = mx2(maxa, x[1].v[k] + dx[1].v[k]);
-    double minb = mn2(x[2].v[k], x[2].v[k] + dx[2].v[k]);
-    minb = mn2(minb, x[3].v[k]);
-    minb = mn2(minb, x[3].v[k] + dx[3].v[k]);
-    double maxb = mx2(x[2].v[k], x[2].v[k] + dx[2].v[k]);
-    maxb = mx2(maxb, x[3].v[k]);
-    maxb = mx2(maxb, x[3].v[k] + dx[3].v[k]);
+    double mina = bmn(x[0].v[k], x[0].v[k] + dx[0].v[k]);
+    mina = bmn(mina, x[1].v[k]);
+    mina = bmn(mina, x[1].v[k] + dx[1].v[k]);
+    double maxa = bmx(x[0].v[k], x[0].v[k] + dx[0].v[k]);
+    maxa = bmx(maxa, x[1].v[k]);
+    maxa = bmx(maxa, x[1].v[k] + dx[1].v[k]);
+    double minb = bmn(x[2].v[k], x[2].v[k] + dx[2].v[k]);
+    minb = bmn(minb, x[3].v[k]);
+    minb = bmn(minb, x[3].v[k] + dx[3].v[k]);
+    double maxb = bmx(x[2].v[k], x[2].v[k] + dx[2].v[k]);
+    maxb = bmx(maxb, x[3].v[k]);
+    maxb = bmx(maxb, x[3].v[k] + dx[3].v[k]);
     if ((mina - maxb) > dist || (minb - maxa) > dist) return false;
   }
   return true;
@@ -332,14 +338,14 @@ __host__ __device__ inline bool cd_filter(int type, const P3* x, double dHat) {
   if (type == 0) {
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      double tmin = mn3(x[1].v[k], x[2].v[k], x[3].v[k]), tmax = mx3(x[1].v[k], x[2].v[k], x[3].v[k]);
+      double tmin = bmn3(x[1].v[k], x[2].v[k], x[3].v[k]), tmax = bmx3(x[1].v[k], x[2].v[k], x[3].v[k]);
       if ((x[0].v[k] - tmax) > dHat || (tmin - x[0].v[k]) > dHat) pass = false;
     }
   } else {
 #pragma unroll
     for (int k = 0; k < 3; k++) {
-      double mina = mn2(x[0].v[k], x[1].v[k]), maxa = mx2(x[0].v[k], x[1].v[k]);
-      double minb = mn2(x[2].v[k], x[3].v[k]), maxb = mx2(x[2].v[k], x[3].v[k]);
+      double mina = bmn(x[0].v[k], x[1].v[k]), maxa = bmx(x[0].v[k], x[1].v[k]);
+      double minb = bmn(x[2].v[k], x[3].v[k]), maxb = bmx(x[2].v[k], x[3].v[k]);
       if ((mina - maxb) > dHat || (minb - maxa) > dHat) pass = false;
     }
   }

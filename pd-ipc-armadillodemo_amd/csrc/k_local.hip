@@ -225,11 +225,15 @@ struct ResidualArgs {
 
 // kRhsLanes lanes per point: ring slots, tet corners and (RES) R1 entries are
 // dealt round-robin and the partial sums combined by a fixed shuffle tree.
-// A lane fetches kRhsBatch of its tet corners (R1 entries) at a time, all
+// A lane fetches kRhsBatch of its tet corners (kResBatch R1 entries) at a time, all
 // loads first, then adds them in order, so that several memory round trips
 // are in flight per lane; the sums are those of the one-at-a-time loop.
 constexpr int kRhsLanes = 4;
 constexpr int kRhsBatch = 4;
+// R1 entries per lane fetched together in the residual pass (RES): a cloth row
+// has about 19, so one batch of 6 over the 4 lanes covers it (14.5 -> 13.7 us
+// for k_rhs<float, true> at 65,536 points against batches of 4)
+constexpr int kResBatch = 6;
 
 template <class V>
 __device__ inline V rhs_lanes_sum(V v) {
@@ -329,21 +333,21 @@ __global__ void k_rhs(int nSim, int ld, int nT, int withCol, ResidualArgs ra, co
   }
   double sx[3] = {0, 0, 0};
   if (RES) {
-    for (int q = r0 + sub; q < r1; q += kRhsBatch * kRhsLanes) {
-      int j[kRhsBatch];
-      double a[kRhsBatch], xv[kRhsBatch][3];
+    for (int q = r0 + sub; q < r1; q += kResBatch * kRhsLanes) {
+      int j[kResBatch];
+      double a[kResBatch], xv[kResBatch][3];
 #pragma unroll
-      for (int b = 0; b < kRhsBatch; b++) {
+      for (int b = 0; b < kResBatch; b++) {
         const bool in = q + b * kRhsLanes < r1;
         j[b] = in ? ra.col[q + b * kRhsLanes] : ii;
         a[b] = in ? ra.val[q + b * kRhsLanes] : 0.0;
       }
 #pragma unroll
-      for (int b = 0; b < kRhsBatch; b++)
+      for (int b = 0; b < kResBatch; b++)
 #pragma unroll
         for (int k = 0; k < 3; k++) xv[b][k] = X[k * ld + j[b]];
 #pragma unroll
-      for (int b = 0; b < kRhsBatch; b++)
+      for (int b = 0; b < kResBatch; b++)
         if (q + b * kRhsLanes < r1)
 #pragma unroll
           for (int k = 0; k < 3; k++) sx[k] += a[b] * xv[b][k];
