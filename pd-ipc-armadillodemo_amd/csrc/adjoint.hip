@@ -395,8 +395,7 @@ int taped_step(pdipc_handle* h, StepTape& st) {
         if ((rc = dalloc(A, &A->recToi, (size_t)A->recToiCap))) return rc;
       }
     }
-    launch_toi_init(m, s);
-    HIPCHK(hipMemsetAsync(m.counters + 4, 0x7f, sizeof(int), s));
+    HIPCHK(hipMemsetAsync(m.counters + 4, 0x7f, sizeof(int), s));  // (launch_direction reset the TOI minimum)
     launch_ccd_records(m, m.recs, h->lastRecs, m.Xold, m.X, p.ccd_filter, p.ccd_iters, p.ccd_step, p.ccd_gap,
                        p.touch_policy, A->recToi, m.counters + 4, s);
     launch_clamp_residual(m, p.toi_scale, s);

@@ -201,7 +201,7 @@ struct DeviceMesh {
 // k_step.hip
 void launch_predict(const DeviceMesh& m, double dt, hipStream_t s);
 void launch_diag(const DeviceMesh& m, int withCollision, hipStream_t s);
-void launch_direction(const DeviceMesh& m, hipStream_t s);
+void launch_direction(const DeviceMesh& m, hipStream_t s);  // also sets the TOI minimum to 1
 void launch_clamp_residual(const DeviceMesh& m, double toiScale, hipStream_t s);
 void launch_velocity(const DeviceMesh& m, double dt, hipStream_t s);
 void launch_zero(double* p, int64_t n, hipStream_t s);

@@ -458,6 +458,20 @@ __global__ void __launch_bounds__(kBlock) k_col_terms(int nRecs, int ld, int nSi
   }
 }
 
+// zeroes the pass's counters (counters[2]: active records, counters[5]: list
+// overflow) and the per-point counts and cursors: one launch instead of four
+// memsets
+__global__ void k_col_clear(int nSim, int* __restrict__ counters, int* __restrict__ ptCount,
+                            int* __restrict__ ptCursor) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) {
+    counters[2] = 0;
+    counters[5] = 0;
+  }
+  if (i <= nSim) ptCount[i] = 0;
+  if (i < nSim) ptCursor[i] = 0;
+}
+
 __global__ void k_col_fill(int ld, int nSim, int actCap, const int4* __restrict__ recs, const int* __restrict__ counters,
                            const int* __restrict__ actRec, const int* __restrict__ ptOff, int* __restrict__ ptCursor,
                            unsigned long long* __restrict__ entKey, int* __restrict__ entA, int* __restrict__ entPt) {
@@ -747,9 +761,8 @@ void launch_adj_ccd(const DeviceMesh& m, const int4* recs, int argmin, const dou
 }
 
 int launch_collision(DeviceMesh& m, int nRecs, double dhat, double kappa, double dt, hipStream_t s) {
-  // counters[2] (active records) and counters[5] (list overflow) are cleared by the caller
-  hipMemsetAsync(m.ptCount, 0, ((size_t)m.nSim + 1) * sizeof(int), s);
-  hipMemsetAsync(m.ptCursor, 0, (size_t)m.nSim * sizeof(int), s);
+  hipLaunchKernelGGL(k_col_clear, dim3((m.nSim + 1 + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.counters,
+                     m.ptCount, m.ptCursor);
   if (nRecs > 0)
     launch_t(m.timer, TC_BARRIER, k_col_terms, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nRecs, m.ld,
              m.nSim, m.recs, m.X, m.colMass, dhat, kappa, dt, m.actCap, m.actRec, m.actW, m.actT, m.ptCount,

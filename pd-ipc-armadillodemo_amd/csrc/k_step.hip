@@ -1,6 +1,8 @@
 // Per-point kernels of the step driver: prediction, system diagonal, search
 // direction, TOI clamp with the residual reduction, velocity update.
 // HBM-bound streaming kernels, one lane per point, SoA loads (coalesced).
+#include <algorithm>
+
 #include "device.hpp"
 
 namespace pdipc {
@@ -69,17 +71,21 @@ void launch_diag(const DeviceMesh& m, int withCollision, hipStream_t s) {
                      m.colDiag, withCollision, m.sysDiag, m.dStale, m.dChanged);
 }
 
-// computeDirectionKernel (PdIpcGpuFunc.cu:185-200)
+// computeDirectionKernel (PdIpcGpuFunc.cu:185-200); thread 0 also resets the
+// device-resident TOI minimum to 1 for the CCD pass that follows (a launch
+// fewer per outer iteration)
 __global__ void k_direction(int n, int ld, const double* __restrict__ X, const double* __restrict__ Xold,
-                            double* __restrict__ D) {
+                            double* __restrict__ D, unsigned long long* __restrict__ toiBits) {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i == 0) *toiBits = (unsigned long long)__double_as_longlong(1.0);
   if (i >= n) return;
 #pragma unroll
   for (int c = 0; c < 3; c++) D[c * ld + i] = X[c * ld + i] - Xold[c * ld + i];
 }
 
 void launch_direction(const DeviceMesh& m, hipStream_t s) {
-  hipLaunchKernelGGL(k_direction, dim3(grid_for(m.nSim)), dim3(kBlock), 0, s, m.nSim, m.ld, m.X, m.Xold, m.Dir);
+  hipLaunchKernelGGL(k_direction, dim3(grid_for(std::max(1, m.nSim))), dim3(kBlock), 0, s, m.nSim, m.ld, m.X, m.Xold,
+                     m.Dir, m.toiBits);
 }
 
 // clampDirectionKernel (PdIpcGpuFunc.cu:222-238) reading the device-resident TOI,

@@ -328,9 +328,7 @@ int collision(pdipc_handle* h) {
   hipStream_t s = h->stream;
   int rc;
   for (int attempt = 0; attempt < 2; attempt++) {
-    HIPCHK(hipMemsetAsync(m.counters + 2, 0, sizeof(int), s));
-    HIPCHK(hipMemsetAsync(m.counters + 5, 0, sizeof(int), s));
-    launch_collision(m, h->lastRecs, p.dhat, p.kappa, p.dt, s);
+    launch_collision(m, h->lastRecs, p.dhat, p.kappa, p.dt, s);  // clears its counters first
     if ((rc = check_launch(h))) return rc;
     if ((rc = read_scalars(h))) return rc;
     if (!h->hCounters()[5]) return PDIPC_OK;
@@ -410,8 +408,7 @@ int advance(pdipc_handle* h) {
       if ((rc = broad_phase(h, m.Xold, m.Dir))) return rc;
       if (b1) HIPCHK(hipEventRecord(b1, s));
     }
-    launch_toi_init(m, s);
-    const bool track = p.track_toi_argmin && h->lastRecs > 0;
+    const bool track = p.track_toi_argmin && h->lastRecs > 0;  // (launch_direction reset the TOI minimum)
     if (track) {  // per-record TOIs, then the first record attaining the minimum
       if (h->trackCap < h->lastRecs) {
         h->release(h->trackToi);
