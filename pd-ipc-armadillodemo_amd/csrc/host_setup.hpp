@@ -100,8 +100,12 @@ struct SellLayout {
   // kSellSortSlices slices; slots past the last row map to themselves) and its
   // inverse over the rows; both empty for the identity order
   std::vector<int> perm, slotOf;
+  // slice s holds 64 lanes x sliceW[s] entries, lane-major: entry `chunk` of
+  // lane l sits at sliceOff[s] + l * sliceW[s] + chunk (a lane's entries are
+  // consecutive); lane l serves slot-row l / lanes, whose CSR entry c is held
+  // by lane (l / lanes) * lanes + c % lanes as its chunk c / lanes
   std::vector<int64_t> sliceOff;   // nSlices + 1
-  std::vector<int> sliceW;         // nSlices: width in 64-entry columns
+  std::vector<int> sliceW;         // nSlices: entries per lane
   std::vector<int> col;            // per SELL entry
   // per SELL entry: col - the entry's slot index (its row without perm; padding
   // rows included);

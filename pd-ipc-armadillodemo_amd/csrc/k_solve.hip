@@ -104,11 +104,11 @@ __global__ void __launch_bounds__(kBlock, 8) k_first64(int n, int nSlices, int l
   const int row = perm ? perm[slot] : slot;
   int sub = lane % G;
   const int width = uw > 0 ? uw : sliceW[slice];
-  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (int64_t)lane * width;
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
-    int64_t e = off + (int64_t)c * kWave;
+    int64_t e = off + c;
     int j = C16 ? slot + (int)col16[e] : col[e];
     double v = val[e];
     acc[0] += v * X[j];
@@ -234,7 +234,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
   const int slot = slice * S + lane / G;  // < ld: slices cover at most ld rows
   const int row = perm ? perm[slot] : slot;
   const int width = uw > 0 ? uw : sliceW[slice];
-  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (int64_t)lane * width;
   Row64 p = pre[row], c = cA[row];  // loaded up front, with the matrix
   double acc[3] = {0.0, 0.0, 0.0};
   for (int c0 = 0; c0 < width; c0 += U) {
@@ -245,7 +245,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
       j[u] = row;
       v[u] = 0.0;
       if (c0 + u < width) {
-        int64_t e = off + (int64_t)(c0 + u) * kWave;
+        int64_t e = off + (c0 + u);
         j[u] = C16 ? slot + (int)col16[e] : col[e];
         v[u] = val[e];
       }
@@ -302,7 +302,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
   const int slot = slice * S + lane / G;  // < ld: slices cover at most ld rows
   const int row = perm ? perm[slot] : slot;
   const int width = uw > 0 ? uw : sliceW[slice];
-  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (int64_t)lane * width;
   // the row's own operands, loaded up front so they arrive with the matrix
   // (the last sweep of a solve also adds into X: its X too)
   float4 p = dpre[row], r = r0[row];
@@ -321,7 +321,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
       j[u] = row;
       v[u] = 0.0f;
       if (c0 + u < width) {
-        int64_t e = off + (int64_t)(c0 + u) * kWave;
+        int64_t e = off + (c0 + u);
         j[u] = C16 ? slot + (int)col16[e] : col[e];
         v[u] = val[e];
       }
@@ -381,16 +381,47 @@ __device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t i) {
 }
 
 template <int CH, class V>
-__device__ __forceinline__ void load_chunk(const V* __restrict__ val, const uint16_t* __restrict__ loc, uint32_t off,
-                                           int c0, int width, V (&v)[CH], int (&li)[CH]) {
+__device__ __forceinline__ void load_chunk_any(const V* __restrict__ val, const uint16_t* __restrict__ loc,
+                                               uint32_t off, int c0, int width, V (&v)[CH], int (&li)[CH]) {
 #pragma unroll
   for (int u = 0; u < CH; u++) {
     const int c = c0 + u < width ? c0 + u : width - 1;
-    const uint32_t e = off + (uint32_t)(c * kWave);
+    const uint32_t e = off + (uint32_t)c;
     const V x = ld32(val, e);
     li[u] = ld32(loc, e);
     v[u] = c0 + u < width ? x : V(0);
   }
+}
+
+// Every slice of width 8: a lane's entries are 8 consecutive, 8-aligned slots,
+// so a chunk is two 16-byte value loads and one index load (FP32: 8 values,
+// one 16-byte index load; FP64: 4 values, one 8-byte index load).
+template <int CH, class V>
+__device__ __forceinline__ void load_chunk_w8(const V* __restrict__ val, const uint16_t* __restrict__ loc,
+                                              uint32_t off, int c0, V (&v)[CH], int (&li)[CH]) {
+  const uint32_t e = off + (uint32_t)c0;
+  if constexpr (sizeof(V) == 4) {
+    static_assert(CH == 8, "FP32 chunks of 8 entries");
+    const float4 a = ld32((const float4*)val, e / 4), b = ld32((const float4*)val, e / 4 + 1);
+    const uint4 l = ld32((const uint4*)loc, e / 8);
+    v[0] = a.x; v[1] = a.y; v[2] = a.z; v[3] = a.w;
+    v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
+    li[0] = l.x & 0xffff; li[1] = l.x >> 16; li[2] = l.y & 0xffff; li[3] = l.y >> 16;
+    li[4] = l.z & 0xffff; li[5] = l.z >> 16; li[6] = l.w & 0xffff; li[7] = l.w >> 16;
+  } else {
+    static_assert(CH == 4, "FP64 chunks of 4 entries");
+    const double2 a = ld32((const double2*)val, e / 2), b = ld32((const double2*)val, e / 2 + 1);
+    const uint2 l = ld32((const uint2*)loc, e / 4);
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    li[0] = l.x & 0xffff; li[1] = l.x >> 16; li[2] = l.y & 0xffff; li[3] = l.y >> 16;
+  }
+}
+
+template <int UW, int CH, class V>
+__device__ __forceinline__ void load_chunk(const V* __restrict__ val, const uint16_t* __restrict__ loc, uint32_t off,
+                                           int c0, int width, V (&v)[CH], int (&li)[CH]) {
+  if constexpr (UW == 8) load_chunk_w8(val, loc, off, c0, v, li);
+  else load_chunk_any(val, loc, off, c0, width, v, li);
 }
 
 // LDS rows of a staged tile: (x, y) pairs then z values, SP * kBlock of each
@@ -456,10 +487,10 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int 
   const int slot = sl * S + lane / G;
   const int row = perm ? perm[slot] : slot;
   const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
-  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + lane;
+  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + (uint32_t)(lane * width);
   float v[kChunk32];
   int li[kChunk32];
-  load_chunk(val, loc, off, 0, width, v, li);
+  load_chunk<UW>(val, loc, off, 0, width, v, li);
   const float4 p = ld32(dpre, (uint32_t)row), r = ld32(r0, (uint32_t)row);
   stage_tile<SP>(xs, dcur, tile, stride, tileBase, tileCol);
   __syncthreads();
@@ -477,7 +508,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int 
     }
     c0 += kChunk32;
     if (c0 >= width) break;
-    load_chunk(val, loc, off, c0, width, v, li);
+    load_chunk<UW>(val, loc, off, c0, width, v, li);
   }
   group_reduce<G>(acc);
   if ((lane % G) != 0 || row >= n) return;
@@ -514,10 +545,10 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int 
   const int slot = sl * S + lane / G;
   const int row = perm ? perm[slot] : slot;
   const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
-  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + lane;
+  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + (uint32_t)(lane * width);
   double v[kChunk64];
   int li[kChunk64];
-  load_chunk(val, loc, off, 0, width, v, li);
+  load_chunk<UW>(val, loc, off, 0, width, v, li);
   stage_tile<SP>(xs, cur, tile, stride, tileBase, tileCol);
   __syncthreads();
   if (!live) return;
@@ -534,7 +565,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int 
     }
     c0 += kChunk64;
     if (c0 >= width) break;
-    load_chunk(val, loc, off, c0, width, v, li);
+    load_chunk<UW>(val, loc, off, c0, width, v, li);
   }
   group_reduce<G>(acc);
   if ((lane % G) != 0 || row >= n) return;
@@ -712,11 +743,11 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv(int n, int nSlices, int 
   const int row = perm ? perm[slot] : slot;
   int sub = lane % G;
   const int width = uw > 0 ? uw : sliceW[slice];
-  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + lane;
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (int64_t)lane * width;
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
-    int64_t e = off + (int64_t)c * kWave;
+    int64_t e = off + c;
     int j = C16 ? slot + (int)col16[e] : col[e];
     double v = val[e];
     acc[0] += v * uin[j];
@@ -829,10 +860,10 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, in
   const int slot = sl * S + lane / G;
   const int row = perm ? perm[slot] : slot;
   const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
-  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + lane;
+  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + (uint32_t)(lane * width);
   double v[kChunk64];
   int li[kChunk64];
-  load_chunk(val, loc, off, 0, width, v, li);
+  load_chunk<UW>(val, loc, off, 0, width, v, li);
   {
     int c[SP];
     tile_columns<SP>(tile, stride, tileBase, tileCol, c);
@@ -859,7 +890,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, in
     }
     c0 += kChunk64;
     if (c0 >= width) break;
-    load_chunk(val, loc, off, c0, width, v, li);
+    load_chunk<UW>(val, loc, off, c0, width, v, li);
   }
   group_reduce<G>(acc);
   if ((lane % G) != 0 || row >= n) return;
@@ -988,12 +1019,13 @@ __global__ void __launch_bounds__(kBlock) k_adj_ops_rows(int n, const int64_t* _
   constexpr int S = kWave / G;
   const int slot = slotOf ? slotOf[row] : row;
   int slice = slot / S, lr = slot % S;
-  int64_t base = sliceOff[slice] + lr * G;
-  int cnt = sliceW[slice] * G;
+  const int64_t base = sliceOff[slice];
+  const int width = sliceW[slice];
+  int cnt = width * G;
   const double w2 = w * w, c1 = 2.0 * (1.0 - w) * w;
   double acc = 0.0;
   for (int t = lane; t < cnt; t += kWave) {
-    int64_t e = base + (int64_t)(t / G) * kWave + (t % G);
+    int64_t e = base + (int64_t)(lr * G + t % G) * width + t / G;  // entry t of the row: lane lr*G + t%G, chunk t/G
     double g = Gbar[e];
     if (g == 0.0) continue;
     double sp = 0.0;

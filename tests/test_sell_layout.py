@@ -53,9 +53,10 @@ def test_sell_layout_holds_every_entry_once(which, lanes):
 
     e = np.arange(len(col))
     sl = np.searchsorted(off, e, side="right") - 1
-    lane_in = (e - off[sl]) % 64
+    lane_in = (e - off[sl]) // w[sl]  # lane-major slices: a lane's entries are consecutive
+    chunk = (e - off[sl]) % w[sl]
     slot = sl * S + lane_in // lanes
-    q = ((e - off[sl]) // 64) * lanes + lane_in % lanes  # position in the row's CSR order
+    q = chunk * lanes + lane_in % lanes  # position in the row's CSR order
     row = order[slot]
     rr = np.where(row < n, row, n - 1)
     diag = flag < 0
