@@ -7,9 +7,10 @@
 //   x_{k+1} = W_k (M^2 x_k + c - x_{k-1}) + x_{k-1}         (:1185-1242)
 //   with r = rho^2, W_1 = 2/(2-r), W_{k+1} = 4/(4 - r W_k)  (:1318-1340)
 //
-// M^2 is stored in a SELL layout (slices of 64/G rows, G lanes per row, every
-// 64-entry column of a slice loaded by one wavefront), so the matrix stream --
-// the dominant HBM traffic of the whole step -- is perfectly coalesced.  Each
+// M^2 is stored in a SELL layout (slices of 64/G rows, G lanes per row, one
+// wavefront per slice, each lane's entries consecutive), so the matrix stream
+// -- the dominant HBM traffic of the whole step -- is read in contiguous
+// wavefront-wide blocks (16-byte loads per lane at the common width 8).  Each
 // matrix row is read once for all three coordinates (the reference re-reads it
 // per coordinate).  When all slices have one width uw (set-up pads them when
 // that costs at most 2% more slots) a sweep computes its slice's offset
