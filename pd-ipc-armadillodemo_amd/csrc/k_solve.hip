@@ -409,8 +409,16 @@ __device__ __forceinline__ void load_chunk_w8(const V* __restrict__ val, const u
     v[4] = b.x; v[5] = b.y; v[6] = b.z; v[7] = b.w;
     li[0] = l.x & 0xffff; li[1] = l.x >> 16; li[2] = l.y & 0xffff; li[3] = l.y >> 16;
     li[4] = l.z & 0xffff; li[5] = l.z >> 16; li[6] = l.w & 0xffff; li[7] = l.w >> 16;
+  } else if constexpr (CH == 8) {
+    const double2 a = ld32((const double2*)val, e / 2), b = ld32((const double2*)val, e / 2 + 1);
+    const double2 c = ld32((const double2*)val, e / 2 + 2), d = ld32((const double2*)val, e / 2 + 3);
+    const uint4 l = ld32((const uint4*)loc, e / 8);
+    v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
+    v[4] = c.x; v[5] = c.y; v[6] = d.x; v[7] = d.y;
+    li[0] = l.x & 0xffff; li[1] = l.x >> 16; li[2] = l.y & 0xffff; li[3] = l.y >> 16;
+    li[4] = l.z & 0xffff; li[5] = l.z >> 16; li[6] = l.w & 0xffff; li[7] = l.w >> 16;
   } else {
-    static_assert(CH == 4, "FP64 chunks of 4 entries");
+    static_assert(CH == 4, "FP64 chunks of 4 or 8 entries");
     const double2 a = ld32((const double2*)val, e / 2), b = ld32((const double2*)val, e / 2 + 1);
     const uint2 l = ld32((const uint2*)loc, e / 4);
     v[0] = a.x; v[1] = a.y; v[2] = b.x; v[3] = b.y;
@@ -547,8 +555,9 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int 
   const int row = perm ? perm[slot] : slot;
   const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
   const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + (uint32_t)(lane * width);
-  double v[kChunk64];
-  int li[kChunk64];
+  constexpr int CH = UW == 8 ? 8 : kChunk64;  // width 8: the lane's 8 entries in one chunk
+  double v[CH];
+  int li[CH];
   load_chunk<UW>(val, loc, off, 0, width, v, li);
   stage_tile<SP>(xs, cur, tile, stride, tileBase, tileCol);
   __syncthreads();
@@ -557,14 +566,14 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int 
 #pragma unroll 1
   for (int c0 = 0;;) {
 #pragma unroll
-    for (int u = 0; u < kChunk64; u++) {
+    for (int u = 0; u < CH; u++) {
       const double2 a = xs.xy[li[u]];
       const double z = xs.z[li[u]];
       acc[0] += v[u] * a.x;
       acc[1] += v[u] * a.y;
       acc[2] += v[u] * z;
     }
-    c0 += kChunk64;
+    c0 += CH;
     if (c0 >= width) break;
     load_chunk<UW>(val, loc, off, c0, width, v, li);
   }
@@ -862,8 +871,9 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, in
   const int row = perm ? perm[slot] : slot;
   const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
   const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + (uint32_t)(lane * width);
-  double v[kChunk64];
-  int li[kChunk64];
+  constexpr int CH = UW == 8 ? 8 : kChunk64;
+  double v[CH];
+  int li[CH];
   load_chunk<UW>(val, loc, off, 0, width, v, li);
   {
     int c[SP];
@@ -882,14 +892,14 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, in
 #pragma unroll 1
   for (int c0 = 0;;) {
 #pragma unroll
-    for (int u = 0; u < kChunk64; u++) {
+    for (int u = 0; u < CH; u++) {
       const double2 xy = us.xy[li[u]];
       const double z = us.z[li[u]];
       acc[0] += v[u] * xy.x;
       acc[1] += v[u] * xy.y;
       acc[2] += v[u] * z;
     }
-    c0 += kChunk64;
+    c0 += CH;
     if (c0 >= width) break;
     load_chunk<UW>(val, loc, off, c0, width, v, li);
   }
