@@ -710,6 +710,15 @@ __global__ void k_rec_count(int Q, const int2* __restrict__ pairs, const int* __
   cntEE[i] = ea * eb;
 }
 
+// Both record-fill kernels write the same records in the same slots: one lane
+// per record while the pairs are few (456 K pairs, 5.5 M records at config 1:
+// 567 us per broad phase against 580 with one lane per pair, too few lanes to
+// hide the pair's list loads), one lane per pair from kRecFillPairsMin pairs
+// (18 M pairs, 265 M records at the 3 x 1024^2 scale: 22.0 -> 19.2 ms per broad
+// phase, each pair's offsets and lists read once instead of per record; at
+// config 1 forced to one lane per pair, the steps were bit-identical).
+constexpr int kRecFillPairsMin = 1 << 21;
+
 __device__ inline int last_le(const int* __restrict__ off, int lo, int hi, int r) {
   while (lo < hi) {  // last i in [lo, hi] with off[i] <= r
     int mid = (lo + hi + 1) >> 1;
@@ -719,13 +728,14 @@ __device__ inline int last_le(const int* __restrict__ off, int lo, int hi, int r
   return lo;
 }
 
-// One lane per record: records [0, nPT) are point-triangle ones, the pair
+// One lane per record (below kRecFillPairsMin patch pairs): records [0, nPT)
+// are point-triangle ones, the pair
 // found in offPT; records [nPT, K) edge-edge ones, the pair found in offEE.
 // Within a pair the order is the reference's (getCcdPotentialICPairsListKernel,
 // PdIpcGpuFunc.cu:1838-1948) restricted to the record type.  A block whose
 // records lie in one segment and span at most kBlock pairs searches a copy
 // of its offsets in LDS.
-__global__ void __launch_bounds__(kBlock) k_rec_fill(int Q, int nPT, int K, const int2* __restrict__ pairs,
+__global__ void __launch_bounds__(kBlock) k_rec_fill_records(int Q, int nPT, int K, const int2* __restrict__ pairs,
                                                      const int* __restrict__ offPT, const int* __restrict__ offEE,
                                                      const int* __restrict__ ppOff, const int* __restrict__ ppList,
                                                      const int* __restrict__ pfOff, const int* __restrict__ pfList,
@@ -775,6 +785,53 @@ __global__ void __launch_bounds__(kBlock) k_rec_fill(int Q, int nPT, int K, cons
     out = make_int4(~e1.x, e1.y, e2.x, e2.y);
   }
   recs[r] = out;
+}
+
+// One lane per patch pair (from kRecFillPairsMin pairs) writes all of the
+// pair's records: the point-triangle ones from offPT[i], the edge-edge ones
+// from nPT + offEE[i], each in the
+// reference's order within the pair (getCcdPotentialICPairsListKernel,
+// PdIpcGpuFunc.cu:1838-1948) restricted to the record type: A's points x B's
+// faces, then B's points x A's faces; A's edges x B's edges.  The pair's
+// offsets and lists are read once for all its records (about 13 on the
+// cloth), instead of a search and the same loads per record.
+__global__ void __launch_bounds__(kBlock) k_rec_fill_pairs(int Q, int nPT, const int2* __restrict__ pairs,
+                                                     const int* __restrict__ offPT, const int* __restrict__ offEE,
+                                                     const int* __restrict__ ppOff, const int* __restrict__ ppList,
+                                                     const int* __restrict__ pfOff, const int* __restrict__ pfList,
+                                                     const int* __restrict__ peOff, const int* __restrict__ peList,
+                                                     const int3* __restrict__ faces, const int2* __restrict__ edges,
+                                                     int4* __restrict__ recs) {
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= Q) return;
+  const int2 pr = pairs[i];
+  const int A = pr.x, B = pr.y;
+  const int pA = ppOff[A], nPA = ppOff[A + 1] - pA, pB = ppOff[B], nPB = ppOff[B + 1] - pB;
+  const int fA = pfOff[A], nFA = pfOff[A + 1] - fA, fB = pfOff[B], nFB = pfOff[B + 1] - fB;
+  const int eA = peOff[A], nEA = peOff[A + 1] - eA, eB = peOff[B], nEB = peOff[B + 1] - eB;
+  int o = offPT[i];
+  for (int u = 0; u < nPA; u++) {
+    const int pt = ppList[pA + u];
+    for (int f = 0; f < nFB; f++) {
+      const int3 t = faces[pfList[fB + f]];
+      recs[o++] = make_int4(pt, t.x, t.y, t.z);
+    }
+  }
+  for (int u = 0; u < nPB; u++) {
+    const int pt = ppList[pB + u];
+    for (int f = 0; f < nFA; f++) {
+      const int3 t = faces[pfList[fA + f]];
+      recs[o++] = make_int4(pt, t.x, t.y, t.z);
+    }
+  }
+  int q = nPT + offEE[i];
+  for (int a = 0; a < nEA; a++) {
+    const int2 e1 = edges[peList[eA + a]];
+    for (int b = 0; b < nEB; b++) {
+      const int2 e2 = edges[peList[eB + b]];
+      recs[q++] = make_int4(~e1.x, e1.y, e2.x, e2.y);
+    }
+  }
 }
 
 // ---------------- zero-intersection check (validation, no counterpart in the reference) ----------------
@@ -958,9 +1015,14 @@ void launch_rec_count(const DeviceMesh& m, int nPairs, hipStream_t s) {
 
 void launch_rec_fill(const DeviceMesh& m, int nPairs, int nPT, int nRecs, hipStream_t s) {
   if (nRecs <= 0 || nPairs <= 0) return;
-  hipLaunchKernelGGL(k_rec_fill, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nPairs, nPT, nRecs, m.pairs,
-                     m.recOff, m.recOff + m.pairCap, m.ppOff, m.ppList, m.pfOff, m.pfList, m.peOff, m.peList,
-                     m.bFaces, m.bEdges, m.recs);
+  if (nPairs >= kRecFillPairsMin)
+    hipLaunchKernelGGL(k_rec_fill_pairs, dim3((nPairs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nPairs, nPT,
+                       m.pairs, m.recOff, m.recOff + m.pairCap, m.ppOff, m.ppList, m.pfOff, m.pfList, m.peOff,
+                       m.peList, m.bFaces, m.bEdges, m.recs);
+  else
+    hipLaunchKernelGGL(k_rec_fill_records, dim3((nRecs + kBlock - 1) / kBlock), dim3(kBlock), 0, s, nPairs, nPT,
+                       nRecs, m.pairs, m.recOff, m.recOff + m.pairCap, m.ppOff, m.ppList, m.pfOff, m.pfList, m.peOff,
+                       m.peList, m.bFaces, m.bEdges, m.recs);
 }
 
 }  // namespace pdipc
