@@ -244,7 +244,7 @@ int ensure_state(pdipc_handle* h) {
       if (e1 - e0 > 65535) return fail(PDIPC_E_ARG, "adjoint: a SELL slice holds more than 65535 entries");
       cols.clear();
       for (int64_t e = e0; e < e1; e++) {
-        const int slot = sl * R + (int)((e - e0) / L.sliceW[sl]) / L.lanes;
+        const int slot = sl * R + L.lane_of(sl, e) / L.lanes;
         const int r = L.perm.empty() ? slot : L.perm[slot];
         if (real(e, r)) cols.push_back(L.col[e]);
       }
@@ -253,7 +253,7 @@ int ensure_state(pdipc_handle* h) {
       const size_t p0 = uPos.size();
       uPos.resize(p0 + cols.size() * R, (uint16_t)0xFFFF);
       for (int64_t e = e0; e < e1; e++) {
-        const int slot = sl * R + (int)((e - e0) / L.sliceW[sl]) / L.lanes;
+        const int slot = sl * R + L.lane_of(sl, e) / L.lanes;
         const int r = L.perm.empty() ? slot : L.perm[slot];
         if (!real(e, r)) continue;
         const size_t t = std::lower_bound(cols.begin(), cols.end(), L.col[e]) - cols.begin();

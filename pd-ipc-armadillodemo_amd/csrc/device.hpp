@@ -122,7 +122,9 @@ struct DeviceMesh {
   double* r1Val = nullptr;
   // M^2 in SELL layout
   int sellLanes = 4, rowsPerSlice = 16, nSlices = 0;
-  int sellUniformW = 0;  // > 0: every slice has this width (offsets by arithmetic)
+  // > 0: every slice has this width (offsets by arithmetic); kSellLaneMajorW:
+  // the slices are lane-major, otherwise chunk-major
+  int sellUniformW = 0;
   int* sellPerm = nullptr;    // SELL slot -> row (nSlices * rowsPerSlice); null: identity
   int* sellSlotOf = nullptr;  // row -> SELL slot (nSim); null: identity
   int64_t sellEntries = 0;

@@ -896,9 +896,7 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
         int r = rowOf(s * L.rowsPerSlice + lr);
         int rr = r < n ? r : n - 1;  // rows past the end pad with a valid index
         for (int c = 0; c < L.sliceW[s] * lanes; c++) {
-          // lane-major within the slice: lane (lr * lanes + c % lanes) holds
-          // its sliceW[s] entries consecutively, chunk c / lanes
-          int64_t e = L.sliceOff[s] + (int64_t)(lr * lanes + c % lanes) * L.sliceW[s] + c / lanes;
+          const int64_t e = L.entry(s, lr * lanes + c % lanes, c / lanes);
           L.col[e] = rr;
           L.rowFlag[e] = rr;
           if (r < n && c < hs.m2Off[r + 1] - hs.m2Off[r]) {
@@ -918,7 +916,7 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
   L.col16ok = true;
   for (int s = 0; s < L.nSlices && L.col16ok; s++)
     for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++) {
-      int slot = s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) / L.sliceW[s]) / lanes;
+      int slot = s * L.rowsPerSlice + L.lane_of(s, e) / lanes;
       int d = L.col[e] - slot;
       if (d < -32768 || d > 32767) {
         L.col16ok = false;
@@ -930,7 +928,7 @@ void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {
     parallel_ranges(L.nSlices, nt, [&](int64_t s0, int64_t s1, int) {
       for (int s = (int)s0; s < (int)s1; s++)
         for (int64_t e = L.sliceOff[s]; e < L.sliceOff[s + 1]; e++)
-          L.col16[e] = (int16_t)(L.col[e] - (s * L.rowsPerSlice + (int)((e - L.sliceOff[s]) / L.sliceW[s]) / lanes));
+          L.col16[e] = (int16_t)(L.col[e] - (s * L.rowsPerSlice + L.lane_of(s, e) / lanes));
     });
   }
   build_stage_tiles(L, nt);

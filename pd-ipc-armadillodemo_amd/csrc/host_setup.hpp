@@ -89,9 +89,17 @@ struct HostSetup {
 std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vector<ModelInput>& statics,
                              const StepConstants& k, HostSetup& out);
 
+constexpr int kSellLaneMajorW = 8;  // slice width of the lane-major layout
+
 // SELL-C layout of the M^2 pattern for the Chebyshev sweep: slices of
-// (64 / lanes) rows; within a slice, row r's c-th entry sits at
-//   sliceOff[s] + (c / lanes) * 64 + (r - s*rows) * lanes + (c % lanes).
+// (64 / lanes) rows, 64 lanes x sliceW[s] entries each; lane l serves slot-row
+// l / lanes, whose CSR entry c is held by lane (l / lanes) * lanes + c % lanes
+// as its chunk c / lanes.  A slice is chunk-major, entry `chunk` of lane l at
+//   sliceOff[s] + chunk * 64 + l
+// (a wave's loads of one chunk are one run of 64 entries), except when every
+// slice has width kSellLaneMajorW: then slices are lane-major, at
+//   sliceOff[s] + l * kSellLaneMajorW + chunk
+// (a lane's entries are one aligned run, read with 16-byte loads).
 // Padding entries point at their own row with no 2-paths.
 struct SellLayout {
   int lanes = 4, rowsPerSlice = 16, nSlices = 0;
@@ -100,12 +108,17 @@ struct SellLayout {
   // kSellSortSlices slices; slots past the last row map to themselves) and its
   // inverse over the rows; both empty for the identity order
   std::vector<int> perm, slotOf;
-  // slice s holds 64 lanes x sliceW[s] entries, lane-major: entry `chunk` of
-  // lane l sits at sliceOff[s] + l * sliceW[s] + chunk (a lane's entries are
-  // consecutive); lane l serves slot-row l / lanes, whose CSR entry c is held
-  // by lane (l / lanes) * lanes + c % lanes as its chunk c / lanes
   std::vector<int64_t> sliceOff;   // nSlices + 1
   std::vector<int> sliceW;         // nSlices: entries per lane
+  bool laneMajor() const { return uniformW == kSellLaneMajorW; }
+  // the SELL entry `chunk` of `lane` in slice s, and the lane holding entry e
+  int64_t entry(int s, int lane, int chunk) const {
+    return sliceOff[s] + (laneMajor() ? (int64_t)lane * sliceW[s] + chunk : (int64_t)chunk * 64 + lane);
+  }
+  int lane_of(int s, int64_t e) const {
+    const int64_t d = e - sliceOff[s];
+    return (int)(laneMajor() ? d / sliceW[s] : d % 64);
+  }
   std::vector<int> col;            // per SELL entry
   // per SELL entry: col - the entry's slot index (its row without perm; padding
   // rows included);

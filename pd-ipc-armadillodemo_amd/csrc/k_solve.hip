@@ -105,11 +105,13 @@ __global__ void __launch_bounds__(kBlock, 8) k_first64(int n, int nSlices, int l
   const int row = perm ? perm[slot] : slot;
   int sub = lane % G;
   const int width = uw > 0 ? uw : sliceW[slice];
-  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (int64_t)lane * width;
+  const bool lm = uw == kSellLaneMajorW;  // lane-major slices, else chunk-major
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (lm ? (int64_t)lane * width : lane);
+  const int step = lm ? 1 : kWave;
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
-    int64_t e = off + c;
+    int64_t e = off + (int64_t)c * step;
     int j = C16 ? slot + (int)col16[e] : col[e];
     double v = val[e];
     acc[0] += v * X[j];
@@ -235,7 +237,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
   const int slot = slice * S + lane / G;  // < ld: slices cover at most ld rows
   const int row = perm ? perm[slot] : slot;
   const int width = uw > 0 ? uw : sliceW[slice];
-  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (int64_t)lane * width;
+  const bool lm = uw == kSellLaneMajorW;  // lane-major slices, else chunk-major
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (lm ? (int64_t)lane * width : lane);
+  const int step = lm ? 1 : kWave;
   Row64 p = pre[row], c = cA[row];  // loaded up front, with the matrix
   double acc[3] = {0.0, 0.0, 0.0};
   for (int c0 = 0; c0 < width; c0 += U) {
@@ -246,7 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep64(int n, int nSlices, int ld, 
       j[u] = row;
       v[u] = 0.0;
       if (c0 + u < width) {
-        int64_t e = off + (c0 + u);
+        int64_t e = off + (int64_t)(c0 + u) * step;
         j[u] = C16 ? slot + (int)col16[e] : col[e];
         v[u] = val[e];
       }
@@ -303,7 +307,9 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
   const int slot = slice * S + lane / G;  // < ld: slices cover at most ld rows
   const int row = perm ? perm[slot] : slot;
   const int width = uw > 0 ? uw : sliceW[slice];
-  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (int64_t)lane * width;
+  const bool lm = uw == kSellLaneMajorW;  // lane-major slices, else chunk-major
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (lm ? (int64_t)lane * width : lane);
+  const int step = lm ? 1 : kWave;
   // the row's own operands, loaded up front so they arrive with the matrix
   // (the last sweep of a solve also adds into X: its X too)
   float4 p = dpre[row], r = r0[row];
@@ -322,7 +328,7 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
       j[u] = row;
       v[u] = 0.0f;
       if (c0 + u < width) {
-        int64_t e = off + (c0 + u);
+        int64_t e = off + (int64_t)(c0 + u) * step;
         j[u] = C16 ? slot + (int)col16[e] : col[e];
         v[u] = val[e];
       }
@@ -371,30 +377,34 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
 // Entry offsets are 32-bit (staging requires fewer than 2^29 entries): each
 // load is a 64-bit base in scalar registers plus a 32-bit lane offset, and a
 // lane's loads past its slice's width re-read its last entry (no branches)
-// and are weighted 0.  UW > 0: every slice has width UW (known at compile
-// time, so the chunk loop and its guards fold away).
+// and are weighted 0.  UW = kSellLaneMajorW: every slice has that width and
+// is lane-major (known at compile time, so the chunk loop and its guards fold
+// away and a lane's entries load as 16-byte vectors); UW = 0: chunk-major
+// slices of any width.
 constexpr int kChunk32 = 8;  // matrix entries per lane loaded together (FP32 values)
 constexpr int kChunk64 = 4;  // (FP64 values)
+static_assert(kChunk32 == kSellLaneMajorW, "one FP32 chunk is a lane's whole lane-major run");
 
 template <class T>
 __device__ __forceinline__ T ld32(const T* __restrict__ base, uint32_t i) {
   return *(const T*)((const char*)base + i * (uint32_t)sizeof(T));
 }
 
+// chunk-major slices: entry c of the lane at off + c * 64
 template <int CH, class V>
 __device__ __forceinline__ void load_chunk_any(const V* __restrict__ val, const uint16_t* __restrict__ loc,
                                                uint32_t off, int c0, int width, V (&v)[CH], int (&li)[CH]) {
 #pragma unroll
   for (int u = 0; u < CH; u++) {
     const int c = c0 + u < width ? c0 + u : width - 1;
-    const uint32_t e = off + (uint32_t)c;
+    const uint32_t e = off + (uint32_t)(c * kWave);
     const V x = ld32(val, e);
     li[u] = ld32(loc, e);
     v[u] = c0 + u < width ? x : V(0);
   }
 }
 
-// Every slice of width 8: a lane's entries are 8 consecutive, 8-aligned slots,
+// Lane-major slices of width 8: a lane's entries are 8 consecutive, 8-aligned slots,
 // so a chunk is two 16-byte value loads and one index load (FP32: 8 values,
 // one 16-byte index load; FP64: 4 values, one 8-byte index load).
 template <int CH, class V>
@@ -429,7 +439,8 @@ __device__ __forceinline__ void load_chunk_w8(const V* __restrict__ val, const u
 template <int UW, int CH, class V>
 __device__ __forceinline__ void load_chunk(const V* __restrict__ val, const uint16_t* __restrict__ loc, uint32_t off,
                                            int c0, int width, V (&v)[CH], int (&li)[CH]) {
-  if constexpr (UW == 8) load_chunk_w8(val, loc, off, c0, v, li);
+  static_assert(UW == 0 || UW == kSellLaneMajorW, "lane-major slices have width kSellLaneMajorW");
+  if constexpr (UW == kSellLaneMajorW) load_chunk_w8(val, loc, off, c0, v, li);
   else load_chunk_any(val, loc, off, c0, width, v, li);
 }
 
@@ -496,7 +507,8 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int 
   const int slot = sl * S + lane / G;
   const int row = perm ? perm[slot] : slot;
   const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
-  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + (uint32_t)(lane * width);
+  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) +
+                       (uint32_t)(UW == kSellLaneMajorW ? lane * width : lane);
   float v[kChunk32];
   int li[kChunk32];
   load_chunk<UW>(val, loc, off, 0, width, v, li);
@@ -554,8 +566,9 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int 
   const int slot = sl * S + lane / G;
   const int row = perm ? perm[slot] : slot;
   const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
-  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + (uint32_t)(lane * width);
-  constexpr int CH = UW == 8 ? 8 : kChunk64;  // width 8: the lane's 8 entries in one chunk
+  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) +
+                       (uint32_t)(UW == kSellLaneMajorW ? lane * width : lane);
+  constexpr int CH = UW == kSellLaneMajorW ? kSellLaneMajorW : kChunk64;  // width 8: the lane's 8 entries in one chunk
   double v[CH];
   int li[CH];
   load_chunk<UW>(val, loc, off, 0, width, v, li);
@@ -605,9 +618,9 @@ void launch_sweep32s(const DeviceMesh& m, float W, const float4* cur, const floa
            (uint32_t)staged_lds_bytes<float>(SP), s, m.nSim, m.nSlices, m.ld, W, m.sliceOff, m.sliceW,       \
            m.sellUniformW, m.sellPerm, m.m2Loc, m.m2ValF, m.stageStride, m.tileBase, m.tileCol, cur, pre, m.r0, next, \
            m.X)
-  if (m.sellUniformW == kChunk32) {
-    if (sp == 2) SW32(2, kChunk32);
-    else SW32(4, kChunk32);
+  if (m.sellUniformW == kSellLaneMajorW) {  // lane-major slices
+    if (sp == 2) SW32(2, kSellLaneMajorW);
+    else SW32(4, kSellLaneMajorW);
   } else {
     if (sp == 2) SW32(2, 0);
     else SW32(4, 0);
@@ -625,9 +638,9 @@ void launch_sweep64s(const DeviceMesh& m, double W, const Row64* cur, const Row6
            (uint32_t)staged_lds_bytes<double>(SP), s, m.nSim, m.nSlices, m.ld, W, m.sliceOff, m.sliceW,      \
            m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.stageStride, m.tileBase, m.tileCol, cur, pre, m.cA, next, \
            out)
-  if (m.sellUniformW == kChunk32) {
-    if (sp == 2) SW64(2, kChunk32);
-    else SW64(4, kChunk32);
+  if (m.sellUniformW == kSellLaneMajorW) {  // lane-major slices
+    if (sp == 2) SW64(2, kSellLaneMajorW);
+    else SW64(4, kSellLaneMajorW);
   } else {
     if (sp == 2) SW64(2, 0);
     else SW64(4, 0);
@@ -753,11 +766,13 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv(int n, int nSlices, int 
   const int row = perm ? perm[slot] : slot;
   int sub = lane % G;
   const int width = uw > 0 ? uw : sliceW[slice];
-  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (int64_t)lane * width;
+  const bool lm = uw == kSellLaneMajorW;  // lane-major slices, else chunk-major
+  const int64_t off = (uw > 0 ? (int64_t)slice * uw * kWave : sliceOff[slice]) + (lm ? (int64_t)lane * width : lane);
+  const int step = lm ? 1 : kWave;
   double acc[3] = {0.0, 0.0, 0.0};
 #pragma unroll 4
   for (int c = 0; c < width; c++) {
-    int64_t e = off + c;
+    int64_t e = off + (int64_t)c * step;
     int j = C16 ? slot + (int)col16[e] : col[e];
     double v = val[e];
     acc[0] += v * uin[j];
@@ -870,8 +885,9 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, in
   const int slot = sl * S + lane / G;
   const int row = perm ? perm[slot] : slot;
   const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
-  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) + (uint32_t)(lane * width);
-  constexpr int CH = UW == 8 ? 8 : kChunk64;
+  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) +
+                       (uint32_t)(UW == kSellLaneMajorW ? lane * width : lane);
+  constexpr int CH = UW == kSellLaneMajorW ? kSellLaneMajorW : kChunk64;
   double v[CH];
   int li[CH];
   load_chunk<UW>(val, loc, off, 0, width, v, li);
@@ -929,9 +945,9 @@ void adj_pass_impl(const DeviceMesh& m, const AdjMesh& a, const double* W, int K
                      staged_lds_bytes<double>(SP), s, m.nSim, m.nSlices, m.ld, W[k], b, m.sliceOff,     \
                      m.sliceW, m.sellUniformW, m.sellPerm, m.m2Loc, m.m2Val, m.stageStride, m.tileBase,         \
                      m.tileCol, u[k + 1], u[k + 2], u[k], uc)
-        if (m.sellUniformW == kChunk32) {
-          if (sp == 2) ADJ(2, kChunk32);
-          else ADJ(4, kChunk32);
+        if (m.sellUniformW == kSellLaneMajorW) {  // lane-major slices
+          if (sp == 2) ADJ(2, kSellLaneMajorW);
+          else ADJ(4, kSellLaneMajorW);
         } else {
           if (sp == 2) ADJ(2, 0);
           else ADJ(4, 0);
@@ -1017,7 +1033,7 @@ __device__ inline double wave_sum(double v) {
 // (slotOf: the row's SELL slot when rows are reordered)
 template <int G>
 __global__ void __launch_bounds__(kBlock) k_adj_ops_rows(int n, const int64_t* __restrict__ sliceOff,
-                                                         const int* __restrict__ sliceW,
+                                                         const int* __restrict__ sliceW, int uw,
                                                          const int* __restrict__ slotOf, const double* __restrict__ r1e,
                                                          const int* __restrict__ r1Idx,
                                                          const int64_t* __restrict__ pOff, const int* __restrict__ pK,
@@ -1035,8 +1051,10 @@ __global__ void __launch_bounds__(kBlock) k_adj_ops_rows(int n, const int64_t* _
   int cnt = width * G;
   const double w2 = w * w, c1 = 2.0 * (1.0 - w) * w;
   double acc = 0.0;
+  const bool lm = uw == kSellLaneMajorW;  // lane-major slices, else chunk-major
   for (int t = lane; t < cnt; t += kWave) {
-    int64_t e = base + (int64_t)(lr * G + t % G) * width + t / G;  // entry t of the row: lane lr*G + t%G, chunk t/G
+    // entry t of the row: lane lr*G + t%G, chunk t/G
+    const int64_t e = base + (lm ? (int64_t)(lr * G + t % G) * width + t / G : (int64_t)(t / G) * kWave + lr * G + t % G);
     double g = Gbar[e];
     if (g == 0.0) continue;
     double sp = 0.0;
@@ -1149,7 +1167,7 @@ void launch_adj_ops_gather(const DeviceMesh& m, const AdjMesh& a, double omega, 
   const int rowsPerBlock = kBlock / kWave;
   const int gr = (m.nSim + rowsPerBlock - 1) / rowsPerBlock;
 #define ROWS(G)                                                                                                    \
-  hipLaunchKernelGGL(k_adj_ops_rows<G>, dim3(gr), dim3(kBlock), 0, s, m.nSim, m.sliceOff, m.sliceW, m.sellSlotOf, m.m2R1,        \
+  hipLaunchKernelGGL(k_adj_ops_rows<G>, dim3(gr), dim3(kBlock), 0, s, m.nSim, m.sliceOff, m.sliceW, m.sellUniformW, m.sellSlotOf, m.m2R1,        \
                      a.sellR1Idx, m.p2Off, m.p2K, m.p2A, m.sysDiag, Gbar, omega, Dbar, R1bar)
   switch (m.sellLanes) {
     case 1: ROWS(1); break;

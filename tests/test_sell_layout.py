@@ -42,7 +42,7 @@ def test_sell_layout_holds_every_entry_once(which, lanes):
     if which == "tets":
         assert perm is not None  # tet rows alternate two widths: never uniform
     if which == "cloth96":
-        assert L["uniform_w"] > 0
+        assert L["uniform_w"] == 8  # the lane-major layout (the others are chunk-major)
     order = np.arange(slots) if perm is None else perm.astype(np.int64)
     assert np.array_equal(np.sort(order), np.arange(slots))
     assert np.array_equal(order[n:], np.arange(n, slots))  # padding slots keep their place
@@ -53,8 +53,10 @@ def test_sell_layout_holds_every_entry_once(which, lanes):
 
     e = np.arange(len(col))
     sl = np.searchsorted(off, e, side="right") - 1
-    lane_in = (e - off[sl]) // w[sl]  # lane-major slices: a lane's entries are consecutive
-    chunk = (e - off[sl]) % w[sl]
+    if L["uniform_w"] == 8:  # lane-major slices: a lane's 8 entries are consecutive
+        lane_in, chunk = (e - off[sl]) // w[sl], (e - off[sl]) % w[sl]
+    else:  # chunk-major: a chunk's 64 lanes are consecutive
+        lane_in, chunk = (e - off[sl]) % 64, (e - off[sl]) // 64
     slot = sl * S + lane_in // lanes
     q = chunk * lanes + lane_in % lanes  # position in the row's CSR order
     row = order[slot]
