@@ -486,6 +486,23 @@ __device__ __forceinline__ void stage_tile(StagedRows<typename std::conditional<
   for (int q = 0; q < SP; q++) rows.put(threadIdx.x + q * kBlock, v[q].x, v[q].y, v[q].z);
 }
 
+// stage_tile for an SoA operand (x, y, z at src, src + ld, src + 2 ld): a
+// staged row is three loads
+template <int SP>
+__device__ __forceinline__ void stage_tile_soa(StagedRows<double> rows, const double* __restrict__ src, int ld,
+                                               int tile, int stride, const int* __restrict__ tileBase,
+                                               const uint16_t* __restrict__ tileCol) {
+  int c[SP];
+  tile_columns<SP>(tile, stride, tileBase, tileCol, c);
+  double g[SP][3];
+#pragma unroll
+  for (int q = 0; q < SP; q++)
+#pragma unroll
+    for (int k = 0; k < 3; k++) g[q][k] = ld32(src, (uint32_t)(k * ld + c[q]));
+#pragma unroll
+  for (int q = 0; q < SP; q++) rows.put(threadIdx.x + q * kBlock, g[q][0], g[q][1], g[q][2]);
+}
+
 template <int G, int LAST, int SP, int UW>
 __global__ void __launch_bounds__(kBlock, 8) k_sweep32s(int n, int nSlices, int ld, float W,
                                                         const int64_t* __restrict__ sliceOff,
@@ -604,6 +621,84 @@ __global__ void __launch_bounds__(kBlock, 8) k_sweep64s(int n, int nSlices, int 
   }
 }
 
+// The first FP64 sweep (k_first64) with x0 = X staged in LDS as in
+// k_sweep64s; the same sums in the same order, so bit-identical to it.
+template <int G, int SP, int UW>
+__global__ void __launch_bounds__(kBlock, 8) k_first64s(int n, int nSlices, int ld,
+                                                        const int64_t* __restrict__ sliceOff,
+                                                        const int* __restrict__ sliceW, int uw,
+                                                        const int* __restrict__ perm, const uint16_t* __restrict__ loc,
+                                                        const double* __restrict__ val, int stride,
+                                                        const int* __restrict__ tileBase,
+                                                        const uint16_t* __restrict__ tileCol,
+                                                        const double* __restrict__ X, Row64* __restrict__ cA,
+                                                        Row64* __restrict__ xa0, Row64* __restrict__ xa1,
+                                                        double* __restrict__ out, FirstTermArgs ft) {
+  extern __shared__ double2 ldsFirst[];
+  const StagedRows<double> xs(ldsFirst, SP * kBlock);
+  const int tile = blockIdx.x;
+  const int slice = tile * kTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const bool live = slice < nSlices;
+  constexpr int S = kWave / G;
+  const int sl = live ? slice : nSlices - 1;
+  const int slot = sl * S + lane / G;
+  const int row = perm ? perm[slot] : slot;
+  const int sub = lane % G;
+  const int width = UW > 0 ? UW : (uw > 0 ? uw : sliceW[sl]);
+  const uint32_t off = (uint32_t)(UW > 0 || uw > 0 ? (int64_t)sl * width * kWave : sliceOff[sl]) +
+                       (uint32_t)(UW == kSellLaneMajorW ? lane * width : lane);
+  constexpr int CH = UW == kSellLaneMajorW ? kSellLaneMajorW : kChunk64;
+  double v[CH];
+  int li[CH];
+  load_chunk<UW>(val, loc, off, 0, width, v, li);
+  stage_tile_soa<SP>(xs, X, ld, tile, stride, tileBase, tileCol);
+  __syncthreads();
+  if (!live) return;
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll 1
+  for (int c0 = 0;;) {
+#pragma unroll
+    for (int u = 0; u < CH; u++) {
+      const double2 a = xs.xy[li[u]];
+      const double z = xs.z[li[u]];
+      acc[0] += v[u] * a.x;
+      acc[1] += v[u] * a.y;
+      acc[2] += v[u] * z;
+    }
+    c0 += CH;
+    if (c0 >= width) break;
+    load_chunk<UW>(val, loc, off, c0, width, v, li);
+  }
+  group_reduce<G>(acc);
+  double cr[3] = {0.0, 0.0, 0.0};
+  if (row < n)
+    for (int q = ft.off[row] + sub; q < ft.off[row + 1]; q += G) {
+      const int j = ft.col[q];
+      const double f = ft.val[q] / ft.D[j];
+#pragma unroll
+      for (int k = 0; k < 3; k++) cr[k] += f * ft.b[k * ld + j];
+    }
+  group_reduce<G>(cr);
+  if (sub != 0 || row >= n) return;
+  const double di = ft.D[row];
+  double c[3], y[3], x0[3];
+#pragma unroll
+  for (int k = 0; k < 3; k++) {
+    const int idx = k * ld + row;
+    c[k] = (ft.w * ft.w * cr[k] + (2.0 - ft.w) * ft.w * ft.b[idx]) / di;
+    y[k] = acc[k] + c[k];
+    x0[k] = X[idx];
+  }
+  cA[row] = Row64{c[0], c[1], c[2], 0.0};
+  xa0[row] = Row64{x0[0], x0[1], x0[2], 0.0};
+  xa1[row] = Row64{y[0], y[1], y[2], 0.0};
+  if (out) {
+    out[row] = y[0];
+    out[ld + row] = y[1];
+    out[2 * ld + row] = y[2];
+  }
+}
+
 // rows of the staged union per lane: 2 or 4 (the union holds at most kStageMaxCols = 4 * kBlock)
 static_assert(kStageMaxCols <= 4 * kBlock, "staged unions are loaded at most four rows per lane");
 inline int stage_per_lane(int stride) { return stride <= 2 * kBlock ? 2 : 4; }
@@ -682,9 +777,30 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   double r = rho * rho;
   double W = 2.0 / (2.0 - r);
   const bool c16 = m.m2Col16 != nullptr;
-  launch_t(m.timer, TC_FIRST, c16 ? k_first64<G, true> : k_first64<G, false>, dim3(grid), dim3(kBlock), 0, s, m.nSim,
-           m.nSlices, m.ld, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2Val, m.X, m.cA,
-           m.xa0, m.xa1, outs ? outs[1] : nullptr, ft);
+  bool stagedFirst = false;
+  if constexpr (G > 1) {  // one lane per row is never staged
+    if (m.m2Loc) {
+      const int sp = stage_per_lane(m.stageStride);
+#define F64(SP, UW)                                                                                                \
+  launch_t(m.timer, TC_FIRST, k_first64s<G, SP, UW>, dim3(tiles_of(m)), dim3(kBlock),                              \
+           (uint32_t)staged_lds_bytes<double>(SP), s, m.nSim, m.nSlices, m.ld, m.sliceOff, m.sliceW, m.sellUniformW, \
+           m.sellPerm, m.m2Loc, m.m2Val, m.stageStride, m.tileBase, m.tileCol, m.X, m.cA, m.xa0, m.xa1,              \
+           outs ? outs[1] : nullptr, ft)
+      if (m.sellUniformW == kSellLaneMajorW) {  // lane-major slices
+        if (sp == 2) F64(2, kSellLaneMajorW);
+        else F64(4, kSellLaneMajorW);
+      } else {
+        if (sp == 2) F64(2, 0);
+        else F64(4, 0);
+      }
+#undef F64
+      stagedFirst = true;
+    }
+  }
+  if (!stagedFirst)
+    launch_t(m.timer, TC_FIRST, c16 ? k_first64<G, true> : k_first64<G, false>, dim3(grid), dim3(kBlock), 0, s,
+             m.nSim, m.nSlices, m.ld, m.sliceOff, m.sliceW, m.sellUniformW, m.sellPerm, m.m2Col, m.m2Col16, m.m2Val,
+             m.X, m.cA, m.xa0, m.xa1, outs ? outs[1] : nullptr, ft);
   Row64 *pre = m.xa0, *cur = m.xa1, *next = m.xa2;
   if (iters <= 1) {
     hipLaunchKernelGGL(k_row64_to_soa, dim3((m.nSim + kBlock - 1) / kBlock), dim3(kBlock), 0, s, m.nSim, m.ld, cur,
@@ -891,17 +1007,7 @@ __global__ void __launch_bounds__(kBlock, 8) k_adj_spmv_s(int n, int nSlices, in
   double v[CH];
   int li[CH];
   load_chunk<UW>(val, loc, off, 0, width, v, li);
-  {
-    int c[SP];
-    tile_columns<SP>(tile, stride, tileBase, tileCol, c);
-    double g[SP][3];
-#pragma unroll
-    for (int q = 0; q < SP; q++)
-#pragma unroll
-      for (int k = 0; k < 3; k++) g[q][k] = ld32(uin, (uint32_t)(k * ld + c[q]));
-#pragma unroll
-    for (int q = 0; q < SP; q++) us.put(threadIdx.x + q * kBlock, g[q][0], g[q][1], g[q][2]);
-  }
+  stage_tile_soa<SP>(us, uin, ld, tile, stride, tileBase, tileCol);
   __syncthreads();
   if (!live) return;
   double acc[3] = {0.0, 0.0, 0.0};

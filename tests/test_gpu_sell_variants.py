@@ -93,14 +93,18 @@ def test_reordered_rows_tet_block(lanes):
             assert worst < 1e-5
 
 
-@pytest.mark.parametrize("which,lanes", [("cloth", 8), ("cloth", 4), ("cloth", 2), ("tets", 8), ("tets", 4)])
+@pytest.mark.parametrize("which,lanes", [("cloth", 8), ("cloth", 4), ("cloth", 2), ("cloth96", 8), ("tets", 8),
+                                         ("tets", 4)])
 def test_staged_sweeps_are_bit_identical(which, lanes):
     """The sweeps that stage their operand in LDS (the default whenever the
     tiles fit) sum every row's entries in the same order as the L2-gather
     sweeps: whole steps agree bit for bit in both precisions, and the staged
-    FP64 step meets the oracle's bar."""
+    FP64 step meets the oracle's bar.  The 96 x 96 cloth has the uniform
+    width 8 (lane-major slices, vector loads); the others are chunk-major."""
     if which == "cloth":
         sc = scenes.cloth_over_sphere(24, 2, ground=True, seed=2, patch_dim=24, sphere_patch_dim=12, height=0.16)
+    elif which == "cloth96":
+        sc = scenes.cloth_over_sphere(96, 2, ground=True, seed=2, patch_dim=96, sphere_patch_dim=16, height=0.16)
     else:
         sc = scenes.config2(cells=4, cloth_n=8)
     ref = oracle.Oracle(sc)
@@ -109,6 +113,7 @@ def test_staged_sweeps_are_bit_identical(which, lanes):
     probe = pdipc.Simulator.from_scene(sc, init=False, sell_lanes=lanes)
     probe.host_setup()
     assert probe.stage_layout(lanes) is not None
+    assert (probe.sell_layout(lanes)["uniform_w"] == 8) == (which == "cloth96")
     for prec in (pdipc.FP64, pdipc.FP32_SPLIT):
         print(f"case: staged vs L2-gather sweeps, {which}, {lanes} lanes, precision {prec}", flush=True)
         sims = [pdipc.Simulator.from_scene(sc, sell_lanes=lanes, precision=prec, sell_stage=st) for st in (1, 0)]
