@@ -144,6 +144,13 @@ struct DeviceMesh {
   int* tileBase = nullptr;
   uint16_t *tileCol = nullptr, *m2Loc = nullptr;
   int stageStride = 0;
+  // narrow staging of the FP32 sweep (lane-major layouts whose FP32 matrix
+  // streams from HBM): tiles of kNarrowTileSlices slices, lists at
+  // narrowStride, one-byte union indices per entry; null when not used
+  int* ntileBase = nullptr;
+  uint16_t* ntileCol = nullptr;
+  uint8_t* m2Loc8 = nullptr;
+  int narrowStride = 0;
   // collision topology
   int nFaces = 0, nEdges = 0, nPatches = 0;
   int3* bFaces = nullptr;

@@ -791,48 +791,63 @@ std::string build_host_setup(const std::vector<ModelInput>& sims, const std::vec
   return std::string();
 }
 
-// The column union of each tile of kTileSlices slices (see SellLayout).
-static void build_stage_tiles(SellLayout& L, int nt) {
-  const int nTiles = (L.nSlices + kTileSlices - 1) / kTileSlices;
-  if (L.sliceOff.back() >= ((int64_t)1 << 29)) return;  // the staged sweeps address entries in 32 bits
-  if (L.lanes < 2) return;  // one lane per row: a tile spans 256 rows, not staged
+// The column union of each tile of TS slices, at most maxCols columns
+// spanning at most 65536; per entry its index in the union (Loc: uint16_t or
+// uint8_t).  Returns false when a tile does not fit.
+template <class Loc>
+static bool stage_tiles(const SellLayout& L, int nt, int TS, int maxCols, std::vector<int>& tileOff,
+                        std::vector<int>& tileBase, std::vector<uint16_t>& tileCol, std::vector<Loc>& loc,
+                        int& stageMax) {
+  const int nTiles = (L.nSlices + TS - 1) / TS;
   std::vector<std::vector<int>> uni(nTiles);
   std::vector<uint8_t> ok(nTiles, 1);
   const int64_t total = L.sliceOff.back();
-  L.loc.assign(total, 0);
+  loc.assign(total, 0);
   parallel_ranges(nTiles, nt, [&](int64_t t0, int64_t t1, int) {
     for (int t = (int)t0; t < (int)t1; t++) {
-      const int64_t e0 = L.sliceOff[(size_t)t * kTileSlices];
-      const int64_t e1 = L.sliceOff[std::min(L.nSlices, (t + 1) * kTileSlices)];
+      const int64_t e0 = L.sliceOff[(size_t)t * TS];
+      const int64_t e1 = L.sliceOff[std::min(L.nSlices, (t + 1) * TS)];
       std::vector<int>& U = uni[t];
       U.assign(L.col.begin() + e0, L.col.begin() + e1);
       std::sort(U.begin(), U.end());
       U.erase(std::unique(U.begin(), U.end()), U.end());
-      if (U.empty() || U.back() - U.front() > 65535 || (int)U.size() > kStageMaxCols) {
+      if (U.empty() || U.back() - U.front() > 65535 || (int)U.size() > maxCols) {
         ok[t] = 0;
         continue;
       }
       for (int64_t e = e0; e < e1; e++)
-        L.loc[e] = (uint16_t)(std::lower_bound(U.begin(), U.end(), L.col[e]) - U.begin());
+        loc[e] = (Loc)(std::lower_bound(U.begin(), U.end(), L.col[e]) - U.begin());
     }
   });
-  L.stageOk = nTiles > 0 && std::all_of(ok.begin(), ok.end(), [](uint8_t v) { return v != 0; });
-  if (!L.stageOk) {
-    L.loc.clear();
-    return;
+  if (nTiles == 0 || !std::all_of(ok.begin(), ok.end(), [](uint8_t v) { return v != 0; })) {
+    loc.clear();
+    return false;
   }
-  L.tileOff.assign(nTiles + 1, 0);
-  L.tileBase.assign(nTiles, 0);
+  tileOff.assign(nTiles + 1, 0);
+  tileBase.assign(nTiles, 0);
+  stageMax = 0;
   for (int t = 0; t < nTiles; t++) {
-    L.tileOff[t + 1] = L.tileOff[t] + (int)uni[t].size();
-    L.tileBase[t] = uni[t].front();
-    L.stageMax = std::max(L.stageMax, (int)uni[t].size());
+    tileOff[t + 1] = tileOff[t] + (int)uni[t].size();
+    tileBase[t] = uni[t].front();
+    stageMax = std::max(stageMax, (int)uni[t].size());
   }
-  L.tileCol.resize(L.tileOff.back());
+  tileCol.resize(tileOff.back());
   parallel_ranges(nTiles, nt, [&](int64_t t0, int64_t t1, int) {
     for (int t = (int)t0; t < (int)t1; t++)
-      for (size_t i = 0; i < uni[t].size(); i++) L.tileCol[L.tileOff[t] + i] = (uint16_t)(uni[t][i] - L.tileBase[t]);
+      for (size_t i = 0; i < uni[t].size(); i++) tileCol[tileOff[t] + i] = (uint16_t)(uni[t][i] - tileBase[t]);
   });
+  return true;
+}
+
+// The staging tables (see SellLayout): tiles of kTileSlices slices, and for
+// lane-major layouts also the narrow tiles of kNarrowTileSlices slices.
+static void build_stage_tiles(SellLayout& L, int nt) {
+  if (L.sliceOff.back() >= ((int64_t)1 << 29)) return;  // the staged sweeps address entries in 32 bits
+  if (L.lanes < 2) return;  // one lane per row: a tile spans 256 rows, not staged
+  L.stageOk = stage_tiles(L, nt, kTileSlices, kStageMaxCols, L.tileOff, L.tileBase, L.tileCol, L.loc, L.stageMax);
+  if (L.stageOk && L.laneMajor() && L.lanes == 8)
+    L.narrowOk = stage_tiles(L, nt, kNarrowTileSlices, kNarrowMaxCols, L.ntileOff, L.ntileBase, L.ntileCol, L.loc8,
+                             L.narrowMax);
 }
 
 void build_sell(const HostSetup& hs, int lanes, SellLayout& L) {

@@ -143,10 +143,21 @@ struct SellLayout {
   std::vector<int> tileBase;       // nTiles: smallest column of the tile
   std::vector<uint16_t> tileCol;   // union columns - tileBase
   std::vector<uint16_t> loc;       // per SELL entry: index in its tile's union
+  // Narrow staging (lane-major layouts with 8 lanes per row; the FP32 sweep
+  // when the matrix streams from HBM): tiles of kNarrowTileSlices slices whose
+  // unions hold at most kNarrowMaxCols columns, so an entry's union index is
+  // one byte.  narrowOk: every narrow tile fits.
+  bool narrowOk = false;
+  int narrowMax = 0;
+  std::vector<int> ntileOff, ntileBase;
+  std::vector<uint16_t> ntileCol;
+  std::vector<uint8_t> loc8;
 };
 constexpr int kSellSortSlices = 2;  // slices per row-sorting window
 constexpr int kTileSlices = 4;      // slices per staged tile (a 256-lane workgroup)
 constexpr int kStageMaxCols = 1024; // largest staged union (16 KiB of FP32 rows, 32 KiB of FP64 rows)
+constexpr int kNarrowTileSlices = 2;  // slices per narrow tile (a 128-lane workgroup)
+constexpr int kNarrowMaxCols = 256;   // largest narrow union (8-bit indices)
 void build_sell(const HostSetup& hs, int lanes, SellLayout& out);
 
 }  // namespace pdipc

@@ -752,9 +752,93 @@ __global__ void k_apply_delta(int n, int ld, const float4* __restrict__ d, doubl
   X[2 * ld + i] += (double)v.z;
 }
 
+// The FP32 sweep with narrow staging: lane-major slices of width 8 at 8
+// lanes per row, one workgroup per tile of kNarrowTileSlices slices, whose
+// union (at most kNarrowMaxCols columns) gives every entry a one-byte index,
+// so a lane's 8 entries are two 16-byte value loads and one 8-byte index
+// load (6 -> 5 bytes per slot).  Otherwise k_sweep32s: the same sums in the
+// same order, so bit-identical to it.
+constexpr int kNarrowBlock = kNarrowTileSlices * kWave;
+static_assert(kNarrowMaxCols <= 2 * kNarrowBlock, "narrow unions are staged at most two rows per lane");
+template <int LAST>
+__global__ void __launch_bounds__(kNarrowBlock) k_sweep32n(int n, int nSlices, int ld, float W,
+                                                           const uint8_t* __restrict__ loc8,
+                                                           const float* __restrict__ val, int stride,
+                                                           const int* __restrict__ tileBase,
+                                                           const uint16_t* __restrict__ tileCol,
+                                                           const float4* __restrict__ dcur,
+                                                           const float4* __restrict__ dpre,
+                                                           const float4* __restrict__ r0, float4* __restrict__ dnext,
+                                                           double* __restrict__ X) {
+  constexpr int G = 8, S = kWave / G, UW = kSellLaneMajorW;
+  __shared__ float2 sxy[kNarrowMaxCols];
+  __shared__ float sz[kNarrowMaxCols];
+  const int tile = blockIdx.x;
+  const int slice = tile * kNarrowTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const bool live = slice < nSlices;  // every lane stages and meets the barrier
+  const int sl = live ? slice : nSlices - 1;
+  const int row = sl * S + lane / G;  // uniform layouts keep the row order
+  const uint32_t off = (uint32_t)sl * (uint32_t)(UW * kWave) + (uint32_t)(lane * UW);
+  const float4 va = ld32((const float4*)val, off / 4), vb = ld32((const float4*)val, off / 4 + 1);
+  const uint2 lw = ld32((const uint2*)loc8, off / 8);
+  const float4 p = ld32(dpre, (uint32_t)row), r = ld32(r0, (uint32_t)row);
+  {
+    const int base = tileBase[tile];
+    const uint32_t t0 = (uint32_t)tile * (uint32_t)stride;
+    int c[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int u = threadIdx.x + q * kNarrowBlock;
+      c[q] = base + (int)ld32(tileCol, t0 + (uint32_t)(u < stride ? u : stride - 1));
+    }
+    float4 g[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) g[q] = ld32(dcur, (uint32_t)c[q]);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      sxy[threadIdx.x + q * kNarrowBlock] = make_float2(g[q].x, g[q].y);
+      sz[threadIdx.x + q * kNarrowBlock] = g[q].z;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const float v[8] = {va.x, va.y, va.z, va.w, vb.x, vb.y, vb.z, vb.w};
+  const int li[8] = {(int)(lw.x & 0xff), (int)((lw.x >> 8) & 0xff), (int)((lw.x >> 16) & 0xff), (int)(lw.x >> 24),
+                     (int)(lw.y & 0xff), (int)((lw.y >> 8) & 0xff), (int)((lw.y >> 16) & 0xff), (int)(lw.y >> 24)};
+  float acc[3] = {0.f, 0.f, 0.f};
+#pragma unroll
+  for (int u = 0; u < 8; u++) {
+    const float2 a = sxy[li[u]];
+    const float z = sz[li[u]];
+    acc[0] += v[u] * a.x;
+    acc[1] += v[u] * a.y;
+    acc[2] += v[u] * z;
+  }
+  group_reduce<G>(acc);
+  if ((lane % G) != 0 || row >= n) return;
+  float d0 = W * (acc[0] + r.x - p.x) + p.x;
+  float d1 = W * (acc[1] + r.y - p.y) + p.y;
+  float d2 = W * (acc[2] + r.z - p.z) + p.z;
+  if (LAST) {
+    X[row] += (double)d0;
+    X[ld + row] += (double)d1;
+    X[2 * ld + row] += (double)d2;
+  } else {
+    dnext[row] = make_float4(d0, d1, d2, 0.0f);
+  }
+}
+
 template <int G, int LAST>
 void launch_sweep32(const DeviceMesh& m, int grid, float W, const float4* cur, const float4* pre, float4* next,
                     hipStream_t s) {
+  if constexpr (G == 8) {
+    if (m.m2Loc8) {
+      launch_t(m.timer, TC_SWEEP32, k_sweep32n<LAST>, dim3((m.nSlices + kNarrowTileSlices - 1) / kNarrowTileSlices),
+               dim3(kNarrowBlock), 0, s, m.nSim, m.nSlices, m.ld, W, m.m2Loc8, m.m2ValF, m.narrowStride,
+               m.ntileBase, m.ntileCol, cur, pre, m.r0, next, m.X);
+      return;
+    }
+  }
   if constexpr (G > 1) {  // one lane per row is never staged (its tiles span 256 rows)
     if (m.m2Loc) {
       launch_sweep32s<G, LAST>(m, W, cur, pre, next, s);

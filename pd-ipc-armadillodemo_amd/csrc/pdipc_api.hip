@@ -153,6 +153,22 @@ int upload_all(pdipc_handle* h) {
     }
     UP(tileBase, L.tileBase); UP(tileCol, padded); UP(m2Loc, L.loc);
     m.stageStride = stride;
+    // narrow staging for the FP32 sweep once its matrix (4 B per slot) is
+    // larger than half the 256 MB Infinity Cache, so that it streams from HBM
+    // every sweep and its one-byte indices save a sixth of the bytes (below
+    // that the smaller tiles' extra staging costs what the byte saves);
+    // sell_stage = 2 takes it at any size
+    if (L.narrowOk && (h->prm.sell_stage == 2 || 4.0 * (double)m.sellEntries > 128e6)) {
+      const int nN = (int)L.ntileBase.size();
+      const int nStride = (L.narrowMax + 7) / 8 * 8;
+      std::vector<uint16_t> npad((size_t)nN * nStride);
+      for (int t = 0; t < nN; t++) {
+        const int u0 = L.ntileOff[t], nU = L.ntileOff[t + 1] - u0;
+        for (int u = 0; u < nStride; u++) npad[(size_t)t * nStride + u] = L.ntileCol[u0 + std::min(u, nU - 1)];
+      }
+      UP(ntileBase, L.ntileBase); UP(ntileCol, npad); UP(m2Loc8, L.loc8);
+      m.narrowStride = nStride;
+    }
   }
   // collision
   m.nFaces = (int)hs.bFaces.size() / 3;
@@ -905,6 +921,36 @@ int pdipc_debug_stage_layout(pdipc_handle* h, int lanes, long long* sizes, int* 
   if (tile_base) std::copy(L.tileBase.begin(), L.tileBase.end(), tile_base);
   if (tile_col) std::copy(L.tileCol.begin(), L.tileCol.end(), tile_col);
   if (loc) std::copy(L.loc.begin(), L.loc.end(), loc);
+  return PDIPC_OK;
+}
+
+int pdipc_debug_narrow_stage_layout(pdipc_handle* h, long long* sizes, int* tile_off, int* tile_base,
+                                    unsigned short* tile_col, unsigned char* loc) {
+  if (!h) return fail(PDIPC_E_ARG, "null handle");
+  if (!h->hostReady) return fail(PDIPC_E_ARG, "pdipc_host_setup has not run");
+  if (!sizes) return fail(PDIPC_E_ARG, "null sizes");
+  SellLayout L;
+  build_sell(h->hs, 8, L);
+  const int nTiles = L.narrowOk ? (int)L.ntileBase.size() : 0;
+  sizes[0] = L.narrowOk ? 1 : 0;
+  sizes[1] = nTiles;
+  sizes[2] = L.narrowOk ? (long long)L.ntileCol.size() : 0;
+  sizes[3] = L.narrowMax;
+  sizes[4] = L.sliceOff.back();
+  sizes[5] = kNarrowTileSlices;
+  if (!L.narrowOk) return PDIPC_OK;
+  if (tile_off) std::copy(L.ntileOff.begin(), L.ntileOff.end(), tile_off);
+  if (tile_base) std::copy(L.ntileBase.begin(), L.ntileBase.end(), tile_base);
+  if (tile_col) std::copy(L.ntileCol.begin(), L.ntileCol.end(), tile_col);
+  if (loc) std::copy(L.loc8.begin(), L.loc8.end(), loc);
+  return PDIPC_OK;
+}
+
+int pdipc_sweep_staging(pdipc_handle* h, int* path) {
+  int rc = require_device(h);
+  if (rc) return rc;
+  if (!path) return fail(PDIPC_E_ARG, "null path");
+  *path = h->m.m2Loc8 ? 2 : (h->m.m2Loc ? 1 : 0);
   return PDIPC_OK;
 }
 

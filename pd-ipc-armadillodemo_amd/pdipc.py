@@ -86,6 +86,10 @@ SIGNATURES = {
     "pdipc_debug_lbvh": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_ulonglong), _I, _I, _I, _I]),
     "pdipc_debug_stage_layout": (ctypes.c_int, [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong), _I, _I,
                                                 ctypes.POINTER(ctypes.c_ushort), ctypes.POINTER(ctypes.c_ushort)]),
+    "pdipc_debug_narrow_stage_layout": (ctypes.c_int, [_H, ctypes.POINTER(ctypes.c_longlong), _I, _I,
+                                                       ctypes.POINTER(ctypes.c_ushort),
+                                                       ctypes.POINTER(ctypes.c_ubyte)]),
+    "pdipc_sweep_staging": (ctypes.c_int, [_H, _I]),
     "pdipc_debug_sell_layout": (ctypes.c_int, [_H, ctypes.c_int, ctypes.POINTER(ctypes.c_longlong),
                                                ctypes.POINTER(ctypes.c_longlong), _I, _I, _I,
                                                ctypes.POINTER(ctypes.c_short), _I]),
@@ -341,6 +345,32 @@ class Simulator:
                                               loc.ctypes.data_as(us)))
         return dict(tile_off=off, tile_base=base, tile_col=tcol[:nu], loc=loc[:ne], stage_max=umax,
                     tile_slices=ts)
+
+    def narrow_stage_layout(self):
+        """The FP32 sweep's narrow staging tables (8 lanes per row, one-byte
+        indices; host precompute only), or None when they do not apply."""
+        sz = np.zeros(6, np.int64)
+        szp = sz.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
+        _check(lib().pdipc_debug_narrow_stage_layout(self.h, szp, None, None, None, None))
+        ok, nt, nu, umax, ne, ts = (int(v) for v in sz)
+        if not ok:
+            return None
+        off = np.zeros(nt + 1, np.int32)
+        base = np.zeros(nt, np.int32)
+        tcol = np.zeros(max(nu, 1), np.uint16)
+        loc = np.zeros(max(ne, 1), np.uint8)
+        _check(lib().pdipc_debug_narrow_stage_layout(self.h, szp, _ip(off), _ip(base),
+                                                     tcol.ctypes.data_as(ctypes.POINTER(ctypes.c_ushort)),
+                                                     loc.ctypes.data_as(ctypes.POINTER(ctypes.c_ubyte))))
+        return dict(tile_off=off, tile_base=base, tile_col=tcol[:nu], loc=loc[:ne], stage_max=umax,
+                    tile_slices=ts)
+
+    def sweep_staging(self):
+        """How the FP32 sweeps read their operand: 0 L2 gathers, 1 staged in
+        LDS (16-bit indices), 2 narrow staging (one-byte indices)."""
+        v = ctypes.c_int()
+        _check(lib().pdipc_sweep_staging(self.h, ctypes.byref(v)))
+        return v.value
 
     def model_dat(self, model, which):
         n = ctypes.c_int()

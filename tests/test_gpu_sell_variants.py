@@ -93,14 +93,16 @@ def test_reordered_rows_tet_block(lanes):
             assert worst < 1e-5
 
 
-@pytest.mark.parametrize("which,lanes", [("cloth", 8), ("cloth", 4), ("cloth", 2), ("cloth96", 8), ("tets", 8),
-                                         ("tets", 4)])
-def test_staged_sweeps_are_bit_identical(which, lanes):
+@pytest.mark.parametrize("which,lanes,stage", [("cloth", 8, 1), ("cloth", 4, 1), ("cloth", 2, 1), ("cloth96", 8, 1),
+                                               ("cloth96", 8, 2), ("tets", 8, 1), ("tets", 4, 1)])
+def test_staged_sweeps_are_bit_identical(which, lanes, stage):
     """The sweeps that stage their operand in LDS (the default whenever the
     tiles fit) sum every row's entries in the same order as the L2-gather
     sweeps: whole steps agree bit for bit in both precisions, and the staged
     FP64 step meets the oracle's bar.  The 96 x 96 cloth has the uniform
-    width 8 (lane-major slices, vector loads); the others are chunk-major."""
+    width 8 (lane-major slices, vector loads; with sell_stage = 2 its FP32
+    sweeps take the narrow tiles with one-byte indices); the others are
+    chunk-major."""
     if which == "cloth":
         sc = scenes.cloth_over_sphere(24, 2, ground=True, seed=2, patch_dim=24, sphere_patch_dim=12, height=0.16)
     elif which == "cloth96":
@@ -115,8 +117,9 @@ def test_staged_sweeps_are_bit_identical(which, lanes):
     assert probe.stage_layout(lanes) is not None
     assert (probe.sell_layout(lanes)["uniform_w"] == 8) == (which == "cloth96")
     for prec in (pdipc.FP64, pdipc.FP32_SPLIT):
-        print(f"case: staged vs L2-gather sweeps, {which}, {lanes} lanes, precision {prec}", flush=True)
-        sims = [pdipc.Simulator.from_scene(sc, sell_lanes=lanes, precision=prec, sell_stage=st) for st in (1, 0)]
+        print(f"case: staged ({stage}) vs L2-gather sweeps, {which}, {lanes} lanes, precision {prec}", flush=True)
+        sims = [pdipc.Simulator.from_scene(sc, sell_lanes=lanes, precision=prec, sell_stage=st) for st in (stage, 0)]
+        assert sims[0].sweep_staging() == stage and sims[1].sweep_staging() == 0
         r = oracle.Oracle(sc)
         r.set_x(ref.x())
         r.set_v(ref.v())

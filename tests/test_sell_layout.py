@@ -125,6 +125,32 @@ def test_stage_tables_map_every_entry_to_its_column(which, lanes):
         assert np.array_equal(u[loc[e0:e1].astype(np.int64)], col[e0:e1])
 
 
+@pytest.mark.parametrize("which", ["cloth", "cloth96", "tets"])
+def test_narrow_stage_tables_map_every_entry_to_its_column(which):
+    """Narrow staging of the FP32 sweep (8 lanes per row, lane-major layouts
+    only): tiles of 2 slices whose unions hold at most 256 columns, sorted and
+    exact, every entry's one-byte index leading back to its column."""
+    sim = pdipc.Simulator.from_scene(_scene(which), init=False)
+    sim.host_setup()
+    L = sim.sell_layout(8)
+    T = sim.narrow_stage_layout()
+    if L["uniform_w"] != 8:
+        assert T is None  # chunk-major layouts keep the 16-bit tiles only
+        return
+    assert T is not None and T["tile_slices"] == 2
+    off, col = L["slice_off"], L["col"].astype(np.int64)
+    ns = len(L["slice_w"])
+    nt = -(-ns // 2)
+    toff, base, tcol, loc = T["tile_off"], T["tile_base"].astype(np.int64), T["tile_col"].astype(np.int64), T["loc"]
+    assert len(toff) == nt + 1 and toff[0] == 0 and len(loc) == len(col) and loc.dtype == np.uint8
+    assert np.diff(toff).max() == T["stage_max"] <= 256
+    for t in range(nt):
+        e0, e1 = off[t * 2], off[min(ns, (t + 1) * 2)]
+        u = base[t] + tcol[toff[t]:toff[t + 1]]
+        assert np.array_equal(u, np.unique(col[e0:e1]))
+        assert np.array_equal(u[loc[e0:e1].astype(np.int64)], col[e0:e1])
+
+
 def test_one_lane_per_row_is_not_staged():
     sim = pdipc.Simulator.from_scene(_scene("cloth96"), init=False)
     sim.host_setup()
