@@ -42,9 +42,17 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
 # a plain 1 GiB copy gives): profiles/k_sweep_config1_hbm_traffic.md.
 SWEEP32_TRAFFIC_BYTES = {(256, 6, 8): 31.63e6}
-# the same for the three stacked 1024^2 cloths at 8 lanes per row:
+# the same for the three stacked 1024^2 cloths at 8 lanes per row, by how the
+# sweep reads its operand (1: 16-bit staged indices, 2: narrow staging):
 # profiles/config4_stacked_kernel_stats.md
-STACKED_SWEEP32_TRAFFIC_BYTES = {8: 1518.6e6}
+STACKED_SWEEP32_TRAFFIC_BYTES = {(8, 1): 1518.6e6, (8, 2): 1370.8e6}
+# the FP32 sweep kernel by pdipc_sweep_staging's answer, and its bytes per
+# stored slot (FP32 value + union index or column)
+SWEEP32_KERNELS = {
+    0: ("k_sweep32 (FP32 Chebyshev sweep, SELL M^2, operand gathered from L2)", 6.0),
+    1: ("k_sweep32s (FP32 Chebyshev sweep, SELL M^2, operand staged in LDS)", 6.0),
+    2: ("k_sweep32n (FP32 Chebyshev sweep, SELL M^2, operand staged in LDS, one-byte union indices)", 5.0),
+}
 # steps of each scene's run from rest; by default the warm-up steps and the
 # timed steps together make up the whole run
 RUN_STEPS = {"cloth": 200, "config0": 100, "tet_block": 100, "stacked": 50}
@@ -146,6 +154,7 @@ def stacked_bench(args):
     log(f"in-step kernel means: {json.dumps(kt)}")
     sweep_bytes = 8 * c0["nnz_r2"] + 4 * 3 * c0["sim_points"] * 4
     sweep_us = kt["sweep_fp32"]["mean_us"]
+    staging = sim.sweep_staging()
     return {
         "metric": "PD-IPC steps/s, scale run (3 stacked 1024^2 cloths)",
         "value": args.steps / secs,
@@ -173,17 +182,17 @@ def stacked_bench(args):
             "kernel_means_us": kt,
         },
         "roofline": {
-            "kernel": "k_sweep32s (FP32 Chebyshev sweep, SELL M^2, operand staged in LDS)",
+            "kernel": SWEEP32_KERNELS[staging][0],
             "bound": "hbm",
             "achieved": sweep_bytes / (sweep_us * 1e-6) / 1e9,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": sweep_bytes / (sweep_us * 1e-6) / 1e9 / HBM_PEAK_GBS,
-            "traffic": STACKED_SWEEP32_TRAFFIC_BYTES.get(args.sell_lanes),
+            "traffic": STACKED_SWEEP32_TRAFFIC_BYTES.get((args.sell_lanes, staging)),
             "avg_us": sweep_us,
             "bytes_per_launch": sweep_bytes,
             "bytes_formula": "8 B per M^2 entry (FP32 value + int32 column) + 4 vectors x 3N x 4 B",
-            "mall_resident": c0["sell_entries"] * 6.0 < 256e6,
+            "mall_resident": c0["sell_entries"] * SWEEP32_KERNELS[staging][1] < 256e6,
         },
     }
 
@@ -558,8 +567,10 @@ def main():
                                                scene["params"]["cheb_iters"])
     sweep = kt["fp32_split"]["sweep_fp32"]
     ach = spmv_bytes / (sweep["mean_us"] * 1e-6) / 1e9
-    stored = c["nnz_r2"] * 6.0 + 3 * c["sim_points"] * 4.0 * 4
-    matrix_bytes = c["sell_entries"] * 6.0
+    staging = sims["fp32_split"].sweep_staging()
+    kname, slot_bytes = SWEEP32_KERNELS[staging]
+    stored = c["nnz_r2"] * slot_bytes + 3 * c["sim_points"] * 4.0 * 4
+    matrix_bytes = c["sell_entries"] * slot_bytes
     out = {
         "metric": METRIC,
         "value": steps / t32,
@@ -597,13 +608,14 @@ def main():
             "adjoint": adj,
         },
         "roofline": {
-            "kernel": "k_sweep32s (FP32 Chebyshev sweep, SELL M^2, operand staged in LDS)",
+            "kernel": kname,
             "bound": "hbm",
             "achieved": ach,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
-            "traffic": None if workload else SWEEP32_TRAFFIC_BYTES.get((args.n, args.subdiv, args.sell_lanes)),
+            "traffic": None if workload or staging != 1 else
+                       SWEEP32_TRAFFIC_BYTES.get((args.n, args.subdiv, args.sell_lanes)),
             "traffic_source": "profiles/k_sweep_config1_hbm_traffic.md: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
                               "per launch, scaled by a 1 GiB copy; a recorded constant, not measured in this run",
             "avg_us": sweep["mean_us"],
@@ -614,9 +626,13 @@ def main():
             "bytes_stored_format": stored,
             "frac_stored_format": stored / (sweep["mean_us"] * 1e-6) / 1e9 / HBM_PEAK_GBS,
             "mall_resident": matrix_bytes < 256e6,
-            "mall_note": "the M^2 matrix (FP32 values + 16-bit union indices, "
-                         f"{matrix_bytes / 1e6:.1f} MB) fits the 256 MB Infinity Cache: after the first sweep it is "
-                         "served from MALL, so this fraction of HBM peak is not evidence of HBM-bound execution",
+            "mall_note": (f"the M^2 matrix (FP32 values + {int(slot_bytes - 4) * 8}-bit indices, "
+                          f"{matrix_bytes / 1e6:.1f} MB) fits the 256 MB Infinity Cache: after the first sweep it is "
+                          "served from MALL, so this fraction of HBM peak is not evidence of HBM-bound execution")
+                         if matrix_bytes < 256e6 else
+                         (f"the M^2 matrix (FP32 values + {int(slot_bytes - 4) * 8}-bit indices, "
+                          f"{matrix_bytes / 1e6:.1f} MB) exceeds the 256 MB Infinity Cache: every sweep streams it "
+                          "from HBM"),
             "fp64_sweep": {"avg_us": kt["fp64"]["sweep_fp64"]["mean_us"],
                            "achieved": (8 * c["nnz_r2"] + 4 * 3 * c["sim_points"] * 8)
                            / (kt["fp64"]["sweep_fp64"]["mean_us"] * 1e-6) / 1e9},
