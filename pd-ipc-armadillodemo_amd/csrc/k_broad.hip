@@ -347,6 +347,30 @@ __global__ void k_scan_add(int n, int nb, const int* __restrict__ sums, int* __r
   if (blockIdx.x == 0 && threadIdx.x == 0 && total) *total = sums[nb];
 }
 
+// k_scan_add with the tile prefix summed in place: workgroup b adds up the
+// first b tile sums itself (at most kScanDirectMax of them), so a scan of up
+// to kScanDirectMax tiles is two launches instead of three.
+constexpr int kScanDirectMax = 1024;
+__global__ void k_scan_add_direct(int n, int nb, const int* __restrict__ blockSums, int* __restrict__ out,
+                                  int* __restrict__ total) {
+  __shared__ int sh[kBlock / kWave];
+  const int last = blockIdx.x == nb - 1 ? nb : (int)blockIdx.x;  // the last workgroup also sums its own: the total
+  int acc = 0;
+  for (int i = threadIdx.x; i < last; i += kBlock) acc += blockSums[i];
+  for (int o = kWave / 2; o > 0; o >>= 1) acc += __shfl_xor(acc, o, kWave);
+  if (threadIdx.x % kWave == 0) sh[threadIdx.x / kWave] = acc;
+  __syncthreads();
+  int sum = 0;
+#pragma unroll
+  for (int w = 0; w < kBlock / kWave; w++) sum += sh[w];
+  const int add = blockIdx.x == nb - 1 ? sum - blockSums[nb - 1] : sum;
+  const int base = blockIdx.x * kScanTile + threadIdx.x * kScanItems;
+#pragma unroll
+  for (int k = 0; k < kScanItems; k++)
+    if (base + k < n) out[base + k] += add;
+  if (blockIdx.x == nb - 1 && threadIdx.x == 0 && total) *total = sum;
+}
+
 // ---------------- Karras LBVH ----------------
 __device__ inline int cub64(uint64_t a, uint64_t b) { return __clzll((long long)(a ^ b)); }
 
@@ -916,6 +940,10 @@ void launch_exclusive_scan(int* in, int* out, int n, int* total_out, int* blockS
   }
   int nb = (n + kScanTile - 1) / kScanTile;
   hipLaunchKernelGGL(k_scan_tiles, dim3(nb), dim3(kBlock), 0, s, n, in, out, blockScratch);
+  if (nb <= kScanDirectMax) {
+    hipLaunchKernelGGL(k_scan_add_direct, dim3(nb), dim3(kBlock), 0, s, n, nb, blockScratch, out, total_out);
+    return;
+  }
   hipLaunchKernelGGL(k_scan_sums, dim3(1), dim3(kBlock), 0, s, nb, blockScratch);
   hipLaunchKernelGGL(k_scan_add, dim3(nb), dim3(kBlock), 0, s, n, nb, blockScratch, out, total_out);
 }
