@@ -252,7 +252,7 @@ def cpu_baseline(scene, x, v, seconds_budget=20.0, label="256x256 cloth"):
 
 
 def adjoint_bench(graphs, fd_dirs, log_every=8):
-    """The differentiable run: 128 x 128 cloth over the config-0 obstacle, 50 FP64
+    """The differentiable run: 128 x 128 cloth over the subdivision-4 icosphere, 50 FP64
     steps, loss |x_T - target|^2, gradient w.r.t. per-triangle stiffness and
     initial velocity by the library's adjoint, in its two modes: the rollout
     keeps the whole run's tape in HBM (checkpoint_every = 0; the backward runs
@@ -352,7 +352,7 @@ def adjoint_bench(graphs, fd_dirs, log_every=8):
     smooth = np.array([w is None for w in switches], bool)
     r, c = res["recorded"], res["checkpointed"]
     return {
-        "workload": "cloth 128x128 (32,258 triangles) over icosphere subdiv 4 (config-0 obstacle), 50 FP64 steps, "
+        "workload": "cloth 128x128 (32,258 triangles) over icosphere subdiv 4, 50 FP64 steps, "
                     "loss |x_T - target|^2, d/d(per-triangle stiffness, v0)",
         "mode": "whole-run tape kept in HBM (checkpoint_every = 0)",
         "forward_s": r["forward_s"], "backward_s": r["backward_s"], "adjoint_fwd_ratio": r["backward_fwd_ratio"],

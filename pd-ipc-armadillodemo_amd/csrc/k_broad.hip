@@ -40,7 +40,8 @@ __device__ inline double dmax(double a, double b) { return a > b ? a : b; }
 // kBoxLanes lanes per patch, each taking every kBoxLanes-th face, then a
 // min / max across the patch's lanes: exact and order-free, so the boxes are
 // those of one lane walking all the faces (one lane per patch left the
-// kernel at ~1.5 wavefronts per CU: 41 us for the 95 k patches of config 1).
+// kernel at ~1.5 wavefronts per CU: 41 us for the 95 k patches of a 256^2 cloth over a
+// subdivision-6 icosphere).
 constexpr int kBoxLanes = 8;
 __global__ void k_patch_boxes(int P, int ld, const int* __restrict__ pfOff, const int* __restrict__ pfList,
                               const int3* __restrict__ faces, const double* __restrict__ X,
@@ -749,12 +750,13 @@ __global__ void k_rec_count(int Q, const int2* __restrict__ pairs, const int* __
 }
 
 // Both record-fill kernels write the same records in the same slots: one lane
-// per record while the pairs are few (456 K pairs, 5.5 M records at config 1:
-// 567 us per broad phase against 580 with one lane per pair, too few lanes to
-// hide the pair's list loads), one lane per pair from kRecFillPairsMin pairs
-// (18 M pairs, 265 M records at the 3 x 1024^2 scale: 22.0 -> 19.2 ms per broad
-// phase, each pair's offsets and lists read once instead of per record; at
-// config 1 forced to one lane per pair, the steps were bit-identical).
+// per record while the pairs are few (456 K pairs, 5.5 M records for a 256^2
+// cloth over a subdivision-6 icosphere: 567 us per broad phase against 580
+// with one lane per pair, too few lanes to hide the pair's list loads), one
+// lane per pair from kRecFillPairsMin pairs (18 M pairs, 265 M records at the
+// 3 x 1024^2 scale: 22.0 -> 19.2 ms per broad phase, each pair's offsets and
+// lists read once instead of per record; the 256^2 scene forced to one lane
+// per pair gave bit-identical steps).
 constexpr int kRecFillPairsMin = 1 << 21;
 
 __device__ inline int last_le(const int* __restrict__ off, int lo, int hi, int r) {

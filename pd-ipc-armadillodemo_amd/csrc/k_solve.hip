@@ -365,13 +365,13 @@ __global__ void __launch_bounds__(kBlock) k_sweep32(int n, int nSlices, int ld, 
 // union index `loc`) instead of L2.  LDS holds the staged rows as (x, y) pairs
 // and z values (8 + 4 bytes in FP32): reads of whole 16-byte rows at the
 // entries' scattered union indices conflicted in the LDS banks more (one
-// sweep at config 1: 5.8 -> 4.9 us with the fixed-stride lists).  On a
+// sweep of a 256^2 cloth: 5.8 -> 4.9 us with the fixed-stride lists).  On a
 // mesh-ordered cloth a 32-row tile touches about 9 runs of ~40 columns (the
 // 4-ring stencil of M^2), so the L2 traffic of the gathers drops from 16 B
 // per entry to about 6 B per row.  Workgroup b runs tile b (the hardware deals
 // consecutive workgroups to the 8 XCDs round-robin; giving each XCD a
 // contiguous range of tiles, whose unions overlap, measured slower in the
-// step: 5.2-5.5 -> 5.6-5.9 us at config 1, 261 -> 279 us at the 3 x 1024^2
+// step: 5.2-5.5 -> 5.6-5.9 us for a 256^2 cloth, 261 -> 279 us at the 3 x 1024^2
 // scale).  Entries are summed in the same
 // order as the L2-gather kernels, so the results are bit-identical to theirs.
 // Entry offsets are 32-bit (staging requires fewer than 2^29 entries): each

@@ -225,7 +225,7 @@ def config2(seed=0, cells=40, cloth_n=128):
 
 def config3(seed=0, n_cloth=128, steps=50):
     """The differentiable run: a 128 x 128 cloth (32,258 triangles) over the
-    config-0 obstacle (icosphere subdivision 4), FP64, `steps` steps.  Returns
+    subdivision-4 icosphere obstacle, FP64, `steps` steps.  Returns
     (scene, v0, target): per-triangle strain stiffness log-uniform in
     [1e3, 1e5] (an edge's spring weight is the mean of its triangles), initial
     velocity N(0, 0.05^2) per component, and a seeded target drape (the rest
