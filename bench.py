@@ -40,8 +40,10 @@ METRIC = "PD-IPC steps/s, 256² cloth: FP32-split vs FP64 at 1e-5 err; adjoint/f
 HBM_PEAK_GBS = 8000.0
 # Bytes one FP32 sweep launch moves past L2 at the default workload, from two
 # rocprofv3 --pmc runs (FETCH_SIZE x 2.000 and WRITE_SIZE x 1.000, the factors
-# a plain 1 GiB copy gives): profiles/k_sweep_config1_hbm_traffic.md.
-SWEEP32_TRAFFIC_BYTES = {(256, 6, 8): 31.63e6}
+# a plain 1 GiB copy gives), by cloth size, sphere subdivision, lanes per row
+# and how the sweep reads its operand (1: 16-bit staged indices, 2: narrow
+# staging): profiles/k_sweep_config1_hbm_traffic.md.
+SWEEP32_TRAFFIC_BYTES = {(256, 6, 8, 1): 31.63e6, (256, 6, 8, 2): 28.55e6}
 # the same for the three stacked 1024^2 cloths at 8 lanes per row, by how the
 # sweep reads its operand (1: 16-bit staged indices, 2: narrow staging):
 # profiles/config4_stacked_kernel_stats.md
@@ -614,8 +616,8 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": ach / HBM_PEAK_GBS,
-            "traffic": None if workload or staging != 1 else
-                       SWEEP32_TRAFFIC_BYTES.get((args.n, args.subdiv, args.sell_lanes)),
+            "traffic": None if workload else
+                       SWEEP32_TRAFFIC_BYTES.get((args.n, args.subdiv, args.sell_lanes, staging)),
             "traffic_source": "profiles/k_sweep_config1_hbm_traffic.md: rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE "
                               "per launch, scaled by a 1 GiB copy; a recorded constant, not measured in this run",
             "avg_us": sweep["mean_us"],

@@ -65,9 +65,8 @@ typedef struct pdipc_params {
   int use_graphs;      /* 1: replay the local/global loop from a HIP graph */
   int sell_stage;      /* 1: the sweeps stage their gathered operand in LDS, per workgroup, when
                           every workgroup's column union fits (default; the FP32 sweep of a
-                          lane-major layout whose FP32 matrix exceeds 128 MB uses narrow tiles
-                          with one-byte indices); 2: the same, narrow tiles at any size;
-                          0: gather from L2 */
+                          lane-major layout uses narrow tiles with one-byte indices when they
+                          fit); 2: the same with 16-bit tiles only; 0: gather from L2 */
   int capacity_hint;   /* 0: default initial capacities of the growable broad-phase and contact
                           buffers (sized from the patch count); > 0: start each at this many
                           entries (they grow on demand either way) */
@@ -215,7 +214,7 @@ int pdipc_debug_narrow_stage_layout(pdipc_handle* h, long long* sizes, int* tile
                                     unsigned short* tile_col, unsigned char* loc);
 /* How this handle's FP32 sweeps read their operand: 0 gathered from L2,
  * 1 staged in LDS with 16-bit union indices, 2 narrow staging (one-byte
- * indices; taken when the FP32 matrix exceeds 128 MB, or with sell_stage = 2). */
+ * indices; taken by default wherever the narrow tiles fit). */
 int pdipc_sweep_staging(pdipc_handle* h, int* path);
 
 /* --- no counterpart in the reference: for callers that cannot allocate device
