@@ -64,9 +64,9 @@ typedef struct pdipc_params {
   int sell_lanes;      /* lanes per row in the Chebyshev sweep (1, 2, 4, 8); 0 = default */
   int use_graphs;      /* 1: replay the local/global loop from a HIP graph */
   int sell_stage;      /* 1: the sweeps stage their gathered operand in LDS, per workgroup, when
-                          every workgroup's column union fits (default; the FP32 sweep of a
-                          lane-major layout uses narrow tiles with one-byte indices when they
-                          fit); 2: the same with 16-bit tiles only; 0: gather from L2 */
+                          every workgroup's column union fits (default; the FP32 and FP64
+                          sweeps of a lane-major layout use narrow tiles with one-byte indices
+                          when they fit); 2: the same with 16-bit tiles only; 0: gather from L2 */
   int capacity_hint;   /* 0: default initial capacities of the growable broad-phase and contact
                           buffers (sized from the patch count); > 0: start each at this many
                           entries (they grow on demand either way) */
@@ -206,13 +206,14 @@ int pdipc_debug_sell_layout(pdipc_handle* h, int lanes, long long* sizes, long l
  * per SELL entry its index in its tile's union. */
 int pdipc_debug_stage_layout(pdipc_handle* h, int lanes, long long* sizes, int* tile_off, int* tile_base,
                              unsigned short* tile_col, unsigned short* loc);
-/* The narrow staging tables of the FP32 sweep (8 lanes per row, lane-major
+/* The narrow staging tables of the sweeps (8 lanes per row, lane-major
  * layouts only), as pdipc_debug_stage_layout: sizes[0] is 1 when every tile of
  * sizes[5] = 2 slices has a column union of at most 256 columns; loc receives
  * one byte per SELL entry. */
 int pdipc_debug_narrow_stage_layout(pdipc_handle* h, long long* sizes, int* tile_off, int* tile_base,
                                     unsigned short* tile_col, unsigned char* loc);
-/* How this handle's FP32 sweeps read their operand: 0 gathered from L2,
+/* How this handle's sweeps (FP32 and FP64, the first of a solve aside) read
+ * their operand: 0 gathered from L2,
  * 1 staged in LDS with 16-bit union indices, 2 narrow staging (one-byte
  * indices; taken by default wherever the narrow tiles fit). */
 int pdipc_sweep_staging(pdipc_handle* h, int* path);

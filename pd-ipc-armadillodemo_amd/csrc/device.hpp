@@ -144,9 +144,9 @@ struct DeviceMesh {
   int* tileBase = nullptr;
   uint16_t *tileCol = nullptr, *m2Loc = nullptr;
   int stageStride = 0;
-  // narrow staging of the FP32 sweep (lane-major layouts whose FP32 matrix
-  // streams from HBM): tiles of kNarrowTileSlices slices, lists at
-  // narrowStride, one-byte union indices per entry; null when not used
+  // narrow staging of the FP32 and FP64 sweeps (lane-major layouts at 8 lanes
+  // per row): tiles of kNarrowTileSlices slices, lists at narrowStride,
+  // one-byte union indices per entry; null when not used
   int* ntileBase = nullptr;
   uint16_t* ntileCol = nullptr;
   uint8_t* m2Loc8 = nullptr;

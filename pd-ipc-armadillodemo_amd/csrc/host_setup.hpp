@@ -143,8 +143,8 @@ struct SellLayout {
   std::vector<int> tileBase;       // nTiles: smallest column of the tile
   std::vector<uint16_t> tileCol;   // union columns - tileBase
   std::vector<uint16_t> loc;       // per SELL entry: index in its tile's union
-  // Narrow staging (lane-major layouts with 8 lanes per row; the FP32 sweep
-  // when the matrix streams from HBM): tiles of kNarrowTileSlices slices whose
+  // Narrow staging (lane-major layouts with 8 lanes per row; the FP32 and
+  // FP64 sweeps): tiles of kNarrowTileSlices slices whose
   // unions hold at most kNarrowMaxCols columns, so an entry's union index is
   // one byte.  narrowOk: every narrow tile fits.
   bool narrowOk = false;

@@ -828,6 +828,77 @@ __global__ void __launch_bounds__(kNarrowBlock) k_sweep32n(int n, int nSlices, i
   }
 }
 
+// The FP64 sweep with narrow staging (as k_sweep32n: a lane's 8 entries are
+// four 16-byte value loads and one 8-byte index load, 9 instead of 10 bytes
+// per slot); otherwise k_sweep64s at width 8, so bit-identical to it.
+__global__ void __launch_bounds__(kNarrowBlock) k_sweep64n(int n, int nSlices, int ld, double W,
+                                                           const uint8_t* __restrict__ loc8,
+                                                           const double* __restrict__ val, int stride,
+                                                           const int* __restrict__ tileBase,
+                                                           const uint16_t* __restrict__ tileCol,
+                                                           const Row64* __restrict__ cur,
+                                                           const Row64* __restrict__ pre,
+                                                           const Row64* __restrict__ cA, Row64* __restrict__ next,
+                                                           double* __restrict__ out) {
+  constexpr int G = 8, S = kWave / G, UW = kSellLaneMajorW;
+  __shared__ double2 sxy[kNarrowMaxCols];
+  __shared__ double sz[kNarrowMaxCols];
+  const int tile = blockIdx.x;
+  const int slice = tile * kNarrowTileSlices + (int)threadIdx.x / kWave, lane = threadIdx.x % kWave;
+  const bool live = slice < nSlices;  // every lane stages and meets the barrier
+  const int sl = live ? slice : nSlices - 1;
+  const int row = sl * S + lane / G;  // uniform layouts keep the row order
+  const uint32_t off = (uint32_t)sl * (uint32_t)(UW * kWave) + (uint32_t)(lane * UW);
+  double2 va[4];
+#pragma unroll
+  for (int k = 0; k < 4; k++) va[k] = ld32((const double2*)val, off / 2 + k);
+  const uint2 lw = ld32((const uint2*)loc8, off / 8);
+  {
+    const int base = tileBase[tile];
+    const uint32_t t0 = (uint32_t)tile * (uint32_t)stride;
+    int c[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      const int u = threadIdx.x + q * kNarrowBlock;
+      c[q] = base + (int)ld32(tileCol, t0 + (uint32_t)(u < stride ? u : stride - 1));
+    }
+    Row64 g[2];
+#pragma unroll
+    for (int q = 0; q < 2; q++) g[q] = ld32(cur, (uint32_t)c[q]);
+#pragma unroll
+    for (int q = 0; q < 2; q++) {
+      sxy[threadIdx.x + q * kNarrowBlock] = make_double2(g[q].x, g[q].y);
+      sz[threadIdx.x + q * kNarrowBlock] = g[q].z;
+    }
+  }
+  __syncthreads();
+  if (!live) return;
+  const double v[8] = {va[0].x, va[0].y, va[1].x, va[1].y, va[2].x, va[2].y, va[3].x, va[3].y};
+  const int li[8] = {(int)(lw.x & 0xff), (int)((lw.x >> 8) & 0xff), (int)((lw.x >> 16) & 0xff), (int)(lw.x >> 24),
+                     (int)(lw.y & 0xff), (int)((lw.y >> 8) & 0xff), (int)((lw.y >> 16) & 0xff), (int)(lw.y >> 24)};
+  double acc[3] = {0.0, 0.0, 0.0};
+#pragma unroll
+  for (int u = 0; u < 8; u++) {
+    const double2 a = sxy[li[u]];
+    const double z = sz[li[u]];
+    acc[0] += v[u] * a.x;
+    acc[1] += v[u] * a.y;
+    acc[2] += v[u] * z;
+  }
+  group_reduce<G>(acc);
+  if ((lane % G) != 0 || row >= n) return;
+  const Row64 p = ld32(pre, (uint32_t)row), c = ld32(cA, (uint32_t)row);
+  double x0 = W * (acc[0] + c.x - p.x) + p.x;
+  double x1 = W * (acc[1] + c.y - p.y) + p.y;
+  double x2 = W * (acc[2] + c.z - p.z) + p.z;
+  next[row] = Row64{x0, x1, x2, 0.0};
+  if (out) {
+    out[row] = x0;
+    out[ld + row] = x1;
+    out[2 * ld + row] = x2;
+  }
+}
+
 template <int G, int LAST>
 void launch_sweep32(const DeviceMesh& m, int grid, float W, const float4* cur, const float4* pre, float4* next,
                     hipStream_t s) {
@@ -894,8 +965,17 @@ void solve64(const DeviceMesh& m, double w, double rho, int iters, double* const
   for (int it = 1; it < iters; it++) {
     double* out = outs ? outs[it + 1] : (it == iters - 1 ? m.X : nullptr);
     bool staged = false;
+    if constexpr (G == 8) {
+      if (m.m2Loc8) {
+        launch_t(m.timer, TC_SWEEP64, k_sweep64n,
+                 dim3((m.nSlices + kNarrowTileSlices - 1) / kNarrowTileSlices), dim3(kNarrowBlock), 0, s, m.nSim,
+                 m.nSlices, m.ld, W, m.m2Loc8, m.m2Val, m.narrowStride, m.ntileBase, m.ntileCol, cur, pre, m.cA,
+                 next, out);
+        staged = true;
+      }
+    }
     if constexpr (G > 1) {
-      if (m.m2Loc) {
+      if (!staged && m.m2Loc) {
         launch_sweep64s<G>(m, W, cur, pre, next, out, s);
         staged = true;
       }

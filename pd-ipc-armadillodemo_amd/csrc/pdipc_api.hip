@@ -153,11 +153,12 @@ int upload_all(pdipc_handle* h) {
     }
     UP(tileBase, L.tileBase); UP(tileCol, padded); UP(m2Loc, L.loc);
     m.stageStride = stride;
-    // narrow staging for the FP32 sweep wherever its tiles fit (the one-byte
-    // indices save a sixth of the sweep's bytes; with the matrix in HBM
-    // 260 -> 233 us per sweep at 3 x 1024^2 cloths, and 1% more steps/s with
-    // it Infinity-Cache resident for a 256^2 cloth); sell_stage = 2 keeps the
-    // 16-bit tiles
+    // narrow staging for the FP32 and FP64 sweeps wherever its tiles fit (the
+    // one-byte indices save a sixth of an FP32 sweep's bytes, a tenth of an
+    // FP64 one's; FP32 with the matrix in HBM 260 -> 233 us per sweep at
+    // 3 x 1024^2 cloths; with it Infinity-Cache resident for a 256^2 cloth
+    // 1.7% more FP32-split and 4.8% more FP64 steps/s); sell_stage = 2 keeps
+    // the 16-bit tiles
     if (L.narrowOk && h->prm.sell_stage != 2) {
       const int nN = (int)L.ntileBase.size();
       const int nStride = (L.narrowMax + 7) / 8 * 8;

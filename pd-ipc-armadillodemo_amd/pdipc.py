@@ -347,7 +347,7 @@ class Simulator:
                     tile_slices=ts)
 
     def narrow_stage_layout(self):
-        """The FP32 sweep's narrow staging tables (8 lanes per row, one-byte
+        """The sweeps' narrow staging tables (8 lanes per row, one-byte
         indices; host precompute only), or None when they do not apply."""
         sz = np.zeros(6, np.int64)
         szp = sz.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong))
@@ -366,8 +366,8 @@ class Simulator:
                     tile_slices=ts)
 
     def sweep_staging(self):
-        """How the FP32 sweeps read their operand: 0 L2 gathers, 1 staged in
-        LDS (16-bit indices), 2 narrow staging (one-byte indices)."""
+        """How the sweeps read their operand: 0 L2 gathers, 1 staged in LDS
+        (16-bit indices), 2 narrow staging (one-byte indices)."""
         v = ctypes.c_int()
         _check(lib().pdipc_sweep_staging(self.h, ctypes.byref(v)))
         return v.value
