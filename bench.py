@@ -16,9 +16,10 @@ Measured, each precision from the same state:
   * the per-step error of FP32-split against FP64 from identical states;
   * in-step kernel durations (HIP events on every launch, over the timed
     window re-run with direct launches): the roofline of the FP32 sweep uses
-    the survey's algorithmic bytes (8 B per M^2 entry) over that mean.
+    the algorithmic bytes (8 B per M^2 entry: an FP32 value and a 32-bit
+    column, plus the row vectors) over that mean.
 
-Prints one JSON line.  --gpus above 1 exits: this build runs on one GPU.
+Prints one JSON line.  --gpus above 1 exits: the step runs on one GPU.
 """
 
 import argparse
@@ -88,8 +89,8 @@ def cloth_ring_slots(scene):
     return n
 
 
-def survey_step_bytes(c, ring_slots, outer_per_step, recs_per_step, p_vec, lg=10, cheb=15):
-    """Algorithmic HBM bytes of one step by the survey's formula (FP32 values,
+def algorithmic_step_bytes(c, ring_slots, outer_per_step, recs_per_step, p_vec, lg=10, cheb=15):
+    """Algorithmic HBM bytes of one step (FP32 values,
     int32 indices, p_vec bytes per vector element in the solve and local step,
     FP64 CCD and barrier): per outer iteration the operator update, lg x
     (local step + cheb sweeps), CCD and barrier over the step's candidate
@@ -428,7 +429,7 @@ def main():
                           "copy_probe_bytes_written_per_launch": b.numel() * 4}))
         return
     if args.gpus != 1:
-        print(f"bench.py: this build runs the PD-IPC step on one GPU; --gpus {args.gpus} is not supported",
+        print(f"bench.py: the PD-IPC step runs on one GPU; --gpus {args.gpus} is not supported",
               file=sys.stderr)
         sys.exit(2)
     if args.scene == "stacked":
@@ -565,7 +566,7 @@ def main():
     outer32 = res["fp32_split"]["outer"] / steps
     recs32 = float(np.mean(res["fp32_split"]["recs"]))
     slots = cloth_ring_slots(scene)
-    step_bytes, spmv_bytes = survey_step_bytes(c, slots, outer32, recs32, 4, scene["params"]["lg_iters"],
+    step_bytes, spmv_bytes = algorithmic_step_bytes(c, slots, outer32, recs32, 4, scene["params"]["lg_iters"],
                                                scene["params"]["cheb_iters"])
     sweep = kt["fp32_split"]["sweep_fp32"]
     ach = spmv_bytes / (sweep["mean_us"] * 1e-6) / 1e9
